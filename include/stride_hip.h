@@ -1,0 +1,202 @@
+/*
+ * stride_hip.h - C ABI of libstride_hip.so, the MI355X (gfx950) STRIDE codec.
+ *
+ * Plain C: pointers, sizes and int status codes; no C++ or torch types cross
+ * this boundary.  Every function returns STRIDE_OK (0) or a negative status;
+ * stride_last_error() copies the message of the last failure on the calling
+ * thread.  Device pointers are HIP device memory; `stream` is a hipStream_t
+ * (NULL = the default stream).  Handles are not reentrant: one thread at a
+ * time per handle; creating handles is thread safe.
+ *
+ * Reference interfaces replaced (file:line in maletsden/meshpress):
+ *   decoder  utils/paradelta_v5_cuda.py:580-718  ParaDeltaV5GpuDecoder
+ *            bench_cpp/stride_dup_decode_bench.cu:192-509  fused DPR4 kernel
+ *            scripts/dump_stride_dup_blob.py:29-152  side-table derivation
+ *   encoder  encoder/implementation/stride.py:18-53  STRIDEEncoder.encode
+ *            encoder/paradelta_codec.py:758-882  prepare_paradelta_arrays
+ *            encoder/paradelta_v5_dup.py:316-464  encode_dup
+ *   search   encoder/paradelta_v5_dup.py:159-171  _pick_best_k
+ *            encoder/_irlp_fit.py:120-226  _cost_seeds_njit / encoder_cost_batch
+ */
+#ifndef STRIDE_HIP_H
+#define STRIDE_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+  STRIDE_OK = 0,
+  STRIDE_E_ARG = -1,        /* invalid argument            -> ValueError   */
+  STRIDE_E_HIP = -2,        /* HIP runtime failure         -> RuntimeError */
+  STRIDE_E_MAGIC = -10,     /* bad magic                   -> ValueError   */
+  STRIDE_E_VERSION = -11,   /* unsupported version         -> ValueError   */
+  STRIDE_E_TRUNCATED = -12, /* stream shorter than fields  -> IndexError   */
+  STRIDE_E_CORRUPT = -13,   /* malformed meshlet           -> ValueError   */
+  STRIDE_E_UNSUPPORTED = -14,/* valid but outside kernel limits -> ValueError */
+  STRIDE_E_PREP = -20       /* encoder front-end failure   -> RuntimeError */
+};
+
+/* Copies the calling thread's last error message (NUL terminated, truncated
+ * to cap); returns its full length. */
+int stride_last_error(char* buf, size_t cap);
+
+/* Selects the device for later calls on this thread (hipSetDevice). */
+int stride_init(int device);
+
+/* Library ABI version. */
+int stride_version(void);
+
+/* Diagnostic: iteration order of a CPython set of small non-negative ints
+ * built by adding in[0..n) in order (the multiseed strip cover keeps the
+ * first strictly better seed in that order, utils/amd_gts_nb.py:160-175).
+ * Returns the number of distinct keys written to out. */
+int stride_debug_pyset_order(const int32_t* in, int32_t n, int32_t* out);
+
+/* ------------------------------------------------------------------------ */
+/* Decoder.  Replaces ParaDeltaV5GpuDecoder(data) (utils/paradelta_v5_cuda.py:597)
+ * for DPR4 v3 streams as written by encode_dup
+ * (encoder/paradelta_v5_dup.py:316-464).  Creation uploads the bytes, runs
+ * the index pre-pass kernel (the side table the reference bench reads from a
+ * DPRB blob, scripts/dump_stride_dup_blob.py:88-152) and validates every
+ * meshlet; decode is then one kernel launch. */
+typedef struct stride_decoder stride_decoder;
+
+int stride_decoder_create(const uint8_t* data, size_t len, stride_decoder** out);
+void stride_decoder_destroy(stride_decoder* h);
+/* n_v = sum of per-meshlet n_local (duplicated boundary vertices included),
+ * n_t = triangles, n_meshlets. */
+int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
+                         uint32_t* n_meshlets);
+/* Decoded world-space positions (float32 xyz, n_v x 3) and uint32 indices
+ * (n_t x 3) into caller-owned device buffers; asynchronous on `stream`. */
+int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
+/* Synchronous decode into caller-owned host buffers. */
+int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris);
+/* Bytes resident on the device: compressed stream, derived side table. */
+int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes,
+                         uint64_t* side_bytes);
+/* Global header fields: center[3], scale, per_coord_err, g_min[3], g_range[3]
+ * into f13; g_bits[3], pred_n[9], pred_K[3], code_width into i16. */
+int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
+
+/* Launch configuration (autotuned): waves per workgroup {1,2,4}, LDS stage
+ * words per meshlet {256,512,1024}, persistent grid (0 = one wave per
+ * meshlet), atomic work queue flag. */
+typedef struct stride_tune {
+  int32_t waves;
+  int32_t stage_words;
+  int32_t grid;
+  int32_t use_queue;
+} stride_tune;
+
+int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t);
+int stride_decoder_get_tune(const stride_decoder* h, stride_tune* t);
+
+/* ------------------------------------------------------------------------ */
+/* Batched decode: many DPR4 streams in one device buffer, decoded by one
+ * launch over a global meshlet table (no reference counterpart beyond the
+ * per-mesh loop of utils/paradelta_v5_cuda.py:686-712).  Output is the
+ * concatenation of every mesh's decode_dup result; indices stay relative
+ * to each mesh's own vertex array. */
+typedef struct stride_batch stride_batch;
+
+int stride_batch_create(const uint8_t* const* blobs, const size_t* lens, int32_t n_meshes,
+                        stride_batch** out);
+void stride_batch_destroy(stride_batch* h);
+int stride_batch_sizes(const stride_batch* h, uint64_t* n_v, uint64_t* n_t,
+                       uint32_t* n_meshlets);
+/* Per mesh first vertex / first triangle in the concatenated outputs,
+ * n_meshes + 1 entries each. */
+int stride_batch_offsets(const stride_batch* h, uint64_t* v_base, uint64_t* t_base);
+int stride_batch_bytes(const stride_batch* h, uint64_t* stream_bytes, uint64_t* side_bytes);
+int stride_batch_set_tune(stride_batch* h, const stride_tune* t);
+int stride_batch_get_tune(const stride_batch* h, stride_tune* t);
+/* One launch over all meshlets of all meshes. */
+int stride_batch_decode(stride_batch* h, float* d_verts, uint32_t* d_tris, void* stream);
+/* Same launch replayed from a captured hipGraph (captured on first use for
+ * this output pair and tune). */
+int stride_batch_decode_graph(stride_batch* h, float* d_verts, uint32_t* d_tris, void* stream);
+/* Baseline: one launch per mesh, in order. */
+int stride_batch_decode_per_mesh(stride_batch* h, float* d_verts, uint32_t* d_tris,
+                                 void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Encoder front-end (CPU).  Restates prepare_paradelta_arrays
+ * (encoder/paradelta_codec.py:758-882) from the partition on: global
+ * triangle adjacency, joint-learned meshlet growth
+ * (utils/meshlet_gen_joint_nb.py:66-430), boundary Morton order, per-meshlet
+ * plan (encoder/paradelta_codec.py:347-439), multiseed strips
+ * (utils/amd_gts_nb.py:150-175), and the walk (encoder/paradelta_v5_dup.py:114-152).
+ * Inputs: tris n_t x 3, normalised positions vn n_v x 3, face normals
+ * fn n_t x 3, global codes n_v x 3, cost constants k6/nm6, strip seed table
+ * (6 ints per row, row = triangle count). */
+typedef struct stride_encoder stride_encoder;
+
+int stride_encoder_create(const int64_t* tris, int64_t n_t, const double* vn, int64_t n_v,
+                          const double* fn, const int64_t* codes, int32_t max_verts,
+                          int32_t max_tris, const double* k6, const double* nm6,
+                          const int32_t* seed_table, int32_t seed_rows, int32_t n_threads,
+                          stride_encoder** out);
+void stride_encoder_destroy(stride_encoder* h);
+int stride_encoder_counts(const stride_encoder* h, int64_t* n_meshlets, int64_t* n_local_total,
+                          int64_t* n_tris_total, int64_t* n_strips_total,
+                          int64_t* n_para_total);
+/* Plans as flat arrays (meshlet-major); walk5 = (v, kind, a, b, c) bytes
+ * per local vertex in walk order. */
+int stride_encoder_export_plans(const stride_encoder* h, int32_t* n_local, int32_t* n_tris,
+                                int32_t* n_strips, int32_t* n_bnd, int64_t* l2g,
+                                uint8_t* tris_local, int32_t* strip_len, int32_t* strip_tris,
+                                uint8_t* walk5);
+/* IRLP fit samples (encoder/_irlp_fit.py:72-97): global codes of (a, b, c, v)
+ * for every parallelogram vertex, n_para x 3 each, and per-meshlet offsets
+ * (n_meshlets + 1). */
+int stride_encoder_para_samples(const stride_encoder* h, int32_t* A, int32_t* B, int32_t* C,
+                                int32_t* V, int64_t* off);
+/* Residuals for predictor (pred_n 3x3, pred_K 3) (paradelta_v5_dup.py:343-392).
+ * Returns the number of residual values and max |residual|. */
+int stride_encoder_residuals(stride_encoder* h, const int32_t* pred_n, const int32_t* pred_K,
+                             int64_t* n_values, int64_t* max_abs);
+/* Residual streams: 6 per meshlet (dx, dy, dz, px, py, pz), int32 values and
+ * 6 * n_meshlets + 1 offsets. */
+int stride_encoder_export_residuals(const stride_encoder* h, int32_t* vals,
+                                    int64_t* stream_off);
+/* DPR4 v3 bytes (paradelta_v5_dup.py:239-313,412-455).  f13 = center[3],
+ * scale, per_coord_err, g_min[3], g_range[3] (float32 values written
+ * verbatim); g_bits[3]; tagk = 12 bytes per meshlet ((tag, k) x 3 delta then
+ * x 3 para).  With out == NULL only *len is set. */
+int stride_encoder_write(const stride_encoder* h, const float* f13, const int32_t* g_bits,
+                         const int32_t* pred_n, const int32_t* pred_K, const uint8_t* tagk,
+                         uint8_t* out, size_t cap, size_t* len);
+
+/* ------------------------------------------------------------------------ */
+/* GPU search kernels (device pointers). */
+
+/* Exhaustive per-stream coder choice (_pick_best_k, paradelta_v5_dup.py:159-171):
+ * Rice k 0..11 and Exp-Golomb k 0..7 over the zigzag residuals of each of
+ * n_streams streams [off[s], off[s+1]); lowest k wins ties, Rice wins ties
+ * against Exp-Golomb, empty streams -> (1, 0).  Writes tag,k bytes
+ * (2 per stream) and the winning body bit count. */
+int stride_select_codes(const int32_t* d_vals, const int64_t* d_off, int64_t n_streams,
+                        uint8_t* d_tagk, int64_t* d_bits, void* stream);
+/* All 20 candidate body costs per stream (n_streams x 20 int64: Rice k
+ * 0..11 then Exp-Golomb k 0..7); the exhaustive labeller. */
+int stride_stream_costs(const int32_t* d_vals, const int64_t* d_off, int64_t n_streams,
+                        int64_t* d_costs, void* stream);
+/* IRLP candidate cost (_cost_seeds_njit, encoder/_irlp_fit.py:120-183): for
+ * each of S candidates (n0, n1, n2) with shift K, the sum over meshlets of
+ * min over the 20 coders of the parallelogram-residual bits of one axis.
+ * a, b, c, v: n samples; off: n_m + 1 meshlet offsets. */
+int stride_irlp_cost(const int32_t* d_a, const int32_t* d_b, const int32_t* d_c,
+                     const int32_t* d_v, const int64_t* d_off, int64_t n_m,
+                     const int32_t* d_cand, int32_t S, int32_t K, int64_t* d_cost,
+                     void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* STRIDE_HIP_H */

@@ -1,0 +1,9 @@
+"""MI355X-native STRIDE (DPR4 v3) mesh codec hot path.
+
+Surfaces (drop-ins for maletsden/meshpress):
+  meshpress_amd.encoder.STRIDEEncoder
+  meshpress_amd.utils.paradelta_v5_cuda.ParaDeltaV5GpuDecoder
+  meshpress_amd.utils.types.Model / CompressedModel
+  meshpress_amd.batch.BatchDecoder   (many streams, one persistent launch)
+"""
+__version__ = "0.1.0"

@@ -1,0 +1,728 @@
+// C ABI of libstride_hip.so: decoder / batch handles, encoder front-end
+// handle.  Host code; kernels live in stride_decode.hip / stride_search.hip.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <tuple>
+#include <vector>
+
+#include "../../include/stride_hip.h"
+#include "stride_decode.h"
+#include "stride_device.h"
+#include "stride_err.h"
+#include "stride_host.h"
+
+using namespace stride;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+}  // namespace
+
+int stride::set_error(int code, const std::string& msg) { return fail(code, msg); }
+
+namespace {
+
+#define HIP_TRY(expr)                                                                 \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return fail(STRIDE_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e));    \
+  } while (0)
+
+uint32_t be32(const uint8_t* p) {
+  return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+float bef32(const uint8_t* p) {
+  uint32_t u = be32(p);
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// DPR4 v3 global header (encoder/paradelta_v5_dup.py:412-446), 86 bytes,
+// every field byte aligned and big endian.
+struct DupHeader {
+  uint32_t magic, version, code_width;
+  float center[3], scale, pce, g_min[3], g_range[3];
+  int32_t g_bits[3], pred_n[9], pred_K[3];
+  uint32_t n_v, n_t, n_m;
+};
+
+int parse_header(const uint8_t* d, size_t len, DupHeader& h) {
+  if (len < 5) return fail(STRIDE_E_TRUNCATED, "stream shorter than the DPR4 header");
+  h.magic = be32(d);
+  if (h.magic != kMagicDup) {
+    char b[64];
+    std::snprintf(b, sizeof b, "bad magic 0x%08X", h.magic);
+    return fail(STRIDE_E_MAGIC, b);
+  }
+  h.version = d[4];
+  if (h.version != kVersionDup) {
+    char b[64];
+    std::snprintf(b, sizeof b, "expected v%u, got v%u", kVersionDup, h.version);
+    return fail(STRIDE_E_VERSION, b);
+  }
+  if (len < 86) return fail(STRIDE_E_TRUNCATED, "stream shorter than the DPR4 header");
+  h.code_width = d[5];
+  for (int i = 0; i < 3; ++i) h.center[i] = bef32(d + 6 + 4 * i);
+  h.scale = bef32(d + 18);
+  h.pce = bef32(d + 22);
+  for (int i = 0; i < 3; ++i) h.g_min[i] = bef32(d + 26 + 4 * i);
+  for (int i = 0; i < 3; ++i) h.g_range[i] = bef32(d + 38 + 4 * i);
+  for (int i = 0; i < 3; ++i) h.g_bits[i] = d[50 + i];
+  for (int i = 0; i < 9; ++i) {
+    uint32_t u = ((uint32_t)d[53 + 2 * i] << 8) | d[54 + 2 * i];
+    h.pred_n[i] = (u & 0x8000u) ? (int32_t)u - 0x10000 : (int32_t)u;
+  }
+  for (int i = 0; i < 3; ++i) h.pred_K[i] = d[71 + i];
+  h.n_v = be32(d + 74);
+  h.n_t = be32(d + 78);
+  h.n_m = be32(d + 82);
+  if ((uint64_t)len < 86ull + 4ull * h.n_m)
+    return fail(STRIDE_E_TRUNCATED, "stream shorter than its meshlet offset table");
+  for (int i = 0; i < 3; ++i)
+    if (h.g_bits[i] < 1 || h.g_bits[i] > 30)
+      return fail(STRIDE_E_UNSUPPORTED, "g_bits outside [1, 30] is not supported by the GPU decoder");
+  for (int i = 0; i < 3; ++i)
+    if (h.pred_K[i] > 30)
+      return fail(STRIDE_E_UNSUPPORTED, "predictor shift K > 30 is not supported");
+  return STRIDE_OK;
+}
+
+const char* scan_err_text(uint32_t e) {
+  switch (e) {
+    case kScanErrSize: return "meshlet counts outside the supported range (n_local, n_tris <= 256)";
+    case kScanErrOverrun: return "meshlet fields run past the meshlet's bit range";
+    case kScanErrFifo: return "reuse FIFO index past the FIFO fill";
+    case kScanErrIndex: return "local vertex id >= n_local";
+    case kScanErrStrips: return "strip lengths do not add up to n_tris";
+    case kScanErrWalk: return "strip walk does not cover every local vertex";
+    case kScanErrTag: return "residual coder parameter k > 24 is not supported";
+    default: return "malformed meshlet";
+  }
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, uint32_t v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
+}
+
+}  // namespace
+
+// One device-resident set of DPR4 streams with its side table.
+struct DecodeSet {
+  int device = 0;
+  std::vector<DupHeader> hdr;
+  std::vector<uint64_t> v_base, t_base;   // n_meshes + 1
+  std::vector<uint32_t> m_base;           // first meshlet of each mesh, n_meshes + 1
+  uint8_t* d_buf = nullptr;
+  size_t buf_bytes = 0, stream_bytes = 0;
+  MeshletEntry* d_tab = nullptr;
+  MeshParams* d_mesh = nullptr;
+  uint32_t* d_queue = nullptr;
+  uint32_t n_m = 0;
+  uint64_t n_v = 0, n_t = 0;
+  float* d_v_own = nullptr;
+  uint32_t* d_t_own = nullptr;
+  DecodeConfig cfg;
+  // captured graphs keyed by (verts, tris, tune)
+  std::map<std::tuple<void*, void*, int, int, int, int>, hipGraphExec_t> graphs;
+  hipStream_t cap_stream = nullptr;
+
+  ~DecodeSet() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    if (cap_stream) (void)hipStreamDestroy(cap_stream);
+    (void)hipFree(d_buf);
+    (void)hipFree(d_tab);
+    (void)hipFree(d_mesh);
+    (void)hipFree(d_queue);
+    (void)hipFree(d_v_own);
+    (void)hipFree(d_t_own);
+  }
+
+  int build(const uint8_t* const* blobs, const size_t* lens, int n_meshes) {
+    if (n_meshes < 1) return fail(STRIDE_E_ARG, "need at least one stream");
+    hdr.resize(n_meshes);
+    std::vector<uint64_t> blob_byte(n_meshes);
+    uint64_t total = 0;
+    uint64_t nm_total = 0;
+    for (int i = 0; i < n_meshes; ++i) {
+      if (!blobs[i] && lens[i]) return fail(STRIDE_E_ARG, "null stream pointer");
+      int rc = parse_header(blobs[i], lens[i], hdr[i]);
+      if (rc) return rc;
+      blob_byte[i] = total;
+      total += (lens[i] + 15) & ~(size_t)15;
+      nm_total += hdr[i].n_m;
+      stream_bytes += lens[i];
+    }
+    if (nm_total > 0xFFFFFFFFull) return fail(STRIDE_E_UNSUPPORTED, "too many meshlets");
+    n_m = (uint32_t)nm_total;
+    buf_bytes = total + 64;
+    std::vector<uint8_t> hbuf(buf_bytes, 0);
+    std::vector<MeshletEntry> seeds(n_m);
+    std::vector<MeshParams> mp(n_meshes);
+    m_base.assign(n_meshes + 1, 0);
+    uint32_t m = 0;
+    for (int i = 0; i < n_meshes; ++i) {
+      std::memcpy(hbuf.data() + blob_byte[i], blobs[i], lens[i]);
+      const DupHeader& h = hdr[i];
+      MeshParams& p = mp[i];
+      std::memset(&p, 0, sizeof p);
+      p.blob_byte = blob_byte[i];
+      for (int d = 0; d < 3; ++d) {
+        p.g_min[d] = h.g_min[d];
+        p.g_range[d] = h.g_range[d];
+        p.center[d] = h.center[d];
+        p.g_bits[d] = h.g_bits[d];
+        p.pred_K[d] = h.pred_K[d];
+      }
+      for (int j = 0; j < 9; ++j) p.pred_n[j] = h.pred_n[j];
+      p.scale = h.scale;
+      m_base[i] = m;
+      const uint64_t total_bits = 8ull * lens[i];
+      const uint64_t region = 8ull * (86ull + 4ull * h.n_m);
+      if (total_bits > 0xFFFFFFF0ull) return fail(STRIDE_E_UNSUPPORTED, "stream larger than 512 MiB");
+      for (uint32_t j = 0; j < h.n_m; ++j) {
+        uint32_t off = be32(blobs[i] + 86 + 4 * j);
+        uint64_t nxt = j + 1 < h.n_m ? be32(blobs[i] + 86 + 4 * (j + 1)) : total_bits;
+        if ((j == 0 && off != region) || off >= nxt || nxt > total_bits)
+          return fail(STRIDE_E_CORRUPT, "meshlet offset table is not consistent with the stream");
+        MeshletEntry& e = seeds[m + j];
+        std::memset(&e, 0, sizeof e);
+        e.mesh = (uint32_t)i;
+        e.start_bit = off;
+        e.end_bit = (uint32_t)nxt;
+      }
+      m += h.n_m;
+    }
+    m_base[n_meshes] = m;
+    // Host-side validation is complete; from here on the device is used.
+    HIP_TRY(hipGetDevice(&device));
+    HIP_TRY(hipMalloc(&d_buf, buf_bytes));
+    HIP_TRY(hipMemcpy(d_buf, hbuf.data(), buf_bytes, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&d_tab, sizeof(MeshletEntry) * std::max<uint32_t>(n_m, 1)));
+    HIP_TRY(hipMalloc(&d_mesh, sizeof(MeshParams) * n_meshes));
+    HIP_TRY(hipMalloc(&d_queue, 256));
+    HIP_TRY(hipMemcpy(d_mesh, mp.data(), sizeof(MeshParams) * n_meshes, hipMemcpyHostToDevice));
+    std::vector<MeshletScan> scan(n_m);
+    if (n_m) {
+      MeshletScan* d_scan = nullptr;
+      HIP_TRY(hipMemcpy(d_tab, seeds.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
+      HIP_TRY(hipMalloc(&d_scan, sizeof(MeshletScan) * n_m));
+      hipError_t e1 = launch_scan(d_buf, d_tab, d_mesh, n_m, d_scan, nullptr);
+      hipError_t e2 = e1 == hipSuccess ? hipDeviceSynchronize() : e1;
+      hipError_t e3 = e2 == hipSuccess
+                          ? hipMemcpy(scan.data(), d_scan, sizeof(MeshletScan) * n_m,
+                                      hipMemcpyDeviceToHost)
+                          : e2;
+      (void)hipFree(d_scan);
+      HIP_TRY(e3);
+    }
+    // Side table: prefix sums, validation of meshlet contiguity.
+    v_base.assign(n_meshes + 1, 0);
+    t_base.assign(n_meshes + 1, 0);
+    uint64_t vt = 0, tt = 0;
+    for (int i = 0; i < n_meshes; ++i) {
+      uint64_t vo = 0, to = 0;
+      v_base[i] = vt;
+      t_base[i] = tt;
+      mp[i].v_base = vt;
+      mp[i].t_base = tt;
+      for (uint32_t j = m_base[i]; j < m_base[i + 1]; ++j) {
+        const MeshletScan& s = scan[j];
+        if (s.error) {
+          char b[160];
+          std::snprintf(b, sizeof b, "meshlet %u of stream %d: %s", j - m_base[i], i,
+                        scan_err_text(s.error));
+          return fail(STRIDE_E_CORRUPT, b);
+        }
+        MeshletEntry& e = seeds[j];
+        if (j + 1 < m_base[i + 1] && s.end_bit != e.end_bit)
+          return fail(STRIDE_E_CORRUPT,
+                      "meshlet sizes disagree with the meshlet offset table");
+        if (vo > 0xFFFFFFFFull - 256 || to > 0xFFFFFFFFull - 256)
+          return fail(STRIDE_E_UNSUPPORTED, "mesh too large for 32-bit indices");
+        e.v_off = (uint32_t)vo;
+        e.t_off = (uint32_t)to;
+        e.n_local = s.n_local;
+        e.n_tris = s.n_tris;
+        e.n_strips = s.n_strips;
+        e.n_kind0 = s.n_kind0;
+        e.resid_bit = s.resid_bit;
+        for (int k = 0; k < 6; ++k) e.sub_bit[k] = s.sub_bit[k];
+        e.end_bit = s.end_bit;
+        vo += s.n_local;
+        to += s.n_tris;
+      }
+      vt += vo;
+      tt += to;
+    }
+    v_base[n_meshes] = vt;
+    t_base[n_meshes] = tt;
+    n_v = vt;
+    n_t = tt;
+    HIP_TRY(hipMemcpy(d_mesh, mp.data(), sizeof(MeshParams) * n_meshes, hipMemcpyHostToDevice));
+    if (n_m)
+      HIP_TRY(hipMemcpy(d_tab, seeds.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
+    return STRIDE_OK;
+  }
+
+  DecodeLaunch launch_args(float* V, uint32_t* T, uint32_t m0, uint32_t m1) const {
+    DecodeLaunch d;
+    d.buf = d_buf;
+    d.tab = d_tab + m0;
+    d.meshes = d_mesh;
+    d.n_m = m1 - m0;
+    d.V = V;
+    d.T = T;
+    d.queue = d_queue;
+    d.use_queue = cfg.use_queue;
+    d.grid = (uint32_t)cfg.grid;
+    return d;
+  }
+
+  int enqueue(float* V, uint32_t* T, hipStream_t s, uint32_t m0, uint32_t m1) {
+    if (cfg.use_queue) {
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_queue, 0u);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(launch_decode(launch_args(V, T, m0, m1), cfg, s));
+    return STRIDE_OK;
+  }
+
+  int decode(float* V, uint32_t* T, hipStream_t s) {
+    if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
+    return enqueue(V, T, s, 0, n_m);
+  }
+
+  int decode_per_mesh(float* V, uint32_t* T, hipStream_t s) {
+    if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
+    for (size_t i = 0; i + 1 < m_base.size(); ++i) {
+      if (m_base[i + 1] == m_base[i]) continue;
+      int rc = enqueue(V, T, s, m_base[i], m_base[i + 1]);
+      if (rc) return rc;
+    }
+    return STRIDE_OK;
+  }
+
+  int decode_graph(float* V, uint32_t* T, hipStream_t s) {
+    if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
+    auto key = std::make_tuple((void*)V, (void*)T, cfg.waves, cfg.stage_words, cfg.grid,
+                               cfg.use_queue);
+    auto it = graphs.find(key);
+    if (it == graphs.end()) {
+      if (!cap_stream) HIP_TRY(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
+      hipGraph_t g = nullptr;
+      HIP_TRY(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
+      int rc = enqueue(V, T, cap_stream, 0, n_m);
+      hipError_t ec = hipStreamEndCapture(cap_stream, &g);
+      if (rc) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+      }
+      HIP_TRY(ec);
+      hipGraphExec_t ge = nullptr;
+      hipError_t ei = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      HIP_TRY(ei);
+      it = graphs.emplace(key, ge).first;
+    }
+    HIP_TRY(hipGraphLaunch(it->second, s));
+    return STRIDE_OK;
+  }
+
+  int set_tune(const stride_tune* t) {
+    if (!t) return fail(STRIDE_E_ARG, "null tune");
+    bool ok_w = t->waves == 1 || t->waves == 2 || t->waves == 4;
+    bool ok_s = t->stage_words == 256 || t->stage_words == 512 || t->stage_words == 1024;
+    if (!ok_w || !ok_s || t->grid < 0 || (t->use_queue && t->grid == 0))
+      return fail(STRIDE_E_ARG,
+                  "tune: waves in {1,2,4}, stage_words in {256,512,1024}, grid >= 0, "
+                  "use_queue needs a persistent grid");
+    cfg.waves = t->waves;
+    cfg.stage_words = t->stage_words;
+    cfg.grid = t->grid;
+    cfg.use_queue = t->use_queue;
+    return STRIDE_OK;
+  }
+  void get_tune(stride_tune* t) const {
+    t->waves = cfg.waves;
+    t->stage_words = cfg.stage_words;
+    t->grid = cfg.grid;
+    t->use_queue = cfg.use_queue;
+  }
+};
+
+struct stride_decoder {
+  DecodeSet set;
+};
+struct stride_batch {
+  DecodeSet set;
+};
+
+struct stride_encoder {
+  Prep prep;
+  std::vector<int64_t> codes;
+  Residuals res;
+  bool have_res = false;
+  int32_t pred_n[9];
+  int32_t pred_K[3];
+  int n_threads = 1;
+};
+
+extern "C" {
+
+int stride_last_error(char* buf, size_t cap) {
+  if (buf && cap) {
+    size_t n = std::min(cap - 1, g_err.size());
+    std::memcpy(buf, g_err.data(), n);
+    buf[n] = 0;
+  }
+  return (int)g_err.size();
+}
+
+int stride_init(int device) {
+  HIP_TRY(hipSetDevice(device));
+  return STRIDE_OK;
+}
+
+int stride_version(void) { return 1; }
+
+int stride_debug_pyset_order(const int32_t* in, int32_t n, int32_t* out) {
+  if (!in || !out || n < 0) return fail(STRIDE_E_ARG, "null argument");
+  std::vector<int> v(in, in + n);
+  for (int x : v)
+    if (x < 0) return fail(STRIDE_E_ARG, "negative key");
+  std::vector<int> o = pyset_order(v);
+  for (size_t i = 0; i < o.size(); ++i) out[i] = o[i];
+  return (int)o.size();
+}
+
+// ----------------------------------------------------------------- decoder
+int stride_decoder_create(const uint8_t* data, size_t len, stride_decoder** out) {
+  if (!out) return fail(STRIDE_E_ARG, "null out");
+  *out = nullptr;
+  if (!data && len) return fail(STRIDE_E_ARG, "null data");
+  auto h = std::make_unique<stride_decoder>();
+  const uint8_t* blobs[1] = {data};
+  size_t lens[1] = {len};
+  int rc = h->set.build(blobs, lens, 1);
+  if (rc) return rc;
+  *out = h.release();
+  return STRIDE_OK;
+}
+
+void stride_decoder_destroy(stride_decoder* h) { delete h; }
+
+int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
+                         uint32_t* n_meshlets) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (n_v) *n_v = h->set.n_v;
+  if (n_t) *n_t = h->set.n_t;
+  if (n_meshlets) *n_meshlets = h->set.n_m;
+  return STRIDE_OK;
+}
+
+int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  return h->set.decode(d_verts, d_tris, (hipStream_t)stream);
+}
+
+int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris) {
+  if (!h || (!verts && h->set.n_v) || (!tris && h->set.n_t))
+    return fail(STRIDE_E_ARG, "null argument");
+  DecodeSet& s = h->set;
+  if (!s.d_v_own) HIP_TRY(hipMalloc(&s.d_v_own, 12 * std::max<uint64_t>(s.n_v, 1)));
+  if (!s.d_t_own) HIP_TRY(hipMalloc(&s.d_t_own, 12 * std::max<uint64_t>(s.n_t, 1)));
+  int rc = s.decode(s.d_v_own, s.d_t_own, nullptr);
+  if (rc) return rc;
+  HIP_TRY(hipMemcpy(verts, s.d_v_own, 12 * s.n_v, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(tris, s.d_t_own, 12 * s.n_t, hipMemcpyDeviceToHost));
+  return STRIDE_OK;
+}
+
+int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes, uint64_t* side_bytes) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (stream_bytes) *stream_bytes = h->set.stream_bytes;
+  if (side_bytes) *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) + sizeof(MeshParams);
+  return STRIDE_OK;
+}
+
+int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  const DupHeader& d = h->set.hdr[0];
+  if (f13) {
+    for (int i = 0; i < 3; ++i) f13[i] = d.center[i];
+    f13[3] = d.scale;
+    f13[4] = d.pce;
+    for (int i = 0; i < 3; ++i) f13[5 + i] = d.g_min[i];
+    for (int i = 0; i < 3; ++i) f13[8 + i] = d.g_range[i];
+    f13[11] = f13[12] = 0.f;
+  }
+  if (i16) {
+    for (int i = 0; i < 3; ++i) i16[i] = d.g_bits[i];
+    for (int i = 0; i < 9; ++i) i16[3 + i] = d.pred_n[i];
+    for (int i = 0; i < 3; ++i) i16[12 + i] = d.pred_K[i];
+    i16[15] = (int32_t)d.code_width;
+  }
+  return STRIDE_OK;
+}
+
+int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  return h->set.set_tune(t);
+}
+int stride_decoder_get_tune(const stride_decoder* h, stride_tune* t) {
+  if (!h || !t) return fail(STRIDE_E_ARG, "null argument");
+  h->set.get_tune(t);
+  return STRIDE_OK;
+}
+
+// ------------------------------------------------------------------- batch
+int stride_batch_create(const uint8_t* const* blobs, const size_t* lens, int32_t n_meshes,
+                        stride_batch** out) {
+  if (!out || !blobs || !lens) return fail(STRIDE_E_ARG, "null argument");
+  *out = nullptr;
+  auto h = std::make_unique<stride_batch>();
+  int rc = h->set.build(blobs, lens, n_meshes);
+  if (rc) return rc;
+  *out = h.release();
+  return STRIDE_OK;
+}
+void stride_batch_destroy(stride_batch* h) { delete h; }
+int stride_batch_sizes(const stride_batch* h, uint64_t* n_v, uint64_t* n_t, uint32_t* n_meshlets) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (n_v) *n_v = h->set.n_v;
+  if (n_t) *n_t = h->set.n_t;
+  if (n_meshlets) *n_meshlets = h->set.n_m;
+  return STRIDE_OK;
+}
+int stride_batch_offsets(const stride_batch* h, uint64_t* v_base, uint64_t* t_base) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (v_base) std::copy(h->set.v_base.begin(), h->set.v_base.end(), v_base);
+  if (t_base) std::copy(h->set.t_base.begin(), h->set.t_base.end(), t_base);
+  return STRIDE_OK;
+}
+int stride_batch_bytes(const stride_batch* h, uint64_t* stream_bytes, uint64_t* side_bytes) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (stream_bytes) *stream_bytes = h->set.stream_bytes;
+  if (side_bytes)
+    *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) +
+                  (uint64_t)h->set.hdr.size() * sizeof(MeshParams);
+  return STRIDE_OK;
+}
+int stride_batch_set_tune(stride_batch* h, const stride_tune* t) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  return h->set.set_tune(t);
+}
+int stride_batch_get_tune(const stride_batch* h, stride_tune* t) {
+  if (!h || !t) return fail(STRIDE_E_ARG, "null argument");
+  h->set.get_tune(t);
+  return STRIDE_OK;
+}
+int stride_batch_decode(stride_batch* h, float* d_verts, uint32_t* d_tris, void* stream) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  return h->set.decode(d_verts, d_tris, (hipStream_t)stream);
+}
+int stride_batch_decode_graph(stride_batch* h, float* d_verts, uint32_t* d_tris, void* stream) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  return h->set.decode_graph(d_verts, d_tris, (hipStream_t)stream);
+}
+int stride_batch_decode_per_mesh(stride_batch* h, float* d_verts, uint32_t* d_tris,
+                                 void* stream) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  return h->set.decode_per_mesh(d_verts, d_tris, (hipStream_t)stream);
+}
+
+// ----------------------------------------------------------------- encoder
+int stride_encoder_create(const int64_t* tris, int64_t n_t, const double* vn, int64_t n_v,
+                          const double* fn, const int64_t* codes, int32_t max_verts,
+                          int32_t max_tris, const double* k6, const double* nm6,
+                          const int32_t* seed_table, int32_t seed_rows, int32_t n_threads,
+                          stride_encoder** out) {
+  if (!out || !tris || !vn || !fn || !codes || !k6 || !nm6 || !seed_table)
+    return fail(STRIDE_E_ARG, "null argument");
+  *out = nullptr;
+  if (n_t <= 0 || n_v <= 0) return fail(STRIDE_E_ARG, "empty mesh");
+  PrepParams P;
+  P.max_verts = max_verts;
+  P.max_tris = max_tris;
+  for (int i = 0; i < 6; ++i) {
+    P.K[i] = k6[i];
+    P.NM[i] = nm6[i];
+  }
+  P.seed_table = seed_table;
+  P.seed_rows = seed_rows;
+  P.n_threads = std::max(1, (int)n_threads);
+  auto h = std::make_unique<stride_encoder>();
+  h->n_threads = P.n_threads;
+  try {
+    if (!build_prep(tris, n_t, vn, n_v, fn, codes, P, h->prep))
+      return fail(STRIDE_E_PREP, h->prep.error);
+  } catch (const std::exception& e) {
+    return fail(STRIDE_E_PREP, e.what());
+  }
+  h->codes.assign(codes, codes + 3 * n_v);
+  *out = h.release();
+  return STRIDE_OK;
+}
+
+void stride_encoder_destroy(stride_encoder* h) { delete h; }
+
+int stride_encoder_counts(const stride_encoder* h, int64_t* n_meshlets, int64_t* n_local_total,
+                          int64_t* n_tris_total, int64_t* n_strips_total,
+                          int64_t* n_para_total) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  int64_t nl = 0, nt = 0, ns = 0, np = 0;
+  for (size_t m = 0; m < h->prep.plans.size(); ++m) {
+    const Plan& p = h->prep.plans[m];
+    nl += (int64_t)p.l2g.size();
+    nt += (int64_t)p.tris_local.size() / 3;
+    ns += (int64_t)p.strip_len.size();
+    for (const auto& w : h->prep.walks[m]) np += w.kind;
+  }
+  if (n_meshlets) *n_meshlets = (int64_t)h->prep.plans.size();
+  if (n_local_total) *n_local_total = nl;
+  if (n_tris_total) *n_tris_total = nt;
+  if (n_strips_total) *n_strips_total = ns;
+  if (n_para_total) *n_para_total = np;
+  return STRIDE_OK;
+}
+
+int stride_encoder_export_plans(const stride_encoder* h, int32_t* n_local, int32_t* n_tris,
+                                int32_t* n_strips, int32_t* n_bnd, int64_t* l2g,
+                                uint8_t* tris_local, int32_t* strip_len, int32_t* strip_tris,
+                                uint8_t* walk5) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  size_t pl = 0, pt = 0, ps = 0, pst = 0, pw = 0;
+  for (size_t m = 0; m < h->prep.plans.size(); ++m) {
+    const Plan& p = h->prep.plans[m];
+    const int nl = (int)p.l2g.size(), nt = (int)p.tris_local.size() / 3;
+    if (n_local) n_local[m] = nl;
+    if (n_tris) n_tris[m] = nt;
+    if (n_strips) n_strips[m] = (int32_t)p.strip_len.size();
+    if (n_bnd) n_bnd[m] = p.n_bnd;
+    if (l2g) std::copy(p.l2g.begin(), p.l2g.end(), l2g + pl);
+    if (tris_local) std::copy(p.tris_local.begin(), p.tris_local.end(), tris_local + 3 * pt);
+    if (strip_len) std::copy(p.strip_len.begin(), p.strip_len.end(), strip_len + ps);
+    if (strip_tris) std::copy(p.strip_tris.begin(), p.strip_tris.end(), strip_tris + pst);
+    if (walk5)
+      for (const auto& w : h->prep.walks[m]) {
+        uint8_t* o = walk5 + 5 * pw++;
+        o[0] = w.v; o[1] = w.kind; o[2] = w.a; o[3] = w.b; o[4] = w.c;
+      }
+    pl += nl;
+    pt += nt;
+    ps += p.strip_len.size();
+    pst += p.strip_tris.size();
+  }
+  return STRIDE_OK;
+}
+
+int stride_encoder_para_samples(const stride_encoder* h, int32_t* A, int32_t* B, int32_t* C,
+                                int32_t* V, int64_t* off) {
+  if (!h || !A || !B || !C || !V || !off) return fail(STRIDE_E_ARG, "null argument");
+  int64_t r = 0;
+  off[0] = 0;
+  const int64_t* cd = h->codes.data();
+  for (size_t m = 0; m < h->prep.plans.size(); ++m) {
+    const Plan& p = h->prep.plans[m];
+    for (const auto& w : h->prep.walks[m]) {
+      if (!w.kind) continue;
+      for (int d = 0; d < 3; ++d) {
+        A[3 * r + d] = (int32_t)cd[3 * p.l2g[w.a] + d];
+        B[3 * r + d] = (int32_t)cd[3 * p.l2g[w.b] + d];
+        C[3 * r + d] = (int32_t)cd[3 * p.l2g[w.c] + d];
+        V[3 * r + d] = (int32_t)cd[3 * p.l2g[w.v] + d];
+      }
+      r++;
+    }
+    off[m + 1] = r;
+  }
+  return STRIDE_OK;
+}
+
+int stride_encoder_residuals(stride_encoder* h, const int32_t* pred_n, const int32_t* pred_K,
+                             int64_t* n_values, int64_t* max_abs) {
+  if (!h || !pred_n || !pred_K) return fail(STRIDE_E_ARG, "null argument");
+  for (int d = 0; d < 3; ++d)
+    if (pred_K[d] < 0 || pred_K[d] > 30) return fail(STRIDE_E_ARG, "predictor K outside [0, 30]");
+  std::memcpy(h->pred_n, pred_n, sizeof h->pred_n);
+  std::memcpy(h->pred_K, pred_K, sizeof h->pred_K);
+  compute_residuals(h->prep, h->codes.data(), pred_n, pred_K, h->res, h->n_threads);
+  h->have_res = true;
+  if (n_values) *n_values = (int64_t)h->res.vals.size();
+  if (max_abs) *max_abs = h->res.max_abs;
+  return STRIDE_OK;
+}
+
+int stride_encoder_export_residuals(const stride_encoder* h, int32_t* vals, int64_t* stream_off) {
+  if (!h || !h->have_res) return fail(STRIDE_E_ARG, "residuals not computed");
+  const Residuals& R = h->res;
+  const size_t n_m = h->prep.plans.size();
+  if (R.max_abs > 0x7FFFFFFF) return fail(STRIDE_E_UNSUPPORTED, "residual exceeds int32");
+  if (vals)
+    for (size_t i = 0; i < R.vals.size(); ++i) vals[i] = (int32_t)R.vals[i];
+  if (stream_off) {
+    stream_off[0] = 0;
+    for (size_t m = 0; m < n_m; ++m) {
+      int64_t base = R.off[m];
+      for (int s = 0; s < 6; ++s) {
+        int64_t cnt = s < 3 ? R.n_delta[m] : R.n_para[m];
+        stream_off[6 * m + s] = base;
+        base += cnt;
+      }
+    }
+    stream_off[6 * n_m] = R.off[n_m];
+  }
+  return STRIDE_OK;
+}
+
+int stride_encoder_write(const stride_encoder* h, const float* f13, const int32_t* g_bits,
+                         const int32_t* pred_n, const int32_t* pred_K, const uint8_t* tagk,
+                         uint8_t* out, size_t cap, size_t* len) {
+  if (!h || !f13 || !g_bits || !pred_n || !pred_K || !tagk || !len)
+    return fail(STRIDE_E_ARG, "null argument");
+  if (!h->have_res) return fail(STRIDE_E_ARG, "residuals not computed");
+  if (std::memcmp(pred_n, h->pred_n, sizeof h->pred_n) ||
+      std::memcmp(pred_K, h->pred_K, sizeof h->pred_K))
+    return fail(STRIDE_E_ARG, "predictor differs from the one the residuals were computed with");
+  const size_t n_m = h->prep.plans.size();
+  for (size_t m = 0; m < n_m; ++m)
+    for (int s = 0; s < 6; ++s) {
+      uint8_t tag = tagk[12 * m + 2 * s];
+      if (tag != 1 && tag != 2) return fail(STRIDE_E_ARG, "coder tag must be 1 (Rice) or 2 (EG)");
+    }
+  Header H;
+  for (int i = 0; i < 3; ++i) {
+    H.center[i] = f13[i];
+    H.g_min[i] = f13[5 + i];
+    H.g_range[i] = f13[8 + i];
+    H.g_bits[i] = g_bits[i];
+    H.pred_K[i] = pred_K[i];
+  }
+  H.scale = f13[3];
+  H.per_coord_err = f13[4];
+  for (int i = 0; i < 9; ++i) H.pred_n[i] = pred_n[i];
+  std::vector<uint8_t> bytes = write_dup(h->prep, h->res, H, tagk);
+  *len = bytes.size();
+  if (out) {
+    if (cap < bytes.size()) return fail(STRIDE_E_ARG, "output buffer too small");
+    std::memcpy(out, bytes.data(), bytes.size());
+  }
+  return STRIDE_OK;
+}
+
+}  // extern "C"

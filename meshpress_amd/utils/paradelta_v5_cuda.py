@@ -1,0 +1,134 @@
+"""GPU decoder surface (reference: utils/paradelta_v5_cuda.py:580-718).
+
+`ParaDeltaV5GpuDecoder(data)` keeps the reference's class name, module path
+and keyword, and decodes the DPR4 v3 bytes that `STRIDEEncoder.encode()`
+returns, on MI355X through libstride_hip.so:
+
+* construction uploads the stream, runs the index pre-pass kernel that
+  derives the per-meshlet side table, validates every meshlet and allocates
+  the outputs;
+* `decode()` is one kernel launch on the current torch stream; it returns
+  the decoder-owned device tensors (overwritten by each call): float32
+  (n_v, 3) WORLD positions and int32 (n_t, 3) indices (the reference's v5
+  path returned normalised positions; DPR4 positions are dequantised to
+  world space inside the kernel, as decode_dup does);
+* `decode_to_host()` synchronises and returns numpy float32 (n_v, 3) and
+  uint32 (n_t, 3).
+
+Errors: ValueError("bad magic 0x........") / ValueError("expected v3, got
+vN") as the reference; IndexError for truncated streams (the reference's
+BitReader raises IndexError); ValueError for malformed meshlets.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .. import _lib
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("ParaDeltaV5GpuDecoder needs a ROCm GPU (torch.cuda is unavailable)")
+    return torch
+
+
+class ParaDeltaV5GpuDecoder:
+    def __init__(self, data: bytes, meshlets_per_block: int = 1):
+        if meshlets_per_block < 1 or meshlets_per_block > 16:
+            raise ValueError("meshlets_per_block must be in [1, 16]")
+        torch = _torch()
+        L = _lib.lib()
+        self._L = L
+        self.meshlets_per_block = int(meshlets_per_block)
+        self.data = bytes(data)
+        self.device = torch.device("cuda", torch.cuda.current_device())
+        h = C.c_void_p()
+        buf = np.frombuffer(self.data, dtype=np.uint8)
+        _lib.check(L.stride_init(self.device.index))
+        _lib.check(L.stride_decoder_create(_lib.ptr(buf) if len(buf) else None, len(buf),
+                                           C.byref(h)))
+        self._h = h
+        nv, nt, nm = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        _lib.check(L.stride_decoder_sizes(h, C.byref(nv), C.byref(nt), C.byref(nm)))
+        self.n_v_total = int(nv.value)
+        self.n_t_total = int(nt.value)
+        self.n_meshlets = int(nm.value)
+        sb, side = C.c_uint64(), C.c_uint64()
+        _lib.check(L.stride_decoder_bytes(h, C.byref(sb), C.byref(side)))
+        self.stream_bytes = int(sb.value)
+        self.side_bytes = int(side.value)
+        f13 = np.zeros(13, np.float32)
+        i16 = np.zeros(16, np.int32)
+        _lib.check(L.stride_decoder_header(h, _lib.ptr(f13), _lib.ptr(i16)))
+        self.center = f13[0:3].copy()
+        self.scale = float(f13[3])
+        self.per_coord_err = float(f13[4])
+        self.g_min, self.g_range = f13[5:8].copy(), f13[8:11].copy()
+        self.g_bits = i16[0:3].copy()
+        self.pred_n = i16[3:12].reshape(3, 3).copy()
+        self.pred_K = i16[12:15].copy()
+        self.code_width = int(i16[15])
+        self.d_verts = torch.empty((self.n_v_total, 3), dtype=torch.float32, device=self.device)
+        self.d_tris = torch.empty((self.n_t_total, 3), dtype=torch.int32, device=self.device)
+        # the reference harnesses read these names (scripts/bench_stride_decode_sweep.py:53-79)
+        self.d_verts_norm = self.d_verts
+        self.d_buf_bytes = self.stream_bytes
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.stride_decoder_destroy(h)
+            self._h = None
+
+    # --------------------------------------------------------------- tuning
+    def get_tune(self) -> dict:
+        t = _lib.StrideTune()
+        _lib.check(self._L.stride_decoder_get_tune(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def set_tune(self, waves=None, stage_words=None, grid=None, use_queue=None) -> None:
+        cur = self.get_tune()
+        t = _lib.StrideTune(
+            waves if waves is not None else cur["waves"],
+            stage_words if stage_words is not None else cur["stage_words"],
+            grid if grid is not None else cur["grid"],
+            use_queue if use_queue is not None else cur["use_queue"])
+        _lib.check(self._L.stride_decoder_set_tune(self._h, C.byref(t)))
+
+    # --------------------------------------------------------------- decode
+    def decode(self, stream=None):
+        """Launch on `stream` (default: torch's current stream); returns the
+        decoder-owned (verts f32 (n_v,3), tris int32 (n_t,3)) device tensors."""
+        _lib.check(self._L.stride_decode(self._h, _lib.ptr(self.d_verts), _lib.ptr(self.d_tris),
+                                         _lib.stream_ptr(stream)))
+        return self.d_verts, self.d_tris
+
+    def decode_into(self, d_verts, d_tris, stream=None) -> None:
+        """Decode into caller-owned contiguous device tensors."""
+        if not (d_verts.is_contiguous() and d_tris.is_contiguous()):
+            raise ValueError("outputs must be contiguous")
+        if d_verts.numel() < 3 * self.n_v_total or d_tris.numel() < 3 * self.n_t_total:
+            raise ValueError("output tensors are too small")
+        if d_verts.dtype.itemsize != 4 or d_tris.dtype.itemsize != 4:
+            raise ValueError("outputs must be 4-byte types")
+        _lib.check(self._L.stride_decode(self._h, _lib.ptr(d_verts), _lib.ptr(d_tris),
+                                         _lib.stream_ptr(stream)))
+
+    def decode_to_host(self):
+        """(verts float32 (n_v,3) world, tris uint32 (n_t,3)), synchronous."""
+        torch = _torch()
+        v, t = self.decode()
+        torch.cuda.current_stream().synchronize()
+        return (v.cpu().numpy(), t.cpu().numpy().view(np.uint32))
+
+    @property
+    def bytes_read(self) -> int:
+        """Compressed bytes a decode reads: stream + derived side table."""
+        return self.stream_bytes + self.side_bytes
+
+    @property
+    def bytes_written(self) -> int:
+        return 12 * self.n_v_total + 12 * self.n_t_total

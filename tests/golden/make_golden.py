@@ -1,0 +1,185 @@
+"""Writes golden vectors from the reference's own Python codec functions.
+
+Run in the development container only (it reads /root/reference, which never
+reaches a GPU machine):
+
+    python tests/golden/make_golden.py
+
+The reference functions are called on this repository's own plans (the
+reference front-end needs numba, which is absent), so the fixtures pin the
+parts no closed form pins:
+
+  * encode_dup(prep, predictor="canonical") -> exact DPR4 bytes
+    (encoder/paradelta_v5_dup.py:316-464);
+  * decode_dup(bytes) -> V float32, T int64, including the triangle corner
+    order (encoder/paradelta_v5_dup.py:471-618), also on streams with a
+    fitted IRLP predictor;
+  * the IRLP fit: _gather + subsample + _do_fit + per-axis canonical fallback
+    exactly as fit_predictor (encoder/_irlp_fit.py:364-394) but without its
+    sample cache (the cache would be written into the read-only reference
+    tree);
+  * _pick_best_k (paradelta_v5_dup.py:159-171) and encoder_cost_batch
+    (_irlp_fit.py:186-226) on seeded random residuals.
+
+Fixtures are data only: inputs (bytes, sample arrays) and expected outputs.
+Large outputs are stored as SHA-256 digests.
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import sys
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parents[1]
+REF = Path("/root/reference")
+sys.path.insert(0, str(ROOT))
+
+
+def sha(a: np.ndarray) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def meshes():
+    from meshpress_amd import workloads as w
+    out = [("uv_12x24",) + w.displaced_uv_sphere(12, 24, 0.05, 4, 3),
+           ("torus_30x10",) + w.displaced_torus(30, 10, 1.0, 0.3, 0.05, 4, 5),
+           ("hf_24",) + w.heightfield(24, 0.1, 6, 7),
+           ("ico_2",) + w.displaced_icosphere(2, 0.08, 6, 1),
+           ("one_tri", np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0.5]]), np.array([[0, 1, 2]])),
+           ("uv_c1",) + w.uv_sphere_c1(),
+           ("torus_q16",) + w.displaced_torus(60, 20, 1.0, 0.3, 0.05, 4, 11)]
+    return out
+
+
+def main():
+    from meshpress_amd.encoder.prep import prepare
+    sys.path.insert(0, str(REF))
+    from encoder.paradelta_v5_dup import (encode_dup, decode_dup, _pick_best_k,  # noqa: E402
+                                          _walk_meshlet)
+    import encoder._irlp_fit as F  # noqa: E402
+
+    summary = {}
+    for name, V, T in meshes():
+        kw = {}
+        if name == "torus_q16":
+            kw = dict(precision_error=1.0 / 65534.5, precision_mode="bbox_frac")
+        p = prepare(V, T, **kw)
+        prep = dict(p)
+        prep.pop("native")
+        plans = p["native"].export_plans()
+        for pl in plans:
+            pl.pop("walk")
+        prep["plans"] = plans
+        data = encode_dup(prep, predictor="canonical")
+        Vd, Td = decode_dup(data)
+        rec = {"data": np.frombuffer(data, np.uint8)}
+        # IRLP fit with the reference's functions (fit_predictor minus cache)
+        A, B, C, Vs, off = F._gather(prep["global_codes"], plans, _walk_meshlet, use_cache=False)
+        n_m = len(off) - 1
+        if n_m == 0 or len(A) == 0:
+            n_fit, K_fit = F.CANON_N.copy(), F.CANON_K.copy()
+        else:
+            if n_m > F.SUBSAMPLE_MESHLETS:
+                rng = np.random.default_rng(0)
+                sel = np.sort(rng.choice(n_m, size=F.SUBSAMPLE_MESHLETS, replace=False))
+                rows = np.concatenate([np.arange(off[i], off[i + 1]) for i in sel])
+                As, Bs, Cs, Vss = A[rows], B[rows], C[rows], Vs[rows]
+                off_s = np.concatenate(([0], np.cumsum(np.diff(off)[sel])))
+            else:
+                As, Bs, Cs, Vss, off_s = A, B, C, Vs, off
+            n_fit, K_fit = F._do_fit(As, Bs, Cs, Vss, off_s)
+            for d in range(3):
+                b_fit = F.encoder_cost_one(A[:, d], B[:, d], C[:, d], Vs[:, d], off,
+                                           n_fit[d], int(K_fit[d]))
+                b_can = F.encoder_cost_one(A[:, d], B[:, d], C[:, d], Vs[:, d], off,
+                                           F.CANON_N[d], int(F.CANON_K[d]))
+                if b_can < b_fit:
+                    n_fit[d] = F.CANON_N[d]
+                    K_fit[d] = F.CANON_K[d]
+        rec["fit_n"] = np.asarray(n_fit, np.int64)
+        rec["fit_K"] = np.asarray(K_fit, np.int64)
+        small = len(V) <= 5000
+        if small:
+            rec["V"] = Vd
+            rec["T"] = Td.astype(np.int64)
+            rec["samples_A"] = A
+            rec["samples_B"] = B
+            rec["samples_C"] = C
+            rec["samples_V"] = Vs
+            rec["samples_off"] = off
+        rec["V_sha"] = np.frombuffer(bytes.fromhex(sha(Vd)), np.uint8)
+        rec["T_sha"] = np.frombuffer(bytes.fromhex(sha(Td.astype(np.int64))), np.uint8)
+        np.savez_compressed(HERE / f"dup_{name}.npz", **rec)
+        summary[name] = {"bytes": len(data), "n_v": int(len(Vd)), "n_t": int(len(Td)),
+                         "fit_n": np.asarray(n_fit).tolist(), "fit_K": np.asarray(K_fit).tolist()}
+        print(name, summary[name], flush=True)
+
+    # Generalized-predictor streams: our oracle writes them, the reference decodes.
+    from oracle import stride_oracle as O
+    for name in ("uv_12x24", "ico_2", "uv_c1"):
+        V, T = {n: (v, t) for n, v, t in meshes()}[name]
+        p = prepare(V, T)
+        prep = dict(p)
+        prep["plans"] = p["native"].export_plans()
+        z = np.load(HERE / f"dup_{name}.npz")
+        n_fit, K_fit = z["fit_n"], z["fit_K"]
+        if name == "uv_12x24":   # force a non-canonical predictor with K > 0
+            n_fit = np.array([[33, 32, -33], [31, 34, -33], [32, 32, -32]], np.int64)
+            K_fit = np.array([5, 5, 5], np.int64)
+        data = O.encode_dup(prep, n_fit, K_fit)
+        Vd, Td = decode_dup(data)
+        rec = {"data": np.frombuffer(data, np.uint8), "pred_n": n_fit, "pred_K": K_fit,
+               "V_sha": np.frombuffer(bytes.fromhex(sha(Vd)), np.uint8),
+               "T_sha": np.frombuffer(bytes.fromhex(sha(Td.astype(np.int64))), np.uint8)}
+        if len(V) <= 5000:
+            rec["V"] = Vd
+            rec["T"] = Td.astype(np.int64)
+        np.savez_compressed(HERE / f"irlp_{name}.npz", **rec)
+        print("irlp", name, len(data), flush=True)
+
+    # _pick_best_k and encoder_cost_batch on seeded residuals.
+    rng = np.random.default_rng(1234)
+    streams = []
+    for i in range(64):
+        n = int(rng.integers(0, 200))
+        scale = float(rng.choice([0.3, 2, 20, 300, 5000]))
+        kind = i % 3
+        if kind == 0:
+            x = np.round(rng.laplace(0, scale, n))
+        elif kind == 1:
+            x = np.round(rng.normal(0, scale, n))
+        else:   # heavy tail: a few outliers make Exp-Golomb win
+            x = np.round(rng.laplace(0, 1, n))
+            if n:
+                x[rng.integers(0, n, max(1, n // 20))] = rng.integers(-50000, 50000, max(1, n // 20))
+        streams.append(x.astype(np.int64))
+    lens = np.array([len(s) for s in streams], np.int64)
+    flat = np.concatenate(streams)
+    tags, ks = [], []
+    for s in streams:
+        t, k = _pick_best_k(s.reshape(-1, 1).repeat(3, axis=1))
+        tags.append(t[0])
+        ks.append(k[0])
+    m_off = np.concatenate(([0], np.cumsum(rng.integers(1, 60, 80))))
+    ns = int(m_off[-1])
+    a, b, c = (rng.integers(0, 4096, ns) for _ in range(3))
+    v = np.clip(a + b - c + np.round(rng.laplace(0, 3, ns)).astype(np.int64), 0, 4095)
+    cands = np.array([[1, 1, -1], [32, 32, -32], [33, 31, -32], [30, 35, -33], [64, 64, -64],
+                      [70, 60, -66]], np.int64)
+    Ks = np.array([0, 5, 5, 5, 6, 6], np.int64)
+    costs = np.array([int(F.encoder_cost_batch(a, b, c, v, m_off, cands[i:i + 1], int(Ks[i]))[0])
+                      for i in range(len(cands))], np.int64)
+    np.savez_compressed(HERE / "coder_choice.npz", stream_lens=lens, stream_vals=flat,
+                        tags=np.array(tags, np.int64), ks=np.array(ks, np.int64),
+                        irlp_a=a, irlp_b=b, irlp_c=c, irlp_v=v, irlp_off=m_off,
+                        irlp_cands=cands, irlp_K=Ks, irlp_costs=costs)
+    (HERE / "summary.json").write_text(json.dumps(summary, indent=1))
+    print("coder_choice", list(zip(tags, ks))[:10])
+
+
+if __name__ == "__main__":
+    main()
