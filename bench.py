@@ -108,7 +108,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="heightfield_1900_q12")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (counter calibration)")
     a = ap.parse_args()
+    if a.copy_calibration:
+        import torch
+        n = a.copy_calibration << 20
+        x = torch.ones(n // 4, dtype=torch.int32, device="cuda")
+        y = torch.empty_like(x)
+        torch.cuda.synchronize()
+        y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes": n, "read": n, "written": n}), flush=True)
+        return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(decode shards by meshlet ranges with no exchange; see DESIGN.md)", file=sys.stderr)

@@ -83,14 +83,24 @@ int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes,
  * into f13; g_bits[3], pred_n[9], pred_K[3], code_width into i16. */
 int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
 
-/* Launch configuration (autotuned): waves per workgroup {1,2,4}, LDS stage
- * words per meshlet {256,512,1024}, persistent grid (0 = one wave per
- * meshlet), atomic work queue flag. */
+/* Launch configuration (autotuned).
+ *   mode 1 (default): groups of up to group_size consecutive meshlets per
+ *     wave, one lane per meshlet, lds_bytes of LDS per group (staged bits +
+ *     packed codes); meshlets that do not fit, or whose codes leave the
+ *     packed range, are decoded by mode 0.
+ *   mode 0: one wave per meshlet: waves per workgroup {1,2,4}, LDS stage
+ *     words per meshlet {256,512,1024}, use_queue = persistent grid pulling
+ *     meshlets from a device atomic counter.
+ *   grid: 0 = one workgroup per unit of work, else a persistent grid. */
 typedef struct stride_tune {
   int32_t waves;
   int32_t stage_words;
   int32_t grid;
   int32_t use_queue;
+  int32_t mode;
+  int32_t group_size;
+  int32_t lds_bytes;
+  int32_t reserved;
 } stride_tune;
 
 int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t);

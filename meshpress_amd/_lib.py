@@ -28,10 +28,20 @@ P = C.POINTER
 
 
 class StrideTune(C.Structure):
-    _fields_ = [("waves", i32), ("stage_words", i32), ("grid", i32), ("use_queue", i32)]
+    _fields_ = [("waves", i32), ("stage_words", i32), ("grid", i32), ("use_queue", i32),
+                ("mode", i32), ("group_size", i32), ("lds_bytes", i32), ("reserved", i32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
+
+
+def make_tune(cur: dict, kw: dict) -> StrideTune:
+    names = [n for n, _ in StrideTune._fields_]
+    bad = set(kw) - set(names)
+    if bad:
+        raise ValueError(f"unknown tune fields {sorted(bad)}")
+    vals = {**cur, **{k: int(v) for k, v in kw.items() if v is not None}}
+    return StrideTune(*[vals[n] for n in names])
 
 
 # name -> (restype, argtypes)

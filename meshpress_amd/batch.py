@@ -59,13 +59,8 @@ class BatchDecoder:
         _lib.check(self._L.stride_batch_get_tune(self._h, C.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, waves=None, stage_words=None, grid=None, use_queue=None) -> None:
-        cur = self.get_tune()
-        t = _lib.StrideTune(
-            waves if waves is not None else cur["waves"],
-            stage_words if stage_words is not None else cur["stage_words"],
-            grid if grid is not None else cur["grid"],
-            use_queue if use_queue is not None else cur["use_queue"])
+    def set_tune(self, **kw) -> None:
+        t = _lib.make_tune(self.get_tune(), kw)
         _lib.check(self._L.stride_batch_set_tune(self._h, C.byref(t)))
 
     def decode(self, mode: str = "launch", stream=None):

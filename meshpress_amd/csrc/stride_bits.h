@@ -7,8 +7,14 @@
 // bits) and encoder/paradelta_codec.py:330-340 (FIFO).
 #pragma once
 
-#include <hip/hip_runtime.h>
 #include <cstdint>
+
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#define STRIDE_DEV __host__ __device__ __forceinline__
+#else
+#define STRIDE_DEV inline
+#endif
 
 namespace stride {
 
@@ -17,13 +23,13 @@ namespace stride {
 struct LdsWords {
   const uint32_t* w;
   uint32_t n;
-  __device__ __forceinline__ uint32_t operator()(uint32_t i) const { return i < n ? w[i] : 0u; }
+  STRIDE_DEV uint32_t operator()(uint32_t i) const { return i < n ? w[i] : 0u; }
 };
 
 struct GlobalWords {
   const uint32_t* w;   // little-endian words of the blob
   uint32_t n;
-  __device__ __forceinline__ uint32_t operator()(uint32_t i) const {
+  STRIDE_DEV uint32_t operator()(uint32_t i) const {
     return i < n ? __builtin_bswap32(w[i]) : 0u;
   }
 };
@@ -36,7 +42,7 @@ struct BitReader {
   uint32_t pos;    // absolute bit position of the next unread bit
   Src src;
 
-  __device__ __forceinline__ void init(const Src& s, uint32_t bit) {
+  STRIDE_DEV void init(const Src& s, uint32_t bit) {
     src = s;
     uint32_t w0 = bit >> 5;
     int skip = (int)(bit & 31u);
@@ -46,14 +52,14 @@ struct BitReader {
     widx = w0 + 2;
     pos = bit;
   }
-  __device__ __forceinline__ void refill() {
+  STRIDE_DEV void refill() {
     if (valid <= 32) {
       win |= (uint64_t)src(widx++) << (32 - valid);
       valid += 32;
     }
   }
   // 0 <= n <= 32
-  __device__ __forceinline__ uint32_t bits(int n) {
+  STRIDE_DEV uint32_t bits(int n) {
     refill();
     uint32_t out = n ? (uint32_t)(win >> (64 - n)) : 0u;
     win = n ? (win << n) : win;
@@ -64,7 +70,7 @@ struct BitReader {
   // Count zeros before the next '1' and consume the '1'.  Stops counting
   // once the run passes `limit` bits (malformed input); the caller checks
   // pos against the meshlet end.
-  __device__ __forceinline__ uint32_t unary(uint32_t limit) {
+  STRIDE_DEV uint32_t unary(uint32_t limit) {
     uint32_t q = 0;
     for (;;) {
       refill();
@@ -83,13 +89,13 @@ struct BitReader {
     }
   }
   // Rice(k): u = (q << k) | low   (utils/bit_codec.py:226-235)
-  __device__ __forceinline__ uint64_t rice(int k, uint32_t limit) {
+  STRIDE_DEV uint64_t rice(int k, uint32_t limit) {
     uint64_t q = unary(limit);
     uint32_t low = bits(k);
     return (q << k) | low;
   }
   // Exp-Golomb(k) (utils/bit_codec.py:237-250)
-  __device__ __forceinline__ uint64_t exp_golomb(int k, uint32_t limit) {
+  STRIDE_DEV uint64_t exp_golomb(int k, uint32_t limit) {
     uint32_t lb = unary(limit);
     uint64_t shifted = 0;
     if (lb > 0) {
@@ -101,7 +107,7 @@ struct BitReader {
   }
 };
 
-__device__ __forceinline__ int64_t unzigzag(uint64_t u) {
+STRIDE_DEV int64_t unzigzag(uint64_t u) {
   return (int64_t)(u >> 1) ^ -(int64_t)(u & 1);
 }
 
@@ -110,13 +116,13 @@ __device__ __forceinline__ int64_t unzigzag(uint64_t u) {
 struct Fifo {
   uint64_t lo, hi;
   uint32_t n;
-  __device__ __forceinline__ void clear() { lo = 0; hi = 0; n = 0; }
-  __device__ __forceinline__ uint32_t get(uint32_t i) const {
+  STRIDE_DEV void clear() { lo = 0; hi = 0; n = 0; }
+  STRIDE_DEV uint32_t get(uint32_t i) const {
     uint64_t w = i < 8 ? lo : hi;
     return (uint32_t)(w >> (8 * (i & 7))) & 0xFFu;
   }
   // Remove entry i (< n); entries above it move down by one.
-  __device__ __forceinline__ void remove(uint32_t i) {
+  STRIDE_DEV void remove(uint32_t i) {
     uint64_t slo = (lo >> 8) | (hi << 56);
     uint64_t shi = hi >> 8;
     if (i < 8) {
@@ -130,13 +136,13 @@ struct Fifo {
     }
     n--;
   }
-  __device__ __forceinline__ void append(uint32_t v) {
+  STRIDE_DEV void append(uint32_t v) {
     if (n < 8) lo |= (uint64_t)(v & 0xFFu) << (8 * n);
     else hi |= (uint64_t)(v & 0xFFu) << (8 * (n - 8));
     n++;
   }
   // Index of entry equal to v among the first n entries, or -1.
-  __device__ __forceinline__ int find(uint32_t v) const {
+  STRIDE_DEV int find(uint32_t v) const {
     const uint64_t ones = 0x0101010101010101ull, low7 = 0x7F7F7F7F7F7F7F7Full;
     const uint64_t pat = (uint64_t)(v & 0xFFu) * ones;
     uint64_t t0 = lo ^ pat, t1 = hi ^ pat;
@@ -153,13 +159,13 @@ struct Fifo {
   // Token resolution (encoder/paradelta_codec.py:330-340): a FIFO hit
   // takes entry fi; a raw id leaves the FIFO if present; the value is
   // moved to the back, the oldest entry dropping out of a full FIFO.
-  __device__ __forceinline__ uint32_t hit(uint32_t fi) {
+  STRIDE_DEV uint32_t hit(uint32_t fi) {
     uint32_t v = get(fi);
     remove(fi);
     append(v);
     return v;
   }
-  __device__ __forceinline__ void raw(uint32_t v) {
+  STRIDE_DEV void raw(uint32_t v) {
     int p = find(v);
     if (p >= 0) remove((uint32_t)p);
     else if (n == 16) remove(0);
@@ -167,10 +173,57 @@ struct Fifo {
   }
 };
 
+// The same reuse FIFO kept newest-first, for branch-free updates: byte j of
+// (lo, hi) is the entry of recency rank j (0 = most recently used); FIFO
+// index fi (oldest first) is rank n - 1 - fi.  Every token is one
+// move-to-front: ranks below j move up by one and the value lands at rank 0;
+// rank j is dropped (j = n for a value not in the FIFO, which with n == 16
+// drops the oldest entry).
+struct Mtf {
+  uint64_t lo, hi;
+  uint32_t n;
+  STRIDE_DEV void clear() { lo = 0; hi = 0; n = 0; }
+  STRIDE_DEV uint32_t rank(uint32_t j) const {
+    const uint64_t w = j < 8 ? lo : hi;
+    return (uint32_t)(w >> (8 * (j & 7))) & 0xFFu;
+  }
+  // 128-bit mask of bytes [0, j), j in [0, 17]
+  STRIDE_DEV static void mask(uint32_t j, uint64_t& mlo, uint64_t& mhi) {
+    const uint32_t s = 8 * j;
+    mlo = s >= 64 ? ~0ull : ((1ull << s) - 1);
+    mhi = s <= 64 ? 0ull : (s >= 128 ? ~0ull : ((1ull << (s - 64)) - 1));
+  }
+  STRIDE_DEV void to_front(uint32_t j, uint32_t v) {
+    uint64_t mlo, mhi, klo, khi;
+    mask(j, mlo, mhi);
+    mask(j + 1, klo, khi);
+    const uint64_t llo = lo & mlo, lhi = hi & mhi;
+    const uint64_t nlo = (lo & ~klo) | (llo << 8) | (uint64_t)(v & 0xFFu);
+    const uint64_t nhi = (hi & ~khi) | (lhi << 8) | (llo >> 56);
+    lo = nlo;
+    hi = nhi;
+  }
+  // rank of v among the n entries, or -1
+  STRIDE_DEV int find(uint32_t v) const {
+    const uint64_t ones = 0x0101010101010101ull, low7 = 0x7F7F7F7F7F7F7F7Full;
+    const uint64_t pat = (uint64_t)(v & 0xFFu) * ones;
+    const uint64_t t0 = lo ^ pat, t1 = hi ^ pat;
+    uint64_t z0 = ~(((t0 & low7) + low7) | t0 | low7);
+    uint64_t z1 = ~(((t1 & low7) + low7) | t1 | low7);
+    uint64_t v0, v1;
+    mask(n, v0, v1);
+    z0 &= v0;
+    z1 &= v1;
+    if (z0) return __builtin_ctzll(z0) >> 3;
+    if (z1) return 8 + (__builtin_ctzll(z1) >> 3);
+    return -1;
+  }
+};
+
 struct Seen256 {
   uint64_t s[4];
-  __device__ __forceinline__ void clear() { s[0] = s[1] = s[2] = s[3] = 0; }
-  __device__ __forceinline__ bool test_set(uint32_t v) {
+  STRIDE_DEV void clear() { s[0] = s[1] = s[2] = s[3] = 0; }
+  STRIDE_DEV bool test_set(uint32_t v) {
     uint32_t q = (v >> 6) & 3u;
     uint64_t b = 1ull << (v & 63);
     uint64_t w = q == 0 ? s[0] : q == 1 ? s[1] : q == 2 ? s[2] : s[3];

@@ -127,6 +127,8 @@ struct DecodeSet {
   std::vector<DupHeader> hdr;
   std::vector<uint64_t> v_base, t_base;   // n_meshes + 1
   std::vector<uint32_t> m_base;           // first meshlet of each mesh, n_meshes + 1
+  std::vector<MeshletEntry> tab;          // host copy of the side table
+  std::vector<MeshParams> mp;
   uint8_t* d_buf = nullptr;
   size_t buf_bytes = 0, stream_bytes = 0;
   MeshletEntry* d_tab = nullptr;
@@ -137,12 +139,19 @@ struct DecodeSet {
   float* d_v_own = nullptr;
   uint32_t* d_t_own = nullptr;
   DecodeConfig cfg;
+  // group layout (mode 1)
+  GroupEntry* d_groups = nullptr;
+  uint32_t n_groups = 0;
+  std::vector<uint32_t> g_base;           // first group of each mesh, n_meshes + 1
+  uint32_t* d_legacy = nullptr;           // meshlets decoded by the wave-per-meshlet kernel
+  uint32_t n_legacy = 0;
+  std::vector<uint32_t> l_base;           // first legacy entry of each mesh
   // captured graphs keyed by (verts, tris, tune)
-  std::map<std::tuple<void*, void*, int, int, int, int>, hipGraphExec_t> graphs;
+  std::map<std::tuple<void*, void*, int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
   hipStream_t cap_stream = nullptr;
 
   ~DecodeSet() {
-    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    drop_graphs();
     if (cap_stream) (void)hipStreamDestroy(cap_stream);
     (void)hipFree(d_buf);
     (void)hipFree(d_tab);
@@ -150,6 +159,13 @@ struct DecodeSet {
     (void)hipFree(d_queue);
     (void)hipFree(d_v_own);
     (void)hipFree(d_t_own);
+    (void)hipFree(d_groups);
+    (void)hipFree(d_legacy);
+  }
+
+  void drop_graphs() {
+    for (auto& kv : graphs) (void)hipGraphExecDestroy(kv.second);
+    graphs.clear();
   }
 
   int build(const uint8_t* const* blobs, const size_t* lens, int n_meshes) {
@@ -167,12 +183,13 @@ struct DecodeSet {
       nm_total += hdr[i].n_m;
       stream_bytes += lens[i];
     }
-    if (nm_total > 0xFFFFFFFFull) return fail(STRIDE_E_UNSUPPORTED, "too many meshlets");
+    if (nm_total > 0x7FFFFFFFull) return fail(STRIDE_E_UNSUPPORTED, "too many meshlets");
+    if (total > 0x3FFFFFFF0ull) return fail(STRIDE_E_UNSUPPORTED, "streams larger than 16 GiB");
     n_m = (uint32_t)nm_total;
     buf_bytes = total + 64;
     std::vector<uint8_t> hbuf(buf_bytes, 0);
-    std::vector<MeshletEntry> seeds(n_m);
-    std::vector<MeshParams> mp(n_meshes);
+    tab.assign(n_m, MeshletEntry{});
+    mp.assign(n_meshes, MeshParams{});
     m_base.assign(n_meshes + 1, 0);
     uint32_t m = 0;
     for (int i = 0; i < n_meshes; ++i) {
@@ -199,8 +216,7 @@ struct DecodeSet {
         uint64_t nxt = j + 1 < h.n_m ? be32(blobs[i] + 86 + 4 * (j + 1)) : total_bits;
         if ((j == 0 && off != region) || off >= nxt || nxt > total_bits)
           return fail(STRIDE_E_CORRUPT, "meshlet offset table is not consistent with the stream");
-        MeshletEntry& e = seeds[m + j];
-        std::memset(&e, 0, sizeof e);
+        MeshletEntry& e = tab[m + j];
         e.mesh = (uint32_t)i;
         e.start_bit = off;
         e.end_bit = (uint32_t)nxt;
@@ -219,7 +235,7 @@ struct DecodeSet {
     std::vector<MeshletScan> scan(n_m);
     if (n_m) {
       MeshletScan* d_scan = nullptr;
-      HIP_TRY(hipMemcpy(d_tab, seeds.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_tab, tab.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
       HIP_TRY(hipMalloc(&d_scan, sizeof(MeshletScan) * n_m));
       hipError_t e1 = launch_scan(d_buf, d_tab, d_mesh, n_m, d_scan, nullptr);
       hipError_t e2 = e1 == hipSuccess ? hipDeviceSynchronize() : e1;
@@ -248,7 +264,7 @@ struct DecodeSet {
                         scan_err_text(s.error));
           return fail(STRIDE_E_CORRUPT, b);
         }
-        MeshletEntry& e = seeds[j];
+        MeshletEntry& e = tab[j];
         if (j + 1 < m_base[i + 1] && s.end_bit != e.end_bit)
           return fail(STRIDE_E_CORRUPT,
                       "meshlet sizes disagree with the meshlet offset table");
@@ -258,7 +274,7 @@ struct DecodeSet {
         e.t_off = (uint32_t)to;
         e.n_local = s.n_local;
         e.n_tris = s.n_tris;
-        e.n_strips = s.n_strips;
+        e.n_strips = s.n_strips | (s.flags & kFlagRawInFifo);
         e.n_kind0 = s.n_kind0;
         e.resid_bit = s.resid_bit;
         for (int k = 0; k < 6; ++k) e.sub_bit[k] = s.sub_bit[k];
@@ -275,43 +291,204 @@ struct DecodeSet {
     n_t = tt;
     HIP_TRY(hipMemcpy(d_mesh, mp.data(), sizeof(MeshParams) * n_meshes, hipMemcpyHostToDevice));
     if (n_m)
-      HIP_TRY(hipMemcpy(d_tab, seeds.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
+      HIP_TRY(hipMemcpy(d_tab, tab.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
+    HIP_TRY(hipMalloc(&d_v_own, 12 * std::max<uint64_t>(n_v, 1)));
+    HIP_TRY(hipMalloc(&d_t_own, 12 * std::max<uint64_t>(n_t, 1)));
+    return layout();
+  }
+
+  // Partition the meshlets into groups for the current tune and run one
+  // validation decode: groups whose codes leave the packed range move to
+  // the wave-per-meshlet list, which must itself fit 32-bit codes.
+  int layout() {
+    drop_graphs();
+    (void)hipFree(d_groups);
+    (void)hipFree(d_legacy);
+    d_groups = nullptr;
+    d_legacy = nullptr;
+    const int n_meshes = (int)mp.size();
+    std::vector<uint8_t> legacy(n_m, cfg.mode == 0 ? 1 : 0);
+    std::vector<GroupEntry> groups;
+    for (int pass = 0; pass < 2; ++pass) {
+      groups.clear();
+      g_base.assign(n_meshes + 1, 0);
+      if (cfg.mode == 1) {
+        const uint64_t budget = (uint64_t)cfg.lds_bytes;
+        for (int i = 0; i < n_meshes; ++i) {
+          g_base[i] = (uint32_t)groups.size();
+          const uint64_t bb = mp[i].blob_byte * 8ull;
+          GroupEntry cur{};
+          uint64_t codes = 0;
+          bool open = false;
+          auto close = [&]() {
+            if (open) {
+              cur.codes_off = (cur.n_words + 1) / 2;
+              groups.push_back(cur);
+            }
+            open = false;
+          };
+          for (uint32_t j = m_base[i]; j < m_base[i + 1]; ++j) {
+            if (legacy[j]) { close(); continue; }
+            const MeshletEntry& e = tab[j];
+            const uint64_t lo = (bb + e.start_bit) >> 5;
+            const uint64_t hi = ((bb + e.end_bit + 31) >> 5) + 2;
+            auto need = [](uint64_t words, uint64_t nloc) {
+              return 8 * ((words + 1) / 2) + 8 * nloc;
+            };
+            if (open && (j - cur.m0 < (uint32_t)cfg.group_size) &&
+                need(hi - cur.w_lo, codes + e.n_local) <= budget) {
+              cur.m1 = j + 1;
+              cur.n_words = (uint32_t)(hi - cur.w_lo);
+              codes += e.n_local;
+              continue;
+            }
+            close();
+            if (need(hi - lo, e.n_local) > budget) {   // too large for any group
+              legacy[j] = 1;
+              continue;
+            }
+            cur = GroupEntry{};
+            cur.m0 = j;
+            cur.m1 = j + 1;
+            cur.w_lo = (uint32_t)lo;
+            cur.n_words = (uint32_t)(hi - lo);
+            codes = e.n_local;
+            open = true;
+          }
+          close();
+        }
+        g_base[n_meshes] = (uint32_t)groups.size();
+      }
+      n_groups = (uint32_t)groups.size();
+      if (n_groups) {
+        HIP_TRY(hipMalloc(&d_groups, sizeof(GroupEntry) * n_groups));
+        HIP_TRY(hipMemcpy(d_groups, groups.data(), sizeof(GroupEntry) * n_groups,
+                          hipMemcpyHostToDevice));
+      }
+      if (pass == 1 || n_groups == 0) break;
+      // validation decode of the groups
+      uint32_t* d_flags = nullptr;
+      HIP_TRY(hipMalloc(&d_flags, 4 * n_groups));
+      hipError_t e = hipMemset(d_flags, 0, 4 * n_groups);
+      GroupLaunch g = group_args(d_v_own, d_t_own, 0, n_groups, d_flags);
+      if (e == hipSuccess) e = launch_group(g, nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      std::vector<uint32_t> flags(n_groups, 0);
+      if (e == hipSuccess) e = hipMemcpy(flags.data(), d_flags, 4 * n_groups, hipMemcpyDeviceToHost);
+      (void)hipFree(d_flags);
+      HIP_TRY(e);
+      bool any = false;
+      for (uint32_t gi = 0; gi < n_groups; ++gi)
+        if (flags[gi]) {
+          any = true;
+          for (uint32_t j = groups[gi].m0; j < groups[gi].m1; ++j) legacy[j] = 1;
+        }
+      if (!any) break;
+      (void)hipFree(d_groups);
+      d_groups = nullptr;
+    }
+    std::vector<uint32_t> lst;
+    l_base.assign(n_meshes + 1, 0);
+    for (int i = 0; i < n_meshes; ++i) {
+      l_base[i] = (uint32_t)lst.size();
+      for (uint32_t j = m_base[i]; j < m_base[i + 1]; ++j)
+        if (legacy[j]) lst.push_back(j);
+    }
+    l_base[n_meshes] = (uint32_t)lst.size();
+    n_legacy = (uint32_t)lst.size();
+    if (n_legacy) {
+      HIP_TRY(hipMalloc(&d_legacy, 4 * n_legacy));
+      HIP_TRY(hipMemcpy(d_legacy, lst.data(), 4 * n_legacy, hipMemcpyHostToDevice));
+      // validation decode of the wave-per-meshlet list: codes must fit int32
+      uint32_t* d_flag = nullptr;
+      HIP_TRY(hipMalloc(&d_flag, 4));
+      hipError_t e = hipMemset(d_flag, 0, 4);
+      DecodeLaunch d = legacy_args(d_v_own, d_t_own, 0, n_legacy);
+      d.overflow = d_flag;
+      d.use_queue = 0;
+      d.grid = 0;
+      if (e == hipSuccess) e = launch_decode(d, cfg, nullptr);
+      if (e == hipSuccess) e = hipDeviceSynchronize();
+      uint32_t flag = 0;
+      if (e == hipSuccess) e = hipMemcpy(&flag, d_flag, 4, hipMemcpyDeviceToHost);
+      (void)hipFree(d_flag);
+      HIP_TRY(e);
+      if (flag)
+        return fail(STRIDE_E_UNSUPPORTED, "decoded codes exceed 32 bits (unsupported by the GPU decoder)");
+    }
     return STRIDE_OK;
   }
 
-  DecodeLaunch launch_args(float* V, uint32_t* T, uint32_t m0, uint32_t m1) const {
+  GroupLaunch group_args(float* V, uint32_t* T, uint32_t g0, uint32_t g1, uint32_t* flags) const {
+    GroupLaunch g;
+    g.buf = d_buf;
+    g.tab = d_tab;
+    g.meshes = d_mesh;
+    g.groups = d_groups + g0;
+    g.n_groups = g1 - g0;
+    g.V = V;
+    g.T = T;
+    g.overflow = flags;
+    g.lds_bytes = (uint32_t)cfg.lds_bytes;
+    g.grid = (uint32_t)cfg.grid;
+    return g;
+  }
+
+  DecodeLaunch legacy_args(float* V, uint32_t* T, uint32_t i0, uint32_t i1) const {
     DecodeLaunch d;
     d.buf = d_buf;
-    d.tab = d_tab + m0;
+    d.tab = d_tab;
     d.meshes = d_mesh;
-    d.n_m = m1 - m0;
+    d.idx = cfg.mode == 0 ? nullptr : d_legacy + i0;
+    d.overflow = nullptr;
+    d.n_m = i1 - i0;
     d.V = V;
     d.T = T;
     d.queue = d_queue;
     d.use_queue = cfg.use_queue;
     d.grid = (uint32_t)cfg.grid;
+    if (cfg.mode == 0) {  // index range is a meshlet range
+      d.tab = d_tab + i0;
+    }
     return d;
   }
 
-  int enqueue(float* V, uint32_t* T, hipStream_t s, uint32_t m0, uint32_t m1) {
+  // Enqueue the decode of meshes [mesh0, mesh1).
+  int enqueue(float* V, uint32_t* T, hipStream_t s, int mesh0, int mesh1) {
+    if (cfg.mode == 1) {
+      const uint32_t g0 = g_base[mesh0], g1 = g_base[mesh1];
+      if (g1 > g0) HIP_TRY(launch_group(group_args(V, T, g0, g1, nullptr), s));
+      const uint32_t l0 = l_base[mesh0], l1 = l_base[mesh1];
+      if (l1 > l0) {
+        DecodeConfig c = cfg;
+        c.use_queue = 0;
+        c.grid = 0;
+        DecodeLaunch d = legacy_args(V, T, l0, l1);
+        d.use_queue = 0;
+        d.grid = 0;
+        HIP_TRY(launch_decode(d, c, s));
+      }
+      return STRIDE_OK;
+    }
+    const uint32_t m0 = m_base[mesh0], m1 = m_base[mesh1];
+    if (m1 == m0) return STRIDE_OK;
     if (cfg.use_queue) {
       hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_queue, 0u);
       HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(launch_decode(launch_args(V, T, m0, m1), cfg, s));
+    HIP_TRY(launch_decode(legacy_args(V, T, m0, m1), cfg, s));
     return STRIDE_OK;
   }
 
   int decode(float* V, uint32_t* T, hipStream_t s) {
     if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
-    return enqueue(V, T, s, 0, n_m);
+    return enqueue(V, T, s, 0, (int)mp.size());
   }
 
   int decode_per_mesh(float* V, uint32_t* T, hipStream_t s) {
     if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
-    for (size_t i = 0; i + 1 < m_base.size(); ++i) {
-      if (m_base[i + 1] == m_base[i]) continue;
-      int rc = enqueue(V, T, s, m_base[i], m_base[i + 1]);
+    for (int i = 0; i < (int)mp.size(); ++i) {
+      int rc = enqueue(V, T, s, i, i + 1);
       if (rc) return rc;
     }
     return STRIDE_OK;
@@ -319,14 +496,14 @@ struct DecodeSet {
 
   int decode_graph(float* V, uint32_t* T, hipStream_t s) {
     if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
-    auto key = std::make_tuple((void*)V, (void*)T, cfg.waves, cfg.stage_words, cfg.grid,
-                               cfg.use_queue);
+    auto key = std::make_tuple((void*)V, (void*)T, cfg.mode, cfg.waves, cfg.stage_words,
+                               cfg.grid, cfg.use_queue, cfg.group_size, cfg.lds_bytes);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
       if (!cap_stream) HIP_TRY(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
       hipGraph_t g = nullptr;
       HIP_TRY(hipStreamBeginCapture(cap_stream, hipStreamCaptureModeThreadLocal));
-      int rc = enqueue(V, T, cap_stream, 0, n_m);
+      int rc = enqueue(V, T, cap_stream, 0, (int)mp.size());
       hipError_t ec = hipStreamEndCapture(cap_stream, &g);
       if (rc) {
         if (g) (void)hipGraphDestroy(g);
@@ -347,21 +524,35 @@ struct DecodeSet {
     if (!t) return fail(STRIDE_E_ARG, "null tune");
     bool ok_w = t->waves == 1 || t->waves == 2 || t->waves == 4;
     bool ok_s = t->stage_words == 256 || t->stage_words == 512 || t->stage_words == 1024;
-    if (!ok_w || !ok_s || t->grid < 0 || (t->use_queue && t->grid == 0))
+    bool ok_m = t->mode == 0 || t->mode == 1;
+    bool ok_g = t->group_size >= 1 && t->group_size <= 64;
+    bool ok_l = t->lds_bytes >= 8192 && t->lds_bytes <= 65536 && t->lds_bytes % 8 == 0;
+    if (!ok_w || !ok_s || !ok_m || !ok_g || !ok_l || t->grid < 0 || (t->use_queue && t->grid == 0))
       return fail(STRIDE_E_ARG,
-                  "tune: waves in {1,2,4}, stage_words in {256,512,1024}, grid >= 0, "
+                  "tune: mode in {0,1}, waves in {1,2,4}, stage_words in {256,512,1024}, "
+                  "group_size in [1,64], lds_bytes in [8192,65536] (multiple of 8), grid >= 0, "
                   "use_queue needs a persistent grid");
+    const bool relayout = t->mode != cfg.mode || t->group_size != cfg.group_size ||
+                          t->lds_bytes != cfg.lds_bytes;
     cfg.waves = t->waves;
     cfg.stage_words = t->stage_words;
     cfg.grid = t->grid;
     cfg.use_queue = t->use_queue;
-    return STRIDE_OK;
+    cfg.mode = t->mode;
+    cfg.group_size = t->group_size;
+    cfg.lds_bytes = t->lds_bytes;
+    drop_graphs();
+    return relayout ? layout() : STRIDE_OK;
   }
   void get_tune(stride_tune* t) const {
     t->waves = cfg.waves;
     t->stage_words = cfg.stage_words;
     t->grid = cfg.grid;
     t->use_queue = cfg.use_queue;
+    t->mode = cfg.mode;
+    t->group_size = cfg.group_size;
+    t->lds_bytes = cfg.lds_bytes;
+    t->reserved = (int32_t)n_legacy;
   }
 };
 
@@ -444,8 +635,6 @@ int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris) {
   if (!h || (!verts && h->set.n_v) || (!tris && h->set.n_t))
     return fail(STRIDE_E_ARG, "null argument");
   DecodeSet& s = h->set;
-  if (!s.d_v_own) HIP_TRY(hipMalloc(&s.d_v_own, 12 * std::max<uint64_t>(s.n_v, 1)));
-  if (!s.d_t_own) HIP_TRY(hipMalloc(&s.d_t_own, 12 * std::max<uint64_t>(s.n_t, 1)));
   int rc = s.decode(s.d_v_own, s.d_t_own, nullptr);
   if (rc) return rc;
   HIP_TRY(hipMemcpy(verts, s.d_v_own, 12 * s.n_v, hipMemcpyDeviceToHost));
@@ -456,7 +645,9 @@ int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris) {
 int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes, uint64_t* side_bytes) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
   if (stream_bytes) *stream_bytes = h->set.stream_bytes;
-  if (side_bytes) *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) + sizeof(MeshParams);
+  if (side_bytes)
+    *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) + sizeof(MeshParams) +
+                  (uint64_t)h->set.n_groups * sizeof(GroupEntry) + 4ull * h->set.n_legacy;
   return STRIDE_OK;
 }
 
@@ -520,7 +711,8 @@ int stride_batch_bytes(const stride_batch* h, uint64_t* stream_bytes, uint64_t* 
   if (stream_bytes) *stream_bytes = h->set.stream_bytes;
   if (side_bytes)
     *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) +
-                  (uint64_t)h->set.hdr.size() * sizeof(MeshParams);
+                  (uint64_t)h->set.hdr.size() * sizeof(MeshParams) +
+                  (uint64_t)h->set.n_groups * sizeof(GroupEntry) + 4ull * h->set.n_legacy;
   return STRIDE_OK;
 }
 int stride_batch_set_tune(stride_batch* h, const stride_tune* t) {

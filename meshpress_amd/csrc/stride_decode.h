@@ -10,16 +10,21 @@
 namespace stride {
 
 struct DecodeConfig {
-  int waves = 1;          // waves per workgroup (1, 2, 4); one meshlet per wave
-  int stage_words = 512;  // LDS words staged per meshlet (256, 512, 1024)
-  int grid = 0;           // 0 = one wave per meshlet, else persistent grid (workgroups)
-  int use_queue = 0;      // persistent grid pulls meshlets from an atomic counter
+  int mode = 1;           // 0 = one wave per meshlet, 1 = groups (one lane per meshlet)
+  int waves = 1;          // mode 0: waves per workgroup (1, 2, 4); one meshlet per wave
+  int stage_words = 512;  // mode 0: LDS words staged per meshlet (256, 512, 1024)
+  int grid = 0;           // 0 = one workgroup per unit of work, else persistent grid
+  int use_queue = 0;      // mode 0: persistent grid pulls meshlets from an atomic counter
+  int group_size = 64;    // mode 1: meshlets per group (lanes), 1..64
+  int lds_bytes = 49152;  // mode 1: LDS per group (stage words + packed codes)
 };
 
 struct DecodeLaunch {
   const uint8_t* buf;
   const MeshletEntry* tab;
   const MeshParams* meshes;
+  const uint32_t* idx;    // optional meshlet index list
+  uint32_t* overflow;     // optional: set to 1 when a code exceeds 32 bits
   uint32_t n_m;
   float* V;
   uint32_t* T;
@@ -28,6 +33,20 @@ struct DecodeLaunch {
   uint32_t grid;
 };
 
+struct GroupLaunch {
+  const uint8_t* buf;
+  const MeshletEntry* tab;
+  const MeshParams* meshes;
+  const GroupEntry* groups;
+  uint32_t n_groups;
+  float* V;
+  uint32_t* T;
+  uint32_t* overflow;     // optional, one flag per group
+  uint32_t lds_bytes;
+  uint32_t grid;          // 0 = one workgroup per group
+};
+
+hipError_t launch_group(const GroupLaunch& d, hipStream_t stream);
 hipError_t launch_scan(const uint8_t* buf, const MeshletEntry* seeds, const MeshParams* meshes,
                        uint32_t n_m, MeshletScan* out, hipStream_t stream);
 hipError_t launch_decode(const DecodeLaunch& d, const DecodeConfig& c, hipStream_t stream);

@@ -13,7 +13,10 @@
 // Semantics follow decode_dup (encoder/paradelta_v5_dup.py:471-618).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "stride_bits.h"
+#include "stride_lane.h"
 #include "stride_decode.h"
 #include "stride_device.h"
 
@@ -30,101 +33,9 @@ __global__ void stride_scan_kernel(const uint8_t* __restrict__ buf,
   if (m >= n_m) return;
   const MeshletEntry e = seeds[m];
   const MeshParams& P = meshes[e.mesh];
-  MeshletScan s = {};
-  const uint32_t start = e.start_bit, end = e.end_bit;
-  const uint32_t limit = end - start;
-  GlobalWords src{reinterpret_cast<const uint32_t*>(buf + P.blob_byte), (end + 31u) / 32u + 2u};
-  BitReader<GlobalWords> r;
-  r.init(src, start);
-  auto over = [&]() { return r.pos > end; };
-  uint32_t nl = r.bits(16), nt = r.bits(16), ns = r.bits(16);
-  s.n_local = nl;
-  s.n_tris = nt;
-  s.n_strips = ns;
-  if (nl < 1 || nl > (uint32_t)kMaxLocal || nt < 1 || nt > (uint32_t)kMaxTris || ns < 1 ||
-      ns > nt) {
-    s.error = kScanErrSize;
-    out[m] = s;
-    return;
-  }
-  int ib = 1;
-  while ((1u << ib) < nl + 1) ++ib;
-  Fifo fifo;
-  fifo.clear();
-  Seen256 seen;
-  seen.clear();
-  uint32_t walk = 0, kind0 = 0, tri_total = 0;
-  uint32_t err = kScanOk;
-  auto token = [&](uint32_t& v) -> bool {
-    uint32_t flag = r.bits(1);
-    if (flag == 0) {
-      uint32_t fi = r.bits(5);
-      if (fi >= fifo.n) { err = kScanErrFifo; return false; }
-      v = fifo.hit(fi);
-    } else {
-      v = r.bits(ib);
-      if (v >= nl) { err = kScanErrIndex; return false; }
-      fifo.raw(v);
-    }
-    return true;
-  };
-  for (uint32_t st = 0; st < ns && err == kScanOk; ++st) {
-    uint32_t L = r.bits(16);
-    tri_total += L;
-    if (L < 1 || tri_total > nt) { err = kScanErrStrips; break; }
-    uint32_t p[3] = {0, 0, 0};
-    for (int j = 0; j < 3 && err == kScanOk; ++j) {
-      if (!token(p[j])) break;
-      if (!seen.test_set(p[j])) { walk++; kind0++; }
-    }
-    for (uint32_t t = 1; t < L && err == kScanOk; ++t) {
-      uint32_t ec = r.bits(1);
-      uint32_t v;
-      if (!token(v)) break;
-      uint32_t a = ec ? p[0] : p[1], b = p[2], c = ec ? p[1] : p[0];
-      if (ec == 0) p[0] = p[1];
-      p[1] = p[2];
-      p[2] = v;
-      if (!seen.test_set(v)) {
-        walk++;
-        // the decode kernel marks strip roots with refs (255, 255, 255)
-        if (a == 255 && b == 255 && c == 255) err = kScanErrWalk;
-      }
-      if (over()) err = kScanErrOverrun;
-    }
-    if (over()) err = kScanErrOverrun;
-  }
-  if (err == kScanOk && tri_total != nt) err = kScanErrStrips;
-  if (err == kScanOk && walk != nl) err = kScanErrWalk;
-  if (err != kScanOk) {
-    s.error = err;
-    out[m] = s;
-    return;
-  }
-  s.n_kind0 = kind0;
-  s.resid_bit = r.pos;
-  r.bits(P.g_bits[0]);
-  r.bits(P.g_bits[1]);
-  r.bits(P.g_bits[2]);
-  uint32_t tag[6], kk[6];
-  for (int j = 0; j < 6; ++j) {
-    tag[j] = r.bits(8);
-    kk[j] = r.bits(8);
-  }
-  const uint32_t n_delta = kind0 - 1, n_para = nl - kind0;
-  for (int j = 0; j < 6 && err == kScanOk; ++j) {
-    s.sub_bit[j] = r.pos;
-    uint32_t cnt = j < 3 ? n_delta : n_para;
-    if (cnt == 0) continue;
-    if (kk[j] > 24) { err = kScanErrTag; break; }
-    for (uint32_t i = 0; i < cnt; ++i) {
-      uint64_t u = tag[j] == 1 ? r.rice((int)kk[j], limit) : r.exp_golomb((int)kk[j], limit);
-      if (over() || (u >> 32) != 0) { err = kScanErrOverrun; break; }
-    }
-  }
-  if (err == kScanOk && over()) err = kScanErrOverrun;
-  s.end_bit = r.pos;
-  s.error = err;
+  GlobalWords src{reinterpret_cast<const uint32_t*>(buf + P.blob_byte), (e.end_bit + 31u) / 32u + 2u};
+  MeshletScan s;
+  scan_meshlet(src, e.start_bit, e.end_bit, P.g_bits, s);
   out[m] = s;
 }
 
@@ -211,8 +122,11 @@ __device__ __forceinline__ void resid_phase(const Src& src, const MeshletEntry& 
   const uint32_t cnt = s < 3 ? n_delta : n_para;
   int32_t* dst = L.resid[s % 3] + (s < 3 ? 0 : n_delta);
   const uint32_t limit = e.end_bit - e.start_bit;
+  // select the substream start without indexing a private array by lane
+  const uint32_t sb = s == 0 ? e.sub_bit[0] : s == 1 ? e.sub_bit[1] : s == 2 ? e.sub_bit[2]
+                    : s == 3 ? e.sub_bit[3] : s == 4 ? e.sub_bit[4] : e.sub_bit[5];
   BitReader<Src> r;
-  r.init(src, e.sub_bit[s]);
+  r.init(src, sb);
   if (tag == 1) {
     for (uint32_t i = 0; i < cnt; ++i) dst[i] = (int32_t)unzigzag(r.rice(k, limit));
   } else {
@@ -220,18 +134,14 @@ __device__ __forceinline__ void resid_phase(const Src& src, const MeshletEntry& 
   }
 }
 
-__device__ __forceinline__ int32_t irlp(int32_t a, int32_t b, int32_t c, int32_t n0, int32_t n1,
-                                        int32_t n2, int32_t K) {
-  int64_t s = (int64_t)n0 * a + (int64_t)n1 * b + (int64_t)n2 * c;
-  if (K != 0) s = (s + ((int64_t)1 << (K - 1))) >> K;
-  return (int32_t)s;
-}
+// irlp64 / fits32: stride_lane.h
 
 // Phase C (lane 0): walk-order reconstruction (paradelta_v5_dup.py:581-600).
 template <int SW>
-__device__ __forceinline__ void chain_phase(const MeshletEntry& e, const MeshParams& P,
+__device__ __forceinline__ bool chain_phase(const MeshletEntry& e, const MeshParams& P,
                                             int32_t ax, int32_t ay, int32_t az,
                                             WaveLds<SW>& L) {
+  bool ok = true;
   const uint32_t n_delta = e.n_kind0 - 1;
   int32_t px = 0, py = 0, pz = 0;
   uint32_t cd = 0, cp = 0;
@@ -248,7 +158,9 @@ __device__ __forceinline__ void chain_phase(const MeshletEntry& e, const MeshPar
         cx = ax; cy = ay; cz = az;
         first = false;
       } else {
-        cx = px + rx[cd]; cy = py + ry[cd]; cz = pz + rz[cd];
+        int64_t x = (int64_t)px + rx[cd], y = (int64_t)py + ry[cd], z = (int64_t)pz + rz[cd];
+        ok &= fits32(x) & fits32(y) & fits32(z);
+        cx = (int32_t)x; cy = (int32_t)y; cz = (int32_t)z;
         ++cd;
       }
     } else {
@@ -256,14 +168,17 @@ __device__ __forceinline__ void chain_phase(const MeshletEntry& e, const MeshPar
       int4 B = L.codes[(pk >> 16) & 0xFFu];
       int4 C = L.codes[pk >> 24];
       uint32_t slot = (n_delta + cp) & (kMaxLocal - 1);
-      cx = irlp(A.x, B.x, C.x, P.pred_n[0], P.pred_n[1], P.pred_n[2], P.pred_K[0]) + rx[slot];
-      cy = irlp(A.y, B.y, C.y, P.pred_n[3], P.pred_n[4], P.pred_n[5], P.pred_K[1]) + ry[slot];
-      cz = irlp(A.z, B.z, C.z, P.pred_n[6], P.pred_n[7], P.pred_n[8], P.pred_K[2]) + rz[slot];
+      int64_t x = irlp64(A.x, B.x, C.x, P.pred_n[0], P.pred_n[1], P.pred_n[2], P.pred_K[0]) + rx[slot];
+      int64_t y = irlp64(A.y, B.y, C.y, P.pred_n[3], P.pred_n[4], P.pred_n[5], P.pred_K[1]) + ry[slot];
+      int64_t z = irlp64(A.z, B.z, C.z, P.pred_n[6], P.pred_n[7], P.pred_n[8], P.pred_K[2]) + rz[slot];
+      ok &= fits32(x) & fits32(y) & fits32(z);
+      cx = (int32_t)x; cy = (int32_t)y; cz = (int32_t)z;
       ++cp;
     }
     L.codes[v] = make_int4(cx, cy, cz, 0);
     px = cx; py = cy; pz = cz;
   }
+  return ok;
 }
 
 // decode_dup dequantisation, in FP64 with the reference's operation order:
@@ -284,8 +199,9 @@ __device__ __forceinline__ float dequant(int32_t code, float gmin, float grange,
 template <int WAVES, int SW>
 __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
-    const MeshParams* __restrict__ meshes, uint32_t n_m, float* __restrict__ V,
-    uint32_t* __restrict__ T, uint32_t* __restrict__ queue, int use_queue) {
+    const MeshParams* __restrict__ meshes, const uint32_t* __restrict__ idx, uint32_t n_m,
+    float* __restrict__ V, uint32_t* __restrict__ T, uint32_t* __restrict__ queue,
+    int use_queue, uint32_t* __restrict__ overflow) {
   __shared__ WaveLds<SW> lds_all[WAVES];
   const int lane = threadIdx.x & 63;
   const int wib = threadIdx.x >> 6;
@@ -298,7 +214,7 @@ __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
     m = __builtin_amdgcn_readfirstlane(t);
   }
   while (m < n_m) {
-    const MeshletEntry e = tab[m];
+    const MeshletEntry e = tab[idx ? idx[m] : m];
     const MeshParams P = meshes[e.mesh];
     const uint32_t* blob = reinterpret_cast<const uint32_t*>(buf + P.blob_byte);
     const uint32_t w0 = e.start_bit >> 5;
@@ -318,7 +234,7 @@ __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
       el.end_bit -= base;
 #pragma unroll
       for (int j = 0; j < 6; ++j) el.sub_bit[j] -= base;
-      if (lane == 0) conn_phase(src, el.start_bit, e.n_local, e.n_strips, L);
+      if (lane == 0) conn_phase(src, el.start_bit, e.n_local, e.n_strips & kStripMask, L);
       if (lane < 6) resid_phase(src, el, lane, gsum, L);
       if (lane == 6) {
         BitReader<LdsWords> r;
@@ -329,7 +245,7 @@ __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
       }
     } else {
       GlobalWords src{blob, ((e.end_bit + 31u) >> 5) + 2u};
-      if (lane == 0) conn_phase(src, e.start_bit, e.n_local, e.n_strips, L);
+      if (lane == 0) conn_phase(src, e.start_bit, e.n_local, e.n_strips & kStripMask, L);
       if (lane < 6) resid_phase(src, e, lane, gsum, L);
       if (lane == 6) {
         BitReader<GlobalWords> r;
@@ -341,7 +257,7 @@ __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
     }
     wave_sync();
     if (lane == 0) {
-      chain_phase(e, P, L.anchor[0], L.anchor[1], L.anchor[2], L);
+      if (!chain_phase(e, P, L.anchor[0], L.anchor[1], L.anchor[2], L) && overflow) *overflow = 1u;
     }
     wave_sync();
     // Phase D: coalesced dword streams of xyz floats and uint32 indices.
@@ -377,6 +293,74 @@ __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Group decode: one wave per group of consecutive meshlets, one lane per
+// meshlet; the per-lane decoder is decode_lane (stride_lane.h).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t& total) {
+  const int lane = threadIdx.x & 63;
+  uint32_t s = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t t = __shfl_up(s, o, 64);
+    if (lane >= o) s += t;
+  }
+  total = __shfl(s, 63, 64);
+  return s - x;
+}
+
+__global__ __launch_bounds__(64) void stride_decode_group_kernel(
+    const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
+    const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
+    uint32_t n_groups, float* __restrict__ V, uint32_t* __restrict__ T,
+    uint32_t* __restrict__ overflow) {
+  extern __shared__ uint64_t lds64[];
+  uint32_t* stage = reinterpret_cast<uint32_t*>(lds64);
+  const int lane = threadIdx.x;
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
+  for (uint32_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+    const GroupEntry G = groups[g];
+    for (uint32_t i = lane; i < G.n_words; i += 64) stage[i] = __builtin_bswap32(words[G.w_lo + i]);
+    uint64_t* codes = lds64 + G.codes_off;
+    const uint32_t m = G.m0 + lane;
+    const bool active = m < G.m1;
+    MeshletEntry e;
+    if (active) e = tab[m];
+    uint32_t total = 0;
+    const uint32_t coff = wave_excl_scan(active ? e.n_local : 0u, total);
+    for (uint32_t i = lane; i < total; i += 64) codes[i] = kUnseen;
+    __syncthreads();
+    if (active) {
+      const MeshParams& P = meshes[e.mesh];
+      const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
+      const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
+      uint32_t* tout = T + 3 * (P.t_base + e.t_off);
+      if (!decode_lane(stage, G.n_words, rel, e, P, codes + coff, tout) && overflow)
+        overflow[g] = 1u;
+    }
+    __syncthreads();
+    // dequantise the group's vertices, one meshlet at a time, coalesced
+    uint32_t cbase = 0;
+    for (uint32_t j = G.m0; j < G.m1; ++j) {
+      const MeshletEntry ej = tab[j];
+      const MeshParams& Pj = meshes[ej.mesh];
+      float* vo = V + 3 * (Pj.v_base + ej.v_off);
+      const uint32_t nv3 = 3 * ej.n_local;
+      for (uint32_t i = lane; i < nv3; i += 64) {
+        const uint32_t vtx = i / 3, a = i - 3 * vtx;
+        const int32_t code = field21(codes[cbase + vtx], (int)a);
+        const float gmin = a == 0 ? Pj.g_min[0] : (a == 1 ? Pj.g_min[1] : Pj.g_min[2]);
+        const float grng = a == 0 ? Pj.g_range[0] : (a == 1 ? Pj.g_range[1] : Pj.g_range[2]);
+        const int gb = a == 0 ? Pj.g_bits[0] : (a == 1 ? Pj.g_bits[1] : Pj.g_bits[2]);
+        const float ce = a == 0 ? Pj.center[0] : (a == 1 ? Pj.center[1] : Pj.center[2]);
+        vo[i] = dequant(code, gmin, grng, gb, Pj.scale, ce);
+      }
+      cbase += ej.n_local;
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_scan(const uint8_t* buf, const MeshletEntry* seeds, const MeshParams* meshes,
@@ -392,7 +376,16 @@ template <int WAVES, int SW>
 static hipError_t launch_decode_t(const DecodeLaunch& d, hipStream_t stream) {
   const uint32_t grid = d.grid ? d.grid : (d.n_m + WAVES - 1) / WAVES;
   hipLaunchKernelGGL((stride_decode_kernel<WAVES, SW>), dim3(grid), dim3(64 * WAVES), 0, stream,
-                     d.buf, d.tab, d.meshes, d.n_m, d.V, d.T, d.queue, d.use_queue);
+                     d.buf, d.tab, d.meshes, d.idx, d.n_m, d.V, d.T, d.queue, d.use_queue,
+                     d.overflow);
+  return hipGetLastError();
+}
+
+hipError_t launch_group(const GroupLaunch& d, hipStream_t stream) {
+  if (d.n_groups == 0) return hipSuccess;
+  const uint32_t grid = d.grid ? std::min(d.grid, d.n_groups) : d.n_groups;
+  hipLaunchKernelGGL(stride_decode_group_kernel, dim3(grid), dim3(64), d.lds_bytes, stream, d.buf,
+                     d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
   return hipGetLastError();
 }
 

@@ -47,7 +47,8 @@ struct MeshletScan {
   uint32_t sub_bit[6];
   uint32_t end_bit;     // bit just after the last residual read
   uint32_t error;       // 0 = ok, see kScanErr*
-  uint32_t pad[3];
+  uint32_t flags;       // kFlag*
+  uint32_t pad[2];
 };
 static_assert(sizeof(MeshletScan) == 64, "MeshletScan is 64 bytes");
 
@@ -61,6 +62,21 @@ enum : uint32_t {
   kScanErrWalk = 6,       // walk does not cover every local vertex
   kScanErrTag = 7,        // residual coder parameter k > 24 (unsupported)
 };
+
+// MeshletEntry::n_strips high bits
+constexpr uint32_t kFlagRawInFifo = 1u << 16;  // a raw token names an id already in the FIFO
+constexpr uint32_t kStripMask = 0xFFFFu;
+
+// A group of consecutive meshlets of one mesh decoded by one wave, one lane
+// per meshlet; its bit range is staged in LDS as one contiguous word run.
+struct GroupEntry {
+  uint32_t m0, m1;        // meshlet range [m0, m1)
+  uint32_t w_lo;          // first staged word, absolute in the device buffer
+  uint32_t n_words;       // staged words
+  uint32_t codes_off;     // LDS offset of the packed codes, in 8-byte units
+  uint32_t pad[3];
+};
+static_assert(sizeof(GroupEntry) == 32, "GroupEntry is 32 bytes");
 
 constexpr int kMaxLocal = 256;   // vertices per meshlet (max_verts cap)
 constexpr int kMaxTris = 256;    // triangles per meshlet (max_tris cap)

@@ -1,0 +1,339 @@
+// Per-lane STRIDE decode logic shared by the gfx950 kernels and the CPU
+// harness that checks it against the oracle (compiled as host C++ there).
+#pragma once
+
+#include <cstdint>
+
+#include "stride_bits.h"
+#include "stride_device.h"
+
+namespace stride {
+
+// Integer-rational predictor (paradelta_v5_dup.py:91-104): floor shift.
+STRIDE_DEV int64_t irlp64(int64_t a, int64_t b, int64_t c, int32_t n0, int32_t n1,
+                                          int32_t n2, int32_t K) {
+  int64_t s = (int64_t)n0 * a + (int64_t)n1 * b + (int64_t)n2 * c;
+  if (K != 0) s = (s + ((int64_t)1 << (K - 1))) >> K;
+  return s;
+}
+
+STRIDE_DEV bool fits32(int64_t v) { return v == (int64_t)(int32_t)v; }
+
+// ---------------------------------------------------------------------------
+// Group decode: one wave per group of consecutive meshlets, one lane per
+// meshlet.  Each lane runs the whole serial decode of its meshlet (strip
+// tokens, reuse FIFO, residual codewords, IRLP chain) in a single token
+// loop, so one wave instruction advances up to 64 meshlets.  Codes live in
+// LDS as three 21-bit fields per vertex (bit 63 marks "not decoded yet"),
+// so the seen test and the code lookup are one 8-byte LDS read.
+// ---------------------------------------------------------------------------
+constexpr uint64_t kUnseen = 1ull << 63;
+constexpr int64_t kCodeLim = 1 << 20;   // packed field range [-2^20, 2^20)
+
+STRIDE_DEV uint64_t pack21(int64_t x, int64_t y, int64_t z) {
+  const uint64_t m = (1ull << 21) - 1;
+  return ((uint64_t)x & m) | (((uint64_t)y & m) << 21) | (((uint64_t)z & m) << 42);
+}
+STRIDE_DEV int32_t field21(uint64_t c, int f) {
+  return (int32_t)((int64_t)(c << (43 - 21 * f)) >> 43);
+}
+STRIDE_DEV bool fits21(int64_t v) { return v >= -kCodeLim && v < kCodeLim; }
+
+// 64 bits of the staged stream starting at bit p (at least 33 valid).
+STRIDE_DEV uint64_t win64(const uint32_t* __restrict__ st, uint32_t p) {
+  const uint32_t w = p >> 5;
+  const uint64_t x = ((uint64_t)st[w] << 32) | st[w + 1];
+  return x << (p & 31);
+}
+
+// Codeword at bit p: Rice(k) (eg == false) or Exp-Golomb(k), zigzag
+// decoded; p advances past it.  Codewords of at most 33 bits come from one
+// window; longer ones (only for outliers) take the bit-serial path.
+STRIDE_DEV int64_t codeword(const uint32_t* __restrict__ st, uint32_t n_words,
+                                            uint32_t& p, bool eg, int k, bool needed) {
+  const uint64_t W = win64(st, p);
+  const uint32_t q = W ? (uint32_t)__builtin_clzll(W) : 64u;
+  const uint32_t extra = eg ? q : 0u;
+  const uint32_t nb = extra + (uint32_t)k;
+  const uint32_t len = q + 1 + nb;
+  uint64_t u;
+  if (len <= 33) {
+    const uint64_t body = nb ? ((W << (q + 1)) >> (64 - nb)) : 0ull;
+    const uint64_t off = body >> k;
+    const uint64_t low = body & ((1ull << k) - 1);
+    const uint64_t shifted = eg ? (((1ull << q) - 1) + off) : (uint64_t)q;
+    u = (shifted << k) | low;
+    p += len;
+  } else if (needed) {
+    BitReader<LdsWords> r;
+    r.init(LdsWords{st, n_words}, p);
+    u = eg ? r.exp_golomb(k, 0xFFFFFFu) : r.rice(k, 0xFFFFFFu);
+    p = r.pos;
+  } else {
+    u = 0;
+  }
+  return unzigzag(u);
+}
+
+// Serial decode of one meshlet by one lane (decode_dup's walk fused with
+// the token stream).  Returns false when a code leaves the packed 21-bit
+// range (the group is then decoded by the wave-per-meshlet kernel).
+STRIDE_DEV bool decode_lane(const uint32_t* __restrict__ st, uint32_t n_words,
+                                            uint32_t rel, const MeshletEntry& e,
+                                            const MeshParams& P, uint64_t* __restrict__ codes,
+                                            uint32_t* __restrict__ tout) {
+  const uint32_t n_local = e.n_local, n_tris = e.n_tris;
+  const uint32_t n_strips = e.n_strips & kStripMask;
+  const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
+  // anchor and the six (tag, k) bytes
+  uint32_t hp = rel + (e.resid_bit - e.start_bit);
+  int32_t anc[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    const int gb = P.g_bits[d];
+    anc[d] = (int32_t)(win64(st, hp) >> (64 - gb));
+    hp += gb;
+  }
+  const uint64_t TK = win64(st, hp);            // 32 bits: dx dy
+  const uint64_t TK2 = win64(st, hp + 32);      // 32 bits: dz px
+  const uint64_t TK3 = win64(st, hp + 64);      // 32 bits: py pz
+  auto tagk = [&](int j, bool& eg, int& k) {
+    const uint64_t src = j < 2 ? TK : (j < 4 ? TK2 : TK3);
+    const int sh = 48 - 16 * (j & 1);
+    eg = ((src >> (sh + 8)) & 0xFF) != 1;
+    k = (int)((src >> sh) & 31);
+  };
+  bool deg[3], peg[3];
+  int dk[3], pk[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    tagk(d, deg[d], dk[d]);
+    tagk(3 + d, peg[d], pk[d]);
+  }
+  uint32_t pd[3], pp[3];
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    pd[d] = rel + (e.sub_bit[d] - e.start_bit);
+    pp[d] = rel + (e.sub_bit[3 + d] - e.start_bit);
+  }
+  uint32_t pc = rel + 48;
+  int ib = 1;
+  while ((1u << ib) < n_local + 1) ++ib;
+  Mtf fifo;
+  fifo.clear();
+  bool ok = true, first = true;
+  int32_t prv[3] = {0, 0, 0};                // previous walk entry
+  uint32_t q0 = 0, q1 = 0, q2 = 0;            // previous triangle ids
+  int32_t c0[3] = {0, 0, 0}, c1[3] = {0, 0, 0}, c2[3] = {0, 0, 0};
+  uint32_t root_j = 3, steps_left = 0, tc = 0;
+  const uint32_t v_off = e.v_off;
+  const uint32_t n_tok = n_tris + 2 * n_strips;
+  for (uint32_t it = 0; it < n_tok; ++it) {
+    // strip header (when a strip starts), edge bit, token: one window
+    uint64_t W = win64(st, pc);
+    const bool start = root_j == 3 && steps_left == 0;
+    const uint32_t L = (uint32_t)(W >> 48);
+    W = start ? (W << 16) : W;
+    steps_left = start ? L - 1 : steps_left;
+    root_j = start ? 0u : root_j;
+    const bool is_root = root_j < 3;
+    const uint32_t ec = is_root ? 0u : (uint32_t)(W >> 63);
+    W = is_root ? W : (W << 1);
+    const uint32_t flag = (uint32_t)(W >> 63);
+    W <<= 1;
+    const uint32_t val = flag ? (uint32_t)(W >> (64 - ib)) : (uint32_t)(W >> 59);
+    pc += (start ? 16u : 0u) + (is_root ? 0u : 1u) + 1u + (flag ? (uint32_t)ib : 5u);
+    // reuse FIFO: a hit names FIFO index val (oldest first), a raw id is
+    // appended (the oldest entry leaving a full FIFO); either way the value
+    // becomes the most recent entry
+    const bool hit = flag == 0;
+    uint32_t j = hit ? ((fifo.n - 1 - (val & 15u)) & 15u) : fifo.n;
+    const uint32_t v = hit ? fifo.rank(j) : (val & 0xFFu);
+    if (raw_in_fifo && !hit) {
+      const int pf = fifo.find(v);
+      j = pf >= 0 ? (uint32_t)pf : j;
+    }
+    fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
+    fifo.to_front(j, v);
+    // code of v: stored, or decoded now from its residual substreams
+    const uint64_t cv = codes[v];
+    const bool fresh = (cv & kUnseen) != 0;
+    const bool para = !is_root;
+    const bool anchor = is_root && first;
+    const bool need = fresh && !anchor;
+    int32_t code[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      uint32_t pos = para ? pp[d] : pd[d];
+      const int64_t r = codeword(st, n_words, pos, para ? peg[d] : deg[d], para ? pk[d] : dk[d],
+                                 need);
+      pp[d] = (need && para) ? pos : pp[d];
+      pd[d] = (need && !para) ? pos : pd[d];
+      // refs (a, b, c): ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
+      const int32_t ca = ec ? c0[d] : c1[d], cc = ec ? c1[d] : c0[d];
+      const int64_t pred = irlp64(ca, c2[d], cc, P.pred_n[3 * d], P.pred_n[3 * d + 1],
+                                  P.pred_n[3 * d + 2], P.pred_K[d]);
+      const int64_t base = para ? pred : (anchor ? (int64_t)anc[d] : (int64_t)prv[d]);
+      const int64_t cnew = base + (anchor ? 0 : r);
+      ok &= !fresh || fits21(cnew);
+      code[d] = fresh ? (int32_t)cnew : field21(cv, d);
+    }
+    if (fresh) {
+      codes[v] = pack21(code[0], code[1], code[2]);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) prv[d] = code[d];
+      first = false;
+    }
+    // triangle (a, b, v) and the next previous triangle
+    uint32_t ta, tb;
+    if (is_root) {
+      ta = q0;
+      tb = q1;
+      if (root_j == 0) {
+        q0 = v;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c0[d] = code[d];
+      } else if (root_j == 1) {
+        q1 = v;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c1[d] = code[d];
+        tb = v;
+      } else {
+        q2 = v;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c2[d] = code[d];
+      }
+      ++root_j;
+    } else {
+      ta = ec ? q0 : q1;
+      tb = q2;
+      if (ec == 0) {
+        q0 = q1;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) c0[d] = c1[d];
+      }
+      q1 = q2;
+      q2 = v;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        c1[d] = c2[d];
+        c2[d] = code[d];
+      }
+      --steps_left;
+    }
+    if (root_j == 3 || !is_root) {
+      uint32_t* o = tout + 3 * (tc & (kMaxTris - 1));
+      o[0] = v_off + ta;
+      o[1] = v_off + tb;
+      o[2] = v_off + v;
+      ++tc;
+    }
+  }
+  return ok;
+}
+
+
+// Index pre-pass for one meshlet (one lane): parses the whole meshlet once,
+// derives the side-table fields the stream does not store, and validates
+// every field against the meshlet's bit range [start, end).
+template <class Src>
+STRIDE_DEV void scan_meshlet(const Src& src, uint32_t start, uint32_t end, const int32_t* g_bits,
+                             MeshletScan& s) {
+  s = MeshletScan{};
+  const uint32_t limit = end - start;
+  BitReader<Src> r;
+  r.init(src, start);
+  auto over = [&]() { return r.pos > end; };
+  uint32_t nl = r.bits(16), nt = r.bits(16), ns = r.bits(16);
+  s.n_local = nl;
+  s.n_tris = nt;
+  s.n_strips = ns;
+  if (nl < 1 || nl > (uint32_t)kMaxLocal || nt < 1 || nt > (uint32_t)kMaxTris || ns < 1 ||
+      ns > nt) {
+    s.error = kScanErrSize;
+    return;
+  }
+  int ib = 1;
+  while ((1u << ib) < nl + 1) ++ib;
+  Fifo fifo;
+  fifo.clear();
+  Seen256 seen;
+  seen.clear();
+  uint32_t walk = 0, kind0 = 0, tri_total = 0;
+  uint32_t err = kScanOk;
+  uint32_t flags = 0;
+  auto token = [&](uint32_t& v) -> bool {
+    uint32_t flag = r.bits(1);
+    if (flag == 0) {
+      uint32_t fi = r.bits(5);
+      if (fi >= fifo.n) { err = kScanErrFifo; return false; }
+      v = fifo.hit(fi);
+    } else {
+      v = r.bits(ib);
+      if (v >= nl) { err = kScanErrIndex; return false; }
+      if (fifo.find(v) >= 0) flags |= kFlagRawInFifo;
+      fifo.raw(v);
+    }
+    return true;
+  };
+  for (uint32_t st = 0; st < ns && err == kScanOk; ++st) {
+    uint32_t L = r.bits(16);
+    tri_total += L;
+    if (L < 1 || tri_total > nt) { err = kScanErrStrips; break; }
+    uint32_t p[3] = {0, 0, 0};
+    for (int j = 0; j < 3 && err == kScanOk; ++j) {
+      if (!token(p[j])) break;
+      if (!seen.test_set(p[j])) { walk++; kind0++; }
+    }
+    for (uint32_t t = 1; t < L && err == kScanOk; ++t) {
+      uint32_t ec = r.bits(1);
+      uint32_t v;
+      if (!token(v)) break;
+      uint32_t a = ec ? p[0] : p[1], b = p[2], c = ec ? p[1] : p[0];
+      if (ec == 0) p[0] = p[1];
+      p[1] = p[2];
+      p[2] = v;
+      if (!seen.test_set(v)) {
+        walk++;
+        // the decode kernel marks strip roots with refs (255, 255, 255)
+        if (a == 255 && b == 255 && c == 255) err = kScanErrWalk;
+      }
+      if (over()) err = kScanErrOverrun;
+    }
+    if (over()) err = kScanErrOverrun;
+  }
+  if (err == kScanOk && tri_total != nt) err = kScanErrStrips;
+  if (err == kScanOk && walk != nl) err = kScanErrWalk;
+  if (err != kScanOk) {
+    s.error = err;
+    return;
+  }
+  s.n_kind0 = kind0;
+  s.resid_bit = r.pos;
+  r.bits(g_bits[0]);
+  r.bits(g_bits[1]);
+  r.bits(g_bits[2]);
+  uint32_t tag[6], kk[6];
+  for (int j = 0; j < 6; ++j) {
+    tag[j] = r.bits(8);
+    kk[j] = r.bits(8);
+  }
+  const uint32_t n_delta = kind0 - 1, n_para = nl - kind0;
+  for (int j = 0; j < 6 && err == kScanOk; ++j) {
+    s.sub_bit[j] = r.pos;
+    uint32_t cnt = j < 3 ? n_delta : n_para;
+    if (cnt == 0) continue;
+    if (kk[j] > 24) { err = kScanErrTag; break; }
+    for (uint32_t i = 0; i < cnt; ++i) {
+      uint64_t u = tag[j] == 1 ? r.rice((int)kk[j], limit) : r.exp_golomb((int)kk[j], limit);
+      if (over() || (u >> 32) != 0) { err = kScanErrOverrun; break; }
+    }
+  }
+  if (err == kScanOk && over()) err = kScanErrOverrun;
+  s.end_bit = r.pos;
+  s.error = err;
+  s.flags = flags;
+}
+
+
+}  // namespace stride

@@ -89,13 +89,10 @@ class ParaDeltaV5GpuDecoder:
         _lib.check(self._L.stride_decoder_get_tune(self._h, C.byref(t)))
         return t.as_dict()
 
-    def set_tune(self, waves=None, stage_words=None, grid=None, use_queue=None) -> None:
-        cur = self.get_tune()
-        t = _lib.StrideTune(
-            waves if waves is not None else cur["waves"],
-            stage_words if stage_words is not None else cur["stage_words"],
-            grid if grid is not None else cur["grid"],
-            use_queue if use_queue is not None else cur["use_queue"])
+    def set_tune(self, **kw) -> None:
+        """Launch shape: mode (1 = lane-per-meshlet groups, 0 = wave per
+        meshlet), group_size, lds_bytes, waves, stage_words, grid, use_queue."""
+        t = _lib.make_tune(self.get_tune(), kw)
         _lib.check(self._L.stride_decoder_set_tune(self._h, C.byref(t)))
 
     # --------------------------------------------------------------- decode

@@ -1,0 +1,112 @@
+// Host build of the per-lane decode logic of the gfx950 kernels
+// (meshpress_amd/csrc/stride_lane.h): the index pre-pass (scan_meshlet) and
+// the lane decoder (decode_lane) run here on the CPU, under the sanitizers
+// when built with them, and their output is compared with the oracle.
+//
+//   lane_check <stream.dpr4> <verts.f32 out> <tris.u32 out>
+//
+// Exit status: 0 ok, 2 stream rejected by the pre-pass, 3 code overflow
+// (the kernels route such groups to the wave-per-meshlet decoder).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "../meshpress_amd/csrc/stride_lane.h"
+
+using namespace stride;
+
+static uint32_t be32(const uint8_t* p) {
+  return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+
+int main(int argc, char** argv) {
+  if (argc < 4) return 1;
+  FILE* f = std::fopen(argv[1], "rb");
+  if (!f) return 1;
+  std::vector<uint8_t> data;
+  uint8_t buf[65536];
+  size_t r;
+  while ((r = std::fread(buf, 1, sizeof buf, f)) > 0) data.insert(data.end(), buf, buf + r);
+  std::fclose(f);
+  if (data.size() < 86) return 2;
+  MeshParams P;
+  std::memset(&P, 0, sizeof P);
+  const uint8_t* d = data.data();
+  auto f32 = [&](int o) {
+    uint32_t u = be32(d + o);
+    float x;
+    std::memcpy(&x, &u, 4);
+    return x;
+  };
+  for (int i = 0; i < 3; ++i) {
+    P.center[i] = f32(6 + 4 * i);
+    P.g_min[i] = f32(26 + 4 * i);
+    P.g_range[i] = f32(38 + 4 * i);
+    P.g_bits[i] = d[50 + i];
+    P.pred_K[i] = d[71 + i];
+  }
+  P.scale = f32(18);
+  for (int i = 0; i < 9; ++i) {
+    uint32_t u = ((uint32_t)d[53 + 2 * i] << 8) | d[54 + 2 * i];
+    P.pred_n[i] = (u & 0x8000u) ? (int32_t)u - 0x10000 : (int32_t)u;
+  }
+  const uint32_t n_m = be32(d + 82);
+  const size_t n_words = (data.size() + 3) / 4 + 16;
+  std::vector<uint32_t> le(n_words, 0), st(n_words, 0);
+  std::memcpy(le.data(), d, data.size());
+  for (size_t i = 0; i < n_words; ++i) st[i] = __builtin_bswap32(le[i]);
+  std::vector<MeshletEntry> tab(n_m);
+  uint64_t nv = 0, nt = 0;
+  for (uint32_t m = 0; m < n_m; ++m) {
+    const uint32_t start = be32(d + 86 + 4 * m);
+    const uint32_t end = m + 1 < n_m ? be32(d + 86 + 4 * (m + 1)) : (uint32_t)(8 * data.size());
+    MeshletScan s;
+    GlobalWords src{le.data(), (end + 31u) / 32u + 2u};
+    scan_meshlet(src, start, end, P.g_bits, s);
+    if (s.error || (m + 1 < n_m && s.end_bit != end)) {   // as the host validation
+      std::fprintf(stderr, "meshlet %u rejected (%u)\n", m, s.error);
+      return 2;
+    }
+    MeshletEntry& e = tab[m];
+    e.start_bit = start;
+    e.v_off = (uint32_t)nv;
+    e.t_off = (uint32_t)nt;
+    e.n_local = s.n_local;
+    e.n_tris = s.n_tris;
+    e.n_strips = s.n_strips | (s.flags & kFlagRawInFifo);
+    e.n_kind0 = s.n_kind0;
+    e.resid_bit = s.resid_bit;
+    for (int j = 0; j < 6; ++j) e.sub_bit[j] = s.sub_bit[j];
+    e.end_bit = s.end_bit;
+    nv += s.n_local;
+    nt += s.n_tris;
+  }
+  std::vector<uint64_t> codes(nv, kUnseen);
+  std::vector<uint32_t> tris(3 * nt, 0xFFFFFFFFu);
+  int rc = 0;
+  for (uint32_t m = 0; m < n_m; ++m) {
+    const MeshletEntry& e = tab[m];
+    // one group = the whole stream staged: rel is the absolute start bit
+    if (!decode_lane(st.data(), (uint32_t)n_words, e.start_bit, e, P, codes.data() + e.v_off,
+                     tris.data() + 3ull * e.t_off))
+      rc = 3;
+  }
+  std::vector<float> V(3 * nv);
+  for (uint64_t i = 0; i < nv; ++i)
+    for (int a = 0; a < 3; ++a) {
+      const int32_t c = field21(codes[i], a);
+      const uint64_t mx = (1ull << P.g_bits[a]) - 1;
+      const double vn = mx == 0 ? (double)P.g_min[a]
+                                : (double)P.g_min[a] + ((double)c / (double)mx) * (double)P.g_range[a];
+      V[3 * i + a] = (float)(vn * (double)P.scale + (double)P.center[a]);
+    }
+  FILE* fv = std::fopen(argv[2], "wb");
+  FILE* ft = std::fopen(argv[3], "wb");
+  if (!fv || !ft) return 1;
+  std::fwrite(V.data(), 4, V.size(), fv);
+  std::fwrite(tris.data(), 4, tris.size(), ft);
+  std::fclose(fv);
+  std::fclose(ft);
+  return rc;
+}

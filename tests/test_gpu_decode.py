@@ -62,10 +62,16 @@ def test_small_meshes(oracle, pred):
         assert_same(Vg, Tg, Vo, To)
 
 
-TUNES = [dict(waves=w, stage_words=s) for w in (1, 2, 4) for s in (256, 512, 1024)] + \
-        [dict(waves=4, stage_words=512, grid=64, use_queue=0),
-         dict(waves=4, stage_words=512, grid=64, use_queue=1),
-         dict(waves=1, stage_words=256, grid=7, use_queue=1)]
+TUNES = [dict(mode=0, waves=w, stage_words=s) for w in (1, 2, 4) for s in (256, 512, 1024)] + \
+        [dict(mode=0, waves=4, stage_words=512, grid=64, use_queue=0),
+         dict(mode=0, waves=4, stage_words=512, grid=64, use_queue=1),
+         dict(mode=0, waves=1, stage_words=256, grid=7, use_queue=1),
+         dict(mode=1, group_size=64, lds_bytes=49152),
+         dict(mode=1, group_size=64, lds_bytes=65536, grid=5),
+         dict(mode=1, group_size=7, lds_bytes=16384),
+         dict(mode=1, group_size=1, lds_bytes=8192),
+         # too small for most meshlets: exercises the wave-per-meshlet fallback list
+         dict(mode=1, group_size=32, lds_bytes=8192, stage_words=256)]
 
 
 @pytest.mark.parametrize("tune", TUNES, ids=lambda t: "-".join(f"{k}{v}" for k, v in t.items()))
@@ -157,7 +163,8 @@ def test_batch_decode_matches_single(coracle):
     blobs += [hand_stream_small()[0], hand_stream_fifo()[0]]
     want = [coracle.decode(b) for b in blobs]
     bd = BatchDecoder(blobs)
-    for tune in (dict(), dict(waves=4, stage_words=512, grid=128, use_queue=1)):
+    for tune in (dict(), dict(mode=0, waves=4, stage_words=512, grid=128, use_queue=1),
+                 dict(mode=1, group_size=16, lds_bytes=16384, grid=0, use_queue=0)):
         if tune:
             bd.set_tune(**tune)
         for mode in ("launch", "per_mesh", "graph", "graph", "graph"):

@@ -160,3 +160,72 @@ def test_encoder_argument_errors():
         prepare(V, T, gen_method="meshopt")
     with pytest.raises(IndexError):      # numpy indexing, as the reference front-end
         prepare(V, np.array([[0, 1, 99999]]))
+
+
+# ------------------------------------------------- kernel logic on the CPU
+LANE_SRC = ROOT / "tests" / "lane_check.cpp"
+
+
+@pytest.fixture(scope="module")
+def lane_check(tmp_path_factory):
+    """Host build (g++, ASan + UBSan) of the kernels' per-lane logic."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path_factory.mktemp("lane") / "lane_check"
+    subprocess.run([gxx, "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", "-ffp-contract=off", str(LANE_SRC),
+                    "-o", str(exe)], check=True)
+    return exe
+
+
+def run_lane_check(exe, data, tmp_path):
+    import subprocess
+    src = tmp_path / "s.dpr4"
+    src.write_bytes(data)
+    fv, ft = tmp_path / "v.f32", tmp_path / "t.u32"
+    r = subprocess.run([str(exe), str(src), str(fv), str(ft)], capture_output=True, text=True)
+    return r.returncode, np.fromfile(fv, np.float32).reshape(-1, 3) if r.returncode in (0, 3) else None, \
+        np.fromfile(ft, np.uint32).reshape(-1, 3) if r.returncode in (0, 3) else None
+
+
+def test_lane_decoder_matches_oracle(lane_check, coracle, tmp_path):
+    """The gfx950 kernels' pre-pass and per-lane decoder, compiled for the
+    host, reproduce the oracle bit-for-bit on every golden stream, the
+    hand-built streams and freshly encoded small meshes."""
+    from tests.test_oracle import hand_stream_fifo, hand_stream_small
+    streams = [np.load(p)["data"].tobytes() for p in sorted((ROOT / "tests" / "golden").glob("*_*.npz"))
+               if "data" in np.load(p)]
+    streams += [hand_stream_small()[0], hand_stream_fifo()[0]]
+    for name, V, T in small_meshes():
+        streams.append(encode_with_oracle(V, T, pred=([[33, 32, -33], [31, 34, -33], [32, 32, -32]],
+                                                      [5, 5, 5]))[1])
+    for data in streams:
+        rc, Vl, Tl = run_lane_check(lane_check, data, tmp_path)
+        assert rc == 0
+        Vo, To = coracle.decode(data)
+        assert np.array_equal(Tl.astype(np.int64), To)
+        assert np.array_equal(Vl.view(np.uint32), Vo.view(np.uint32))
+
+
+def test_lane_prepass_rejects_corruption(lane_check, coracle, tmp_path):
+    """Bit flips: whenever the host-built pre-pass accepts a stream, the lane
+    decoder's output equals the oracle's (or the oracle rejects it too)."""
+    z = np.load(ROOT / "tests" / "golden" / "dup_hf_24.npz")
+    base = bytearray(z["data"].tobytes())
+    rng = np.random.default_rng(11)
+    for trial in range(60):
+        d = bytearray(base)
+        pos = int(rng.integers(8 * (86 + 32) + 8, 8 * len(d)))
+        d[pos >> 3] ^= 1 << (7 - (pos & 7))
+        rc, Vl, Tl = run_lane_check(lane_check, bytes(d), tmp_path)
+        if rc != 0:
+            continue
+        try:
+            Vo, To = coracle.decode(bytes(d))
+        except ValueError:
+            raise AssertionError(f"trial {trial}: pre-pass accepted a stream the oracle rejects")
+        assert np.array_equal(Tl.astype(np.int64), To)
+        assert np.array_equal(Vl.view(np.uint32), Vo.view(np.uint32))
