@@ -1,0 +1,69 @@
+"""Launch-shape autotuner for the STRIDE decode kernels.
+
+Times candidate launch shapes on a decoder (ParaDeltaV5GpuDecoder or
+BatchDecoder, anything with set_tune/get_tune/decode) with HIP events on the
+launch stream and keeps the fastest.  The search space:
+
+  mode 1 (lane-per-meshlet groups):  group_size x lds_bytes x grid
+  mode 0 (wave per meshlet):         waves x stage_words (x persistent queue)
+
+group_size is the number of meshlets (lanes) a wave decodes together;
+lds_bytes the LDS a group may use for its staged bits and packed codes
+(smaller budgets give more resident waves per CU, larger ones fuller
+waves).
+"""
+from __future__ import annotations
+
+import itertools
+import json
+
+GROUP_SIZES = (16, 32, 48, 64)
+LDS_BUDGETS = (16384, 24576, 32768, 40960, 49152, 65536)
+
+
+def candidates(include_wave_mode: bool = True, grids=(0,)):
+    out = []
+    for gs, lds, grid in itertools.product(GROUP_SIZES, LDS_BUDGETS, grids):
+        out.append(dict(mode=1, group_size=gs, lds_bytes=lds, grid=grid, use_queue=0))
+    if include_wave_mode:
+        for w, sw in itertools.product((1, 4), (512, 1024)):
+            out.append(dict(mode=0, waves=w, stage_words=sw, grid=0, use_queue=0))
+    return out
+
+
+def time_once(dec, steps: int, mode_kw=None) -> float:
+    """Average ms per decode over `steps` launches (HIP events)."""
+    import torch
+    kw = mode_kw or {}
+    dec.decode(**kw)
+    torch.cuda.synchronize()
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        dec.decode(**kw)
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
+def tune(dec, steps: int = 10, cands=None, mode_kw=None, log=None):
+    """Tries every candidate, leaves the decoder on the fastest, returns
+    (best_tune, [(tune, ms), ...])."""
+    base = dec.get_tune()
+    results = []
+    for c in cands or candidates():
+        try:
+            dec.set_tune(**c)
+        except ValueError:
+            continue
+        ms = time_once(dec, steps, mode_kw)
+        results.append((dict(c), ms))
+        if log:
+            log(f"tune {json.dumps(c)}: {ms:.4f} ms")
+    if not results:
+        dec.set_tune(**base)
+        return base, results
+    best = min(results, key=lambda r: r[1])[0]
+    dec.set_tune(**best)
+    return best, results
