@@ -1,20 +1,30 @@
 """STRIDE decode benchmark on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
+                    [--batch N] [--quick] [--no-cpu]
 
 A step is one decode of the whole workload (one kernel launch for a single
-mesh).  The default workload is the 1900 x 1900 heightfield (7,212,402
+mesh).  The headline workload is the 1900 x 1900 heightfield (7,212,402
 triangles) with 12-bit bbox quantisation, encoded by this package's
-STRIDEEncoder; inputs are resident in HBM before timing starts.  Prints one
-JSON line (rank 0) with the decode throughput in Gtris/s, the dominant
-kernel's HBM roofline figures and a CPU baseline (the C oracle, one thread,
-timed on this host).
+STRIDEEncoder; inputs are resident in HBM before timing starts.  The same
+run also decodes
+
+* the 16-bit variant of the heightfield and the level-8 icosphere
+  (1,310,720 triangles), reported per variant;
+* the batched workload: N meshes (default 2,048) with triangle counts
+  log-uniform in [1K, 200K] drawn from {displaced UV sphere, displaced torus,
+  noise heightfield}, decoded by one hipGraph replay of the batched kernel
+  versus a loop of one launch per mesh (equal output, checked);
+
+Prints one JSON line with the headline throughput in Gtris/s, the dominant
+kernel's HBM roofline figures, a CPU baseline (the C oracle, one thread,
+timed on this host) and the other workloads under "details".
 """
 from __future__ import annotations
 
 import argparse
+import functools
 import json
-import math
 import os
 import sys
 import time
@@ -27,6 +37,8 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "decode Gtris/s + HBM GB/s (batched vs per-mesh); learned-encode speedup @≤1% bpv"
 HBM_PEAK_GBS = 8000.0
+TUNE_CANDIDATES = [dict(mode=1, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
+                   for gs in (32, 64) for lds in (16384, 32768, 40960, 49152)]
 
 
 def log(*a):
@@ -51,43 +63,46 @@ def encode_mesh(V, T, bits):
     return data, prep, {"prep_s": t1 - t0, "search_write_s": t2 - t1, **enc.last_stats}
 
 
-def time_decoder(dec, steps, warmup):
+def time_calls(fn, steps, warmup):
+    """(wall seconds per call, event seconds per call) on torch's current stream."""
     import torch
     for _ in range(warmup):
-        dec.decode()
+        fn()
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
     t0 = time.perf_counter()
     ev0.record()
     for _ in range(steps):
-        dec.decode()
+        fn()
     ev1.record()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
     return wall / steps, ev0.elapsed_time(ev1) / 1e3 / steps
 
 
-def cpu_baseline(data, n_t, budget_s=10.0):
+def autotune(dec, mode_kw=None):
+    from meshpress_amd.autotune import tune
+    best, res = tune(dec, steps=10, cands=TUNE_CANDIDATES, mode_kw=mode_kw)
+    return best, [[c["group_size"], c["lds_bytes"], round(ms, 4)] for c, ms in res]
+
+
+def cpu_baseline(data, n_t, gpu_out, budget_s=10.0):
+    """The C oracle decode on one host thread over the full stream; its
+    output also checks the GPU decode that was timed."""
     from tests.oracle_c import load
     co = load()
-    reps, t_total = 0, 0.0
+    reps, t_total, out = 0, 0.0, None
     while t_total < budget_s and reps < 20:
         t0 = time.perf_counter()
-        co.decode(data, n_threads=1)
+        out = co.decode(data, n_threads=1)
         t_total += time.perf_counter() - t0
         reps += 1
-    return n_t / (t_total / reps) / 1e9, reps, t_total
-
-
-def check_parity(dec, data):
-    from tests.oracle_c import load
-    V, T = dec.decode_to_host()
-    Vo, To = load().decode(data, n_threads=min(16, os.cpu_count() or 1))
-    ok = np.array_equal(T.astype(np.int64), To) and np.array_equal(V.view(np.uint32), Vo.view(np.uint32))
-    if not ok:
-        raise SystemExit("bench workload decode does not match the oracle")
-    return ok
+    V, T = gpu_out
+    Vo, To = out
+    same = bool(np.array_equal(T.astype(np.int64), To)
+                and np.array_equal(V.view(np.uint32), Vo.view(np.uint32)))
+    return n_t / (t_total / reps) / 1e9, reps, t_total, same
 
 
 def read_traffic(name):
@@ -101,12 +116,105 @@ def read_traffic(name):
     return None
 
 
+def single_mesh(name, V, T, bits, steps, warmup, tune=True):
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    log(f"{name}: encoding {len(T):,} tris at {bits}-bit")
+    data, prep, enc_stats = encode_mesh(V, T, bits)
+    log(f"{name}: {len(data):,} bytes, {prep['n_meshlets']:,} meshlets, g_bits {list(prep['g_bits'])}, "
+        f"prep {enc_stats['prep_s']:.1f}s search+write {enc_stats['search_write_s']:.1f}s")
+    dec = ParaDeltaV5GpuDecoder(data)
+    sweep = None
+    if tune:
+        best, sweep = autotune(dec)
+        log(f"{name}: tuned {best}")
+    s_wall, s_gpu = time_calls(dec.decode, steps, warmup)
+    bytes_alg = dec.bytes_read + dec.bytes_written
+    r = {"workload": name, "n_tris": dec.n_t_total, "n_verts_out": dec.n_v_total,
+         "meshlets": dec.n_meshlets, "g_bits": [int(x) for x in prep["g_bits"]],
+         "stream_bytes": dec.stream_bytes, "side_bytes": dec.side_bytes,
+         "bpv": 8 * len(data) / prep["n_v"], "tune": dec.get_tune(),
+         "ms_wall": s_wall * 1e3, "ms_kernel": s_gpu * 1e3,
+         "gtris_s": dec.n_t_total / s_wall / 1e9,
+         "hbm_gb_s": bytes_alg / s_gpu / 1e9, "algorithmic_bytes": bytes_alg,
+         "pred_n": enc_stats["pred_n"], "pred_K": enc_stats["pred_K"],
+         "prep_s": enc_stats["prep_s"], "search_write_s": enc_stats["search_write_s"]}
+    r["hbm_frac"] = r["hbm_gb_s"] / HBM_PEAK_GBS
+    if sweep is not None:
+        r["tune_sweep_ms"] = sweep
+    log(f"{name}: {s_gpu * 1e3:.3f} ms/launch, {r['gtris_s']:.3f} Gtris/s, {r['hbm_gb_s']:.1f} GB/s")
+    return dec, data, r
+
+
+def batched(n_meshes, steps, warmup, predictor):
+    import torch
+    from meshpress_amd import workloads
+    from meshpress_amd.batch import BatchDecoder
+    from meshpress_amd.encoder.many import encode_many
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    specs = workloads.batch_specs(n_meshes, 1000, 200000, 0)
+    sources = [functools.partial(workloads.mesh_with_tris, f, n, s) for f, n, s in specs]
+    t0 = time.perf_counter()
+
+    def progress(i, n):
+        if i % 256 == 0 or i == n:
+            log(f"batch: encoded {i}/{n} meshes ({time.perf_counter() - t0:.0f} s)")
+
+    blobs, st = encode_many(sources, progress=progress, precision_error=bbox_error(12),
+                            precision_mode="bbox_frac", predictor=predictor)
+    t_enc = time.perf_counter() - t0
+    bd = BatchDecoder(blobs)
+    n_t = bd.n_t_total
+    log(f"batch: {n_meshes} meshes, {n_t:,} tris, {bd.n_meshlets:,} meshlets, "
+        f"{bd.stream_bytes:,} stream bytes, encode {t_enc:.0f} s")
+    best, sweep = autotune(bd, mode_kw={"mode": "graph"})
+    log(f"batch: tuned {best}")
+    r = {"workload": f"batch{n_meshes}_log_uniform_1k_200k_q12", "n_meshes": n_meshes,
+         "n_tris": n_t, "n_verts_out": bd.n_v_total, "meshlets": bd.n_meshlets,
+         "stream_bytes": bd.stream_bytes, "side_bytes": bd.side_bytes, "predictor": predictor,
+         "encode_s": t_enc, "tune": bd.get_tune(), "tune_sweep_ms": sweep}
+    bytes_alg = bd.bytes_read + bd.bytes_written
+    for mode in ("graph", "launch", "per_mesh"):
+        w, g = time_calls(functools.partial(bd.decode, mode=mode), steps, warmup)
+        r[mode] = {"ms_wall": w * 1e3, "ms_gpu": g * 1e3, "gtris_s": n_t / w / 1e9,
+                   "hbm_gb_s": bytes_alg / g / 1e9}
+        log(f"batch {mode}: {w * 1e3:.3f} ms wall, {n_t / w / 1e9:.3f} Gtris/s")
+    # equal output: graph replay vs the per-mesh loop, bit for bit
+    bd.decode(mode="graph")
+    v1, t1 = bd.d_verts.clone(), bd.d_tris.clone()
+    bd.d_verts.fill_(float("nan"))
+    bd.d_tris.fill_(-1)
+    bd.decode(mode="per_mesh")
+    torch.cuda.synchronize()
+    r["equal_output"] = bool(torch.equal(v1.view(torch.int32), bd.d_verts.view(torch.int32))
+                             and torch.equal(t1, bd.d_tris))
+    # the per-mesh loop a caller of the single-mesh surface writes
+    decs = [ParaDeltaV5GpuDecoder(b) for b in blobs]
+    for d in decs:
+        d.set_tune(**{k: bd.get_tune()[k] for k in ("mode", "group_size", "lds_bytes")})
+
+    def loop():
+        for d in decs:
+            d.decode()
+
+    w, g = time_calls(loop, max(2, steps // 4), 2)
+    r["per_mesh_python"] = {"ms_wall": w * 1e3, "ms_gpu": g * 1e3, "gtris_s": n_t / w / 1e9}
+    log(f"batch per-mesh python loop: {w * 1e3:.3f} ms wall")
+    del decs
+    r["speedup_vs_per_mesh"] = r["per_mesh"]["ms_wall"] / r["graph"]["ms_wall"]
+    r["speedup_vs_per_mesh_python"] = r["per_mesh_python"]["ms_wall"] / r["graph"]["ms_wall"]
+    r["bpv_mean"] = float(np.mean([8 * s["bytes"] / s["n_v"] for s in st]))
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--workload", default="heightfield_1900_q12")
+    ap.add_argument("--batch", type=int, default=2048, help="meshes in the batched workload (0: skip)")
+    ap.add_argument("--batch-predictor", default="generalized", choices=("generalized", "canonical"))
+    ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
@@ -123,14 +231,13 @@ def main():
         return
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(decode shards by meshlet ranges with no exchange; see DESIGN.md)", file=sys.stderr)
+              "(decode shards by meshlet ranges with no exchange step)", file=sys.stderr)
         sys.exit(2)
     import torch
     if not torch.cuda.is_available():
         sys.exit("bench.py needs a ROCm GPU")
     torch.cuda.set_device(0)
     from meshpress_amd import workloads
-    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
 
     name = a.workload
     if name.startswith("heightfield_1900"):
@@ -143,37 +250,45 @@ def main():
         V, T = workloads.icosphere_l8()
     else:
         sys.exit(f"unknown workload {name}")
-    log(f"encoding {len(T):,} tris at {bits}-bit")
-    data, prep, enc_stats = encode_mesh(V, T, bits)
-    log(f"encoded: {len(data):,} bytes, {prep['n_meshlets']:,} meshlets, g_bits {list(prep['g_bits'])}, "
-        f"prep {enc_stats['prep_s']:.1f}s search+write {enc_stats['search_write_s']:.1f}s")
-    dec = ParaDeltaV5GpuDecoder(data)
-    check_parity(dec, data)
-    log("parity vs oracle: bit-exact")
-    s_wall, s_gpu = time_decoder(dec, a.steps, a.warmup)
-    n_t = dec.n_t_total
-    bytes_alg = dec.bytes_read + dec.bytes_written
-    gtris = n_t / s_wall / 1e9
-    achieved = bytes_alg / s_gpu / 1e9
-    log(f"decode: {s_gpu * 1e3:.3f} ms/launch (events), {gtris:.3f} Gtris/s, {achieved:.1f} GB/s")
+    dec, data, head = single_mesh(name, V, T, bits, a.steps, a.warmup)
+    details = {}
     cpu = None
     if not a.no_cpu:
-        v, reps, tt = cpu_baseline(data, n_t)
+        gpu_out = dec.decode_to_host()
+        v, reps, tt, same = cpu_baseline(data, head["n_tris"], gpu_out)
         cpu = {"value": v, "unit": "Gtris/s", "cores": 1, "kind": "port",
-               "sample": f"C oracle decode_dup restatement, full {name} stream x{reps} ({tt:.1f} s)"}
-        log(f"cpu baseline: {v:.4f} Gtris/s")
+               "sample": f"C oracle decode_dup restatement, full {name} stream x{reps} ({tt:.1f} s)",
+               "gpu_output_matches": same}
+        log(f"cpu baseline: {v:.4f} Gtris/s, GPU output {'matches' if same else 'DIFFERS'}")
+        if not same:
+            sys.exit("GPU decode differs from the oracle on the bench workload")
+    del dec
+    if not a.quick:
+        if name.startswith("heightfield_1900"):
+            other = "heightfield_1900_q16" if bits == 12 else "heightfield_1900_q12"
+            _, _, details[other] = single_mesh(other, V, T, 28 - bits, a.steps, a.warmup)
+        del V, T
+        torch.cuda.empty_cache()
+        if not name.startswith("icosphere_l8"):
+            log("generating icosphere level 8")
+            Vi, Ti = workloads.icosphere_l8()
+            _, _, details["icosphere_l8_q12"] = single_mesh("icosphere_l8_q12", Vi, Ti, 12,
+                                                            a.steps, a.warmup)
+            del Vi, Ti
+        torch.cuda.empty_cache()
+        if a.batch > 0:
+            details["batched"] = batched(a.batch, a.steps, a.warmup, a.batch_predictor)
     out = {
-        "metric": METRIC, "value": gtris, "unit": "Gtris/s", "n_gpus": 1, "steps": a.steps,
-        "warmup": a.warmup, "ms_per_step": s_wall * 1e3, "higher_is_better": True,
+        "metric": METRIC, "value": head["gtris_s"], "unit": "Gtris/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": head["ms_wall"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "int32+f64", "data": "synthetic",
-        "config": {"workload": name, "n_tris": n_t, "n_verts_out": dec.n_v_total,
-                   "meshlets": dec.n_meshlets, "g_bits": [int(x) for x in prep["g_bits"]],
-                   "stream_bytes": dec.stream_bytes, "side_bytes": dec.side_bytes,
-                   "tune": dec.get_tune()},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": read_traffic(name),
-                     "algorithmic_bytes": bytes_alg, "kernel_ms": s_gpu * 1e3},
+        "config": {k: head[k] for k in ("workload", "n_tris", "n_verts_out", "meshlets", "g_bits",
+                                        "stream_bytes", "side_bytes", "bpv", "tune")},
+        "roofline": {"bound": "hbm", "achieved": head["hbm_gb_s"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": head["hbm_frac"], "traffic": read_traffic(name),
+                     "algorithmic_bytes": head["algorithmic_bytes"], "kernel_ms": head["ms_kernel"]},
         "cpu_baseline": cpu,
+        "details": details,
     }
     print(json.dumps(out), flush=True)
 
