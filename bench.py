@@ -15,6 +15,9 @@ run also decodes
   log-uniform in [1K, 200K] drawn from {displaced UV sphere, displaced torus,
   noise heightfield}, decoded by one hipGraph replay of the batched kernel
   versus a loop of one launch per mesh (equal output, checked);
+* the learned encoder proposal against the exhaustive search on the
+  headline mesh and on held-out meshes: search-time speedup, bit-rate
+  regret, and decode equality of the two streams;
 
 Prints one JSON line with the headline throughput in Gtris/s, the dominant
 kernel's HBM roofline figures, a CPU baseline (the C oracle, one thread,
@@ -142,7 +145,7 @@ def single_mesh(name, V, T, bits, steps, warmup, tune=True):
     if sweep is not None:
         r["tune_sweep_ms"] = sweep
     log(f"{name}: {s_gpu * 1e3:.3f} ms/launch, {r['gtris_s']:.3f} Gtris/s, {r['hbm_gb_s']:.1f} GB/s")
-    return dec, data, r
+    return dec, data, r, prep
 
 
 def batched(n_meshes, steps, warmup, predictor):
@@ -174,7 +177,9 @@ def batched(n_meshes, steps, warmup, predictor):
          "encode_s": t_enc, "tune": bd.get_tune(), "tune_sweep_ms": sweep}
     bytes_alg = bd.bytes_read + bd.bytes_written
     for mode in ("graph", "launch", "per_mesh"):
-        w, g = time_calls(functools.partial(bd.decode, mode=mode), steps, warmup)
+        # the per-mesh loop is ~100x slower: fewer repetitions keep the run short
+        k = steps if mode != "per_mesh" else max(3, steps // 10)
+        w, g = time_calls(functools.partial(bd.decode, mode=mode), k, min(warmup, k))
         r[mode] = {"ms_wall": w * 1e3, "ms_gpu": g * 1e3, "gtris_s": n_t / w / 1e9,
                    "hbm_gb_s": bytes_alg / g / 1e9}
         log(f"batch {mode}: {w * 1e3:.3f} ms wall, {n_t / w / 1e9:.3f} Gtris/s")
@@ -196,7 +201,7 @@ def batched(n_meshes, steps, warmup, predictor):
         for d in decs:
             d.decode()
 
-    w, g = time_calls(loop, max(2, steps // 4), 2)
+    w, g = time_calls(loop, max(3, steps // 10), 1)
     r["per_mesh_python"] = {"ms_wall": w * 1e3, "ms_gpu": g * 1e3, "gtris_s": n_t / w / 1e9}
     log(f"batch per-mesh python loop: {w * 1e3:.3f} ms wall")
     del decs
@@ -204,6 +209,53 @@ def batched(n_meshes, steps, warmup, predictor):
     r["speedup_vs_per_mesh_python"] = r["per_mesh_python"]["ms_wall"] / r["graph"]["ms_wall"]
     r["bpv_mean"] = float(np.mean([8 * s["bytes"] / s["n_v"] for s in st]))
     return r
+
+
+def learned_encode(head_prep, n_heldout):
+    """Search time and bytes of the learned proposal vs the exhaustive search."""
+    from meshpress_amd import workloads
+    from meshpress_amd.encoder import STRIDEEncoder
+    from meshpress_amd.encoder.prep import prepare
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    kw = dict(precision_error=bbox_error(12), precision_mode="bbox_frac")
+    ex = STRIDEEncoder(**kw)
+    le = STRIDEEncoder(selector="learned", **kw)
+    specs = workloads.learned_specs(50, 1)[:n_heldout]
+    preps = []
+    for fam, n, seed in specs:
+        V, T = workloads.mesh_with_tris(fam, n, seed)
+        preps.append(prepare(V, T, **kw))
+    for e in (ex, le):                                   # warm-up
+        e.encode_prepared(preps[0])
+    rows = []
+    for name, prep in [("heightfield_1900_q12", head_prep)] + [(f"{f}_{n}_{sd}", p) for (f, n, sd), p
+                                                                in zip(specs, preps)]:
+        r = {"mesh": name, "n_tris": int(prep["n_t"])}
+        outs = []
+        for tag, e in (("exhaustive", ex), ("learned", le)):
+            data = e.encode_prepared(prep)
+            st = e.last_stats
+            r[tag] = {"bytes": len(data), "search_s": st["predictor_s"] + st["coder_s"]}
+            V, T = ParaDeltaV5GpuDecoder(data).decode_to_host()
+            outs.append((V.view(np.uint32).copy(), T))
+        r["decode_equal"] = bool(np.array_equal(outs[0][0], outs[1][0])
+                                 and np.array_equal(outs[0][1], outs[1][1]))
+        rows.append(r)
+        log(f"learned {name}: search {r['exhaustive']['search_s'] * 1e3:.1f} -> "
+            f"{r['learned']['search_s'] * 1e3:.1f} ms, bytes {r['exhaustive']['bytes']} -> "
+            f"{r['learned']['bytes']}")
+
+    def agg(rs):
+        be = sum(r["exhaustive"]["bytes"] for r in rs)
+        bl = sum(r["learned"]["bytes"] for r in rs)
+        te = sum(r["exhaustive"]["search_s"] for r in rs)
+        tl = sum(r["learned"]["search_s"] for r in rs)
+        return {"meshes": len(rs), "exhaustive_search_s": te, "learned_search_s": tl,
+                "speedup": te / tl, "regret": (bl - be) / be,
+                "decode_equal": all(r["decode_equal"] for r in rs)}
+
+    return {"headline_mesh": agg(rows[:1]), "heldout": agg(rows[1:]), "all": agg(rows),
+            "per_mesh": rows}
 
 
 def main():
@@ -214,6 +266,8 @@ def main():
     ap.add_argument("--workload", default="heightfield_1900_q12")
     ap.add_argument("--batch", type=int, default=2048, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--batch-predictor", default="generalized", choices=("generalized", "canonical"))
+    ap.add_argument("--learned-heldout", type=int, default=12,
+                    help="held-out meshes in the learned-encode comparison (0: skip)")
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -250,7 +304,7 @@ def main():
         V, T = workloads.icosphere_l8()
     else:
         sys.exit(f"unknown workload {name}")
-    dec, data, head = single_mesh(name, V, T, bits, a.steps, a.warmup)
+    dec, data, head, head_prep = single_mesh(name, V, T, bits, a.steps, a.warmup)
     details = {}
     cpu = None
     if not a.no_cpu:
@@ -266,16 +320,19 @@ def main():
     if not a.quick:
         if name.startswith("heightfield_1900"):
             other = "heightfield_1900_q16" if bits == 12 else "heightfield_1900_q12"
-            _, _, details[other] = single_mesh(other, V, T, 28 - bits, a.steps, a.warmup)
+            _, _, details[other], _ = single_mesh(other, V, T, 28 - bits, a.steps, a.warmup)
         del V, T
         torch.cuda.empty_cache()
         if not name.startswith("icosphere_l8"):
             log("generating icosphere level 8")
             Vi, Ti = workloads.icosphere_l8()
-            _, _, details["icosphere_l8_q12"] = single_mesh("icosphere_l8_q12", Vi, Ti, 12,
-                                                            a.steps, a.warmup)
+            _, _, details["icosphere_l8_q12"], _ = single_mesh("icosphere_l8_q12", Vi, Ti, 12,
+                                                               a.steps, a.warmup)
             del Vi, Ti
         torch.cuda.empty_cache()
+        if a.learned_heldout > 0 and bits == 12:
+            details["learned_encode"] = learned_encode(head_prep, a.learned_heldout)
+        del head_prep
         if a.batch > 0:
             details["batched"] = batched(a.batch, a.steps, a.warmup, a.batch_predictor)
     out = {

@@ -174,6 +174,21 @@ int stride_encoder_residuals(stride_encoder* h, const int32_t* pred_n, const int
  * 6 * n_meshlets + 1 offsets. */
 int stride_encoder_export_residuals(const stride_encoder* h, int32_t* vals,
                                     int64_t* stream_off);
+/* Per-meshlet features of the learned proposal, 32 float32 per meshlet (no
+ * reference counterpart: they feed the MLP that replaces the search of
+ * encoder/_irlp_fit.py:364-394 and paradelta_v5_dup.py:159-171).  Stream
+ * statistics come from the canonical predictor a + b - c (source 0) or from
+ * the residuals of the last stride_encoder_residuals call (source 1);
+ * streams s = dx dy dz px py pz, u = zigzag residual:
+ *   0 n_local/256   1 n_tris/256   2 n_strips/n_tris   3 n_kind0/n_local
+ *   4..6   log2(1 + local code extent of axis d) / 16
+ *   7..10  fraction of triangles in strips of length 1, 2-3, 4-15, >= 16
+ *   11..16 log2(1 + count_s) / 8
+ *   17..22 mean of floor(log2(u + 1)) over stream s, / 16
+ *   23..28 log2(1 + mean u over stream s) / 16
+ *   29..31 clamp(sum(x*y) / sum(x*x), 0, 2) / 2 over the parallelogram
+ *          vertices of axis d, x = a + b - 2c, y = v - c (1 when sum(x*x) = 0) */
+int stride_encoder_features(const stride_encoder* h, int32_t source, float* out);
 /* DPR4 v3 bytes (paradelta_v5_dup.py:239-313,412-455).  f13 = center[3],
  * scale, per_coord_err, g_min[3], g_range[3] (float32 values written
  * verbatim); g_bits[3]; tagk = 12 bytes per meshlet ((tag, k) x 3 delta then
@@ -204,6 +219,55 @@ int stride_irlp_cost(const int32_t* d_a, const int32_t* d_b, const int32_t* d_c,
                      const int32_t* d_v, const int64_t* d_off, int64_t n_m,
                      const int32_t* d_cand, int32_t S, int32_t K, int64_t* d_cost,
                      void* stream);
+/* The same costs kept per meshlet: d_cost_m[m * S + s] (n_m x S int64), the
+ * labeller of the learned predictor proposal (no reference counterpart;
+ * the terms are those summed by _cost_seeds_njit). */
+int stride_irlp_cost_meshlets(const int32_t* d_a, const int32_t* d_b, const int32_t* d_c,
+                              const int32_t* d_v, const int64_t* d_off, int64_t n_m,
+                              const int32_t* d_cand, int32_t S, int32_t K, int64_t* d_cost_m,
+                              void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Learned proposal network (device pointers; bf16 buffers are uint16_t bit
+ * patterns).  It replaces the exhaustive searches of
+ * encoder/_irlp_fit.py:364-394 and paradelta_v5_dup.py:159-171 by a proposal
+ * that the search kernels above then score; the reference has no network.
+ * MLP 32 -> 128 -> 128 -> 176, ReLU, BF16 operands, FP32 accumulation on
+ * the matrix cores.  Parameters (float32, n_params): W1 32x128 | W2 128x128 |
+ * W3 128x176 | b1 | b2 | b3; d_wb holds bf16 copies of the n_weights
+ * weights.  Logit heads: predictor 3 x 17 (columns 0..50), coders 6 x 20
+ * (51..170); 171..175 are padding. */
+int stride_mlp_sizes(int32_t* n_in, int32_t* n_hidden, int32_t* n_out, int32_t* n_weights,
+                     int32_t* n_params);
+/* C (M x N) = op(A) op(B) with op(A)[m][k] = trans_a ? A[k*lda+m] : A[m*lda+k]
+ * (likewise B).  epi 0: float C (+ bias[n] if bias); 1: bf16 relu(C + bias);
+ * 2: bf16 C where mask[m*ldm+n] > 0, else 0. */
+int stride_mlp_gemm(const uint16_t* d_a, int32_t lda, int32_t trans_a, const uint16_t* d_b,
+                    int32_t ldb, int32_t trans_b, int32_t M, int32_t N, int32_t K, int32_t epi,
+                    const float* d_bias, const uint16_t* d_mask, int32_t ldm, float* d_cf,
+                    uint16_t* d_cb, int32_t ldc, void* stream);
+/* x (n x 32 bf16) -> h1, h2 (n x 128 bf16), z (n x 176 float logits). */
+int stride_mlp_forward(const uint16_t* d_x, int64_t n, const uint16_t* d_wb, const float* d_params,
+                       uint16_t* d_h1, uint16_t* d_h2, float* d_z, void* stream);
+/* Grouped softmax cross-entropy over the 9 heads; labels n x 9 (255 = head
+ * ignored).  dz = scale * (softmax - onehot) as bf16; loss per sample. */
+int stride_mlp_xent(const float* d_z, const uint8_t* d_labels, int64_t n, float scale,
+                    uint16_t* d_dz, float* d_loss, void* stream);
+/* Gradient of the loss behind dz into d_grad (n_params floats); d_dh2 and
+ * d_dh1 (n x 128 bf16) receive the hidden-layer gradients. */
+int stride_mlp_backward(const uint16_t* d_x, int64_t n, const uint16_t* d_wb, const uint16_t* d_h1,
+                        const uint16_t* d_h2, const uint16_t* d_dz, uint16_t* d_dh2,
+                        uint16_t* d_dh1, float* d_grad, void* stream);
+/* Adam step `step` (>= 1) on the float32 parameters; refreshes d_wb. */
+int stride_mlp_adam(float* d_params, float* d_m, float* d_v, const float* d_grad, uint16_t* d_wb,
+                    int64_t n_params, float lr, float beta1, float beta2, float eps, int32_t step,
+                    void* stream);
+/* float -> bf16, round to nearest even. */
+int stride_mlp_to_bf16(const float* d_x, int64_t n, uint16_t* d_y, void* stream);
+/* Inference: predictor-head probabilities (n x 51 float), coder-head arg-max
+ * class (n x 6) and its probability (n x 6). */
+int stride_mlp_heads(const float* d_z, int64_t n, float* d_pred_prob, uint8_t* d_cls,
+                     float* d_cls_prob, void* stream);
 
 #ifdef __cplusplus
 }

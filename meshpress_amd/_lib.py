@@ -80,6 +80,18 @@ SIGNATURES = {
     "stride_select_codes": (C.c_int, [p, p, i64, p, p, p]),
     "stride_stream_costs": (C.c_int, [p, p, i64, p, p]),
     "stride_irlp_cost": (C.c_int, [p, p, p, p, p, i64, p, i32, i32, p, p]),
+    "stride_irlp_cost_meshlets": (C.c_int, [p, p, p, p, p, i64, p, i32, i32, p, p]),
+    "stride_encoder_features": (C.c_int, [p, i32, p]),
+    "stride_mlp_sizes": (C.c_int, [P(i32), P(i32), P(i32), P(i32), P(i32)]),
+    "stride_mlp_gemm": (C.c_int, [p, i32, i32, p, i32, i32, i32, i32, i32, i32, p, p, i32, p, p,
+                                  i32, p]),
+    "stride_mlp_forward": (C.c_int, [p, i64, p, p, p, p, p, p]),
+    "stride_mlp_xent": (C.c_int, [p, p, i64, C.c_float, p, p, p]),
+    "stride_mlp_backward": (C.c_int, [p, i64, p, p, p, p, p, p, p, p]),
+    "stride_mlp_adam": (C.c_int, [p, p, p, p, p, i64, C.c_float, C.c_float, C.c_float, C.c_float,
+                                  i32, p]),
+    "stride_mlp_to_bf16": (C.c_int, [p, i64, p, p]),
+    "stride_mlp_heads": (C.c_int, [p, i64, p, p, p, p]),
 }
 
 

@@ -882,6 +882,14 @@ int stride_encoder_export_residuals(const stride_encoder* h, int32_t* vals, int6
   return STRIDE_OK;
 }
 
+int stride_encoder_features(const stride_encoder* h, int32_t source, float* out) {
+  if (!h || !out) return fail(STRIDE_E_ARG, "null argument");
+  if (source != 0 && source != 1) return fail(STRIDE_E_ARG, "source must be 0 or 1");
+  if (source == 1 && !h->have_res) return fail(STRIDE_E_ARG, "residuals not computed");
+  meshlet_features(h->prep, h->codes.data(), source ? &h->res : nullptr, out, h->n_threads);
+  return STRIDE_OK;
+}
+
 int stride_encoder_write(const stride_encoder* h, const float* f13, const int32_t* g_bits,
                          const int32_t* pred_n, const int32_t* pred_K, const uint8_t* tagk,
                          uint8_t* out, size_t cap, size_t* len) {

@@ -1021,6 +1021,96 @@ void compute_residuals(const Prep& P, const int64_t* codes, const int32_t* pred_
   for (int64_t v : mx) R.max_abs = std::max(R.max_abs, v);
 }
 
+// ---------------------------------------------------------------------------
+// Per-meshlet features of the learned coder/predictor proposal.  The stream
+// statistics come from the residuals R when given (the predictor the
+// encoder settled on), else from the canonical parallelogram a + b - c, so
+// the first pass does not depend on the predictor being chosen.  Sums are
+// exact integers; the float conversions happen once at the end.
+// ---------------------------------------------------------------------------
+static inline uint64_t zz_host(int64_t r) { return r < 0 ? (uint64_t)(-r) * 2 - 1 : (uint64_t)r * 2; }
+static inline int bitlen1(uint64_t u) { return 63 - __builtin_clzll(u + 1); }   // floor(log2(u + 1))
+
+void meshlet_features(const Prep& P, const int64_t* codes, const Residuals* R, float* out,
+                      int n_threads) {
+  const int64_t n_m = (int64_t)P.plans.size();
+  parallel_for(n_m, n_threads, [&](int64_t m, int) {
+    const Plan& pl = P.plans[m];
+    const auto& walk = P.walks[m];
+    float* f = out + (size_t)kFeatures * m;
+    const int n_local = (int)walk.size();
+    const int n_tris = (int)(pl.tris_local.size() / 3);
+    const int n_strips = (int)pl.strip_len.size();
+    int n_kind0 = 0;
+    int64_t lo[3] = {INT64_MAX, INT64_MAX, INT64_MAX}, hi[3] = {INT64_MIN, INT64_MIN, INT64_MIN};
+    for (const auto& w : walk) {
+      n_kind0 += w.kind == 0;
+      const int64_t* tc = codes + 3 * pl.l2g[w.v];
+      for (int d = 0; d < 3; ++d) {
+        lo[d] = std::min(lo[d], tc[d]);
+        hi[d] = std::max(hi[d], tc[d]);
+      }
+    }
+    int64_t hist[4] = {0, 0, 0, 0};
+    for (int L : pl.strip_len) hist[L <= 1 ? 0 : (L <= 3 ? 1 : (L <= 15 ? 2 : 3))] += L;
+    // streams dx dy dz px py pz: count, sum of zigzag values, sum of bit lengths
+    int64_t cnt[6] = {0, 0, 0, 0, 0, 0}, sbl[6] = {0, 0, 0, 0, 0, 0};
+    uint64_t su[6] = {0, 0, 0, 0, 0, 0};
+    int64_t sxy[3] = {0, 0, 0}, sxx[3] = {0, 0, 0};
+    auto add = [&](int s, int64_t r) {
+      const uint64_t u = zz_host(r);
+      su[s] += u;
+      sbl[s] += bitlen1(u);
+      cnt[s]++;
+    };
+    int64_t prev[3] = {0, 0, 0};
+    bool first = true;
+    for (const auto& w : walk) {
+      const int64_t* tc = codes + 3 * pl.l2g[w.v];
+      if (w.kind == 0) {
+        if (!first && !R)
+          for (int d = 0; d < 3; ++d) add(d, tc[d] - prev[d]);
+        first = false;
+      } else {
+        const int64_t* ta = codes + 3 * pl.l2g[w.a];
+        const int64_t* tb = codes + 3 * pl.l2g[w.b];
+        const int64_t* tq = codes + 3 * pl.l2g[w.c];
+        for (int d = 0; d < 3; ++d) {
+          if (!R) add(3 + d, tc[d] - (ta[d] + tb[d] - tq[d]));
+          const int64_t x = ta[d] + tb[d] - 2 * tq[d], y = tc[d] - tq[d];
+          sxy[d] += x * y;
+          sxx[d] += x * x;
+        }
+      }
+      for (int d = 0; d < 3; ++d) prev[d] = tc[d];
+    }
+    if (R) {
+      const int nd = R->n_delta[m], np = R->n_para[m];
+      const int64_t* v = R->vals.data() + R->off[m];
+      for (int d = 0; d < 3; ++d) {
+        for (int i = 0; i < nd; ++i) add(d, v[d * nd + i]);
+        for (int i = 0; i < np; ++i) add(3 + d, v[3 * nd + d * np + i]);
+      }
+    }
+    f[0] = (float)(n_local / 256.0);
+    f[1] = (float)(n_tris / 256.0);
+    f[2] = (float)(n_tris ? (double)n_strips / n_tris : 0.0);
+    f[3] = (float)(n_local ? (double)n_kind0 / n_local : 0.0);
+    for (int d = 0; d < 3; ++d)
+      f[4 + d] = (float)(n_local ? std::log2(1.0 + (double)(hi[d] - lo[d])) / 16.0 : 0.0);
+    for (int b = 0; b < 4; ++b) f[7 + b] = (float)(n_tris ? (double)hist[b] / n_tris : 0.0);
+    for (int s = 0; s < 6; ++s) {
+      f[11 + s] = (float)(std::log2(1.0 + (double)cnt[s]) / 8.0);
+      f[17 + s] = (float)(cnt[s] ? (double)sbl[s] / (double)cnt[s] / 16.0 : 0.0);
+      f[23 + s] = (float)(cnt[s] ? std::log2(1.0 + (double)su[s] / (double)cnt[s]) / 16.0 : 0.0);
+    }
+    for (int d = 0; d < 3; ++d) {
+      const double s = sxx[d] > 0 ? (double)sxy[d] / (double)sxx[d] : 1.0;
+      f[29 + d] = (float)(std::min(2.0, std::max(0.0, s)) / 2.0);
+    }
+  });
+}
+
 void BitWriter::put(uint64_t value, int n) {
   if (n <= 0) return;
   if (n < 64) value &= ((uint64_t)1 << n) - 1;

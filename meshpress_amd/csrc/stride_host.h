@@ -106,6 +106,14 @@ struct Residuals {
 void compute_residuals(const Prep& P, const int64_t* codes, const int32_t* pred_n,
                        const int32_t* pred_K, Residuals& R, int n_threads);
 
+// Per-meshlet features of the learned proposal, kFeatures floats per
+// meshlet (layout in include/stride_hip.h, stride_encoder_features); stream
+// statistics from R, or from canonical residuals when R is null.
+constexpr int kFeatures = 32;
+void meshlet_features(const Prep& P, const int64_t* codes, const Residuals* R, float* out,
+                      int n_threads);
+
+
 // tagk: per meshlet 12 bytes: (tag,k) x 3 delta then (tag,k) x 3 para.
 std::vector<uint8_t> write_dup(const Prep& P, const Residuals& R, const Header& H,
                                const uint8_t* tagk);

@@ -103,11 +103,16 @@ __global__ __launch_bounds__(64) void irlp_cost_kernel(const int32_t* __restrict
                                                        const int32_t* __restrict__ Vv,
                                                        const int64_t* __restrict__ off,
                                                        const int32_t* __restrict__ cand, int S,
-                                                       int K, unsigned long long* __restrict__ cost) {
+                                                       int K, unsigned long long* __restrict__ cost,
+                                                       int64_t* __restrict__ per_m) {
   const int64_t m = blockIdx.x;
   const int sc = blockIdx.y * 64 + threadIdx.x;
   const int64_t b = off[m], e = off[m + 1];
-  if (e == b || sc >= S) return;
+  if (sc >= S) return;
+  if (e == b) {
+    if (per_m) per_m[m * S + sc] = 0;
+    return;
+  }
   const int64_t n0 = cand[3 * sc], n1 = cand[3 * sc + 1], n2 = cand[3 * sc + 2];
   const int64_t half = K ? ((int64_t)1 << (K - 1)) : 0;
   uint64_t acc[kCoders];
@@ -125,7 +130,10 @@ __global__ __launch_bounds__(64) void irlp_cost_kernel(const int32_t* __restrict
     uint64_t t = body_bits(j, acc[j], n);
     best = t < best ? t : best;
   }
-  atomicAdd(&cost[sc], (unsigned long long)best);
+  if (per_m)
+    per_m[m * S + sc] = (int64_t)best;
+  else
+    atomicAdd(&cost[sc], (unsigned long long)best);
 }
 
 __global__ void zero_i64_kernel(int64_t* p, int n) {
@@ -177,8 +185,25 @@ extern "C" int stride_irlp_cost(const int32_t* d_a, const int32_t* d_b, const in
     if (n_m > 0x7FFFFFFF) return set_error(STRIDE_E_ARG, "stride_irlp_cost: too many meshlets");
     hipLaunchKernelGGL(irlp_cost_kernel, dim3((unsigned)n_m, (S + 63) / 64), dim3(64), 0, st, d_a,
                        d_b, d_c, d_v, d_off, d_cand, (int)S, (int)K,
-                       reinterpret_cast<unsigned long long*>(d_cost));
+                       reinterpret_cast<unsigned long long*>(d_cost), nullptr);
   }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_error(STRIDE_E_HIP, hipGetErrorString(e));
+  return STRIDE_OK;
+}
+
+extern "C" int stride_irlp_cost_meshlets(const int32_t* d_a, const int32_t* d_b, const int32_t* d_c,
+                                         const int32_t* d_v, const int64_t* d_off, int64_t n_m,
+                                         const int32_t* d_cand, int32_t S, int32_t K,
+                                         int64_t* d_cost_m, void* stream) {
+  if (n_m < 0 || S < 0 || K < 0 || K > 30 || (S && !d_cand) ||
+      (n_m && (!d_a || !d_b || !d_c || !d_v || !d_off || !d_cost_m)))
+    return set_error(STRIDE_E_ARG, "stride_irlp_cost_meshlets: invalid argument");
+  if (S == 0 || n_m == 0) return STRIDE_OK;
+  if (n_m > 0x7FFFFFFF) return set_error(STRIDE_E_ARG, "stride_irlp_cost_meshlets: too many meshlets");
+  hipLaunchKernelGGL(irlp_cost_kernel, dim3((unsigned)n_m, (S + 63) / 64), dim3(64), 0,
+                     (hipStream_t)stream, d_a, d_b, d_c, d_v, d_off, d_cand, (int)S, (int)K, nullptr,
+                     d_cost_m);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return set_error(STRIDE_E_HIP, hipGetErrorString(e));
   return STRIDE_OK;

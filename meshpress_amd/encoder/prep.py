@@ -155,17 +155,34 @@ class NativePrep:
             self._h, _lib.ptr(A), _lib.ptr(B), _lib.ptr(Cc), _lib.ptr(V), _lib.ptr(off)))
         return A, B, Cc, V, off
 
-    def residuals(self, pred_n, pred_K):
-        """(int32 values, int64 stream offsets (6 n_m + 1), max |residual|)."""
+    def residuals(self, pred_n, pred_K, export: bool = True):
+        """Computes the residual streams for (pred_n, pred_K); returns (int32
+        values or None when export is False, int64 stream offsets (6 n_m + 1),
+        max |residual|)."""
         pn = np.ascontiguousarray(np.asarray(pred_n, dtype=np.int32).reshape(9))
         pk = np.ascontiguousarray(np.asarray(pred_K, dtype=np.int32).reshape(3))
         nv, mx = C.c_int64(), C.c_int64()
         _lib.check(self._L.stride_encoder_residuals(self._h, _lib.ptr(pn), _lib.ptr(pk),
                                                     C.byref(nv), C.byref(mx)))
-        vals = np.zeros(int(nv.value), np.int32)
+        self._n_vals = int(nv.value)
+        vals = self.residual_values() if export else None
         off = np.zeros(6 * self.n_meshlets + 1, np.int64)
-        _lib.check(self._L.stride_encoder_export_residuals(self._h, _lib.ptr(vals), _lib.ptr(off)))
+        _lib.check(self._L.stride_encoder_export_residuals(self._h, None, _lib.ptr(off)))
         return vals, off, int(mx.value)
+
+    def residual_values(self) -> np.ndarray:
+        """int32 values of the residual streams of the last residuals() call."""
+        vals = np.zeros(self._n_vals, np.int32)
+        _lib.check(self._L.stride_encoder_export_residuals(self._h, _lib.ptr(vals), None))
+        return vals
+
+    def features(self, source: int = 0) -> np.ndarray:
+        """Per-meshlet proposal features, float32 (n_meshlets, 32)
+        (stride_encoder_features): stream statistics of the canonical
+        predictor (source 0) or of the last residuals() call (source 1)."""
+        out = np.zeros((self.n_meshlets, 32), np.float32)
+        _lib.check(self._L.stride_encoder_features(self._h, int(source), _lib.ptr(out)))
+        return out
 
     def write(self, f13, g_bits, pred_n, pred_K, tagk) -> bytes:
         f13 = np.ascontiguousarray(np.asarray(f13, dtype=np.float32).reshape(13))

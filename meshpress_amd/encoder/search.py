@@ -182,3 +182,20 @@ def fit_predictor(A, B, C, V, off, stats=None):
     if stats is not None:
         stats["irlp_evals"] = evals
     return n_fit, K_fit
+
+
+def fit_axis(A, B, C, V, off, d, sub=None):
+    """Exhaustive fit of axis d alone (the per-axis body of fit_predictor):
+    ((n0, n1, n2) int64, K, bits over all samples, candidate evaluations)."""
+    off = np.asarray(off, dtype=np.int64)
+    As, Bs, Cs, Vs, off_s = sub if sub is not None else subsample(A, B, C, V, off)
+    X = np.stack([As[:, d], Bs[:, d], Cs[:, d]], axis=1).astype(np.float64)
+    w = irls_l1(X, Vs[:, d].astype(np.float64))
+    cost = IrlpCost(As[:, d], Bs[:, d], Cs[:, d], Vs[:, d], off_s)
+    K, ns, _ = search_axis(cost, w)
+    full = IrlpCost(A[:, d], B[:, d], C[:, d], V[:, d], off)
+    b_fit = int(full(np.array(ns)[None], int(K))[0])
+    b_can = int(full(CANON_N[d][None], 0)[0])
+    if b_can < b_fit:
+        return CANON_N[d].copy(), 0, b_can, cost.evals + full.evals
+    return np.array(ns, dtype=np.int64), int(K), b_fit, cost.evals + full.evals

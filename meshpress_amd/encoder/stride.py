@@ -6,8 +6,10 @@ search runs on the GPU:
 
   selector="exhaustive"       IRLP grid search + per-stream (tag, k) scoring
                               on the GPU (the reference's exhaustive search);
-  selector="learned"          the learned proposal (learned.py) picks the
-                              predictor and coders, scored on the GPU;
+  selector="learned"          the learned proposal (learned/proposal.py)
+                              picks the predictor and coders, scored on
+                              the GPU (packaged weights unless `proposal`
+                              is given);
   selector="learned+fallback" as "learned", with exhaustive search wherever
                               the proposal's confidence is low.
 
@@ -64,22 +66,22 @@ class STRIDEEncoder:
         learned = None
         if self.selector != "exhaustive":
             if self.proposal is None:
-                raise ValueError(f"selector={self.selector!r} needs a trained proposal")
+                from ..learned.proposal import LearnedProposal
+                self.proposal = LearnedProposal()
             learned = self.proposal
+        fb = self.selector == "learned+fallback"
         if self.predictor == "canonical":
             pred_n, pred_K = search.CANON_N.copy(), search.CANON_K.copy()
         else:
             A, B, C, V, off = nat.para_samples()
             if learned is not None:
-                pred_n, pred_K = learned.fit_predictor(A, B, C, V, off, stats=st,
-                                                       fallback=self.selector == "learned+fallback")
+                pred_n, pred_K = learned.fit_predictor(nat, A, B, C, V, off, stats=st, fallback=fb)
             else:
                 pred_n, pred_K = search.fit_predictor(A, B, C, V, off, stats=st)
         t1 = time.perf_counter()
-        vals, soff, max_abs = nat.residuals(pred_n, pred_K)
+        vals, soff, max_abs = nat.residuals(pred_n, pred_K, export=learned is None)
         if learned is not None:
-            tagk = learned.select_codes(vals, soff, stats=st,
-                                        fallback=self.selector == "learned+fallback")
+            tagk = learned.select_codes(nat, vals, soff, stats=st, fallback=fb)
         else:
             tagk, _ = search.select_codes(vals, soff)
         t2 = time.perf_counter()

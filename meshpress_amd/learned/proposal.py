@@ -1,0 +1,153 @@
+"""Encoder proposals: the learned one (MLP on the GPU, scored on the GPU,
+exhaustive fallback) and the network-free baseline it is compared with.
+
+LearnedProposal replaces the two searches of encode_dup:
+
+* predictor (encoder/_irlp_fit.py:364-394): the network runs on the
+  canonical-pass features; per axis its per-meshlet class probabilities
+  are summed with weights n_para; the
+  canonical variant and the `top` best-voted variants are scored exactly on
+  all parallelogram samples by the IRLP cost kernel, the best one is
+  refined by `refine_rounds` rounds of +-1 moves on its three weights, and
+  with fallback an axis whose vote share is below `tau_pred` also runs the
+  exhaustive per-axis search, keeping whichever costs fewer bits;
+* coders (paradelta_v5_dup.py:159-171): the network runs again on the
+  features of the residuals under the chosen predictor and each residual
+  stream takes its arg-max (tag, k); with fallback a stream whose class
+  probability is below `tau_coder` is scored exhaustively instead.  The
+  residual values themselves never leave the native encoder unless a
+  stream falls back.
+
+Every choice is a valid DPR4 v3 parameter, so the decoder is unchanged.
+"""
+from __future__ import annotations
+
+import time
+from pathlib import Path
+
+import numpy as np
+
+from ..encoder import search
+from .dataset import VARIANT_K, VARIANTS, class_tagk
+from .mlp import PRED_CLASSES, ProposalMLP
+
+WEIGHTS = Path(__file__).resolve().parent / "weights" / "proposal.npz"
+MOVES = ((0, -1), (0, 1), (1, -1), (1, 1), (2, -1), (2, 1))
+
+
+def gather_streams(vals: np.ndarray, soff: np.ndarray, idx: np.ndarray):
+    """Values and offsets of the streams `idx` (in that order)."""
+    soff = np.asarray(soff, np.int64)
+    lens = soff[idx + 1] - soff[idx]
+    new_off = np.concatenate(([0], np.cumsum(lens))).astype(np.int64)
+    pos = np.repeat(soff[idx] - new_off[:-1], lens) + np.arange(new_off[-1], dtype=np.int64)
+    return vals[pos], new_off
+
+
+class LearnedProposal:
+    def __init__(self, mlp: ProposalMLP | None = None, weights=None, top: int = 2,
+                 refine_rounds: int = 1, tau_pred: float = 0.3, tau_coder: float = 0.5):
+        if mlp is None:
+            path = Path(weights) if weights is not None else WEIGHTS
+            if not path.exists():
+                raise RuntimeError(f"proposal weights {path} are missing (train them with "
+                                   "python -m meshpress_amd.learned.train)")
+            mlp = ProposalMLP.load(path)
+        self.mlp = mlp
+        self.top = int(top)
+        self.refine_rounds = int(refine_rounds)
+        self.tau_pred = float(tau_pred)
+        self.tau_coder = float(tau_coder)
+
+    # ------------------------------------------------------------ predictor
+    def fit_predictor(self, nat, A, B, C, V, off, stats=None, fallback=False):
+        t0 = time.perf_counter()
+        pp, _, _ = self.mlp.infer(nat.features(0))
+        off = np.asarray(off, np.int64)
+        w = np.diff(off).astype(np.float64)
+        pred_n = search.CANON_N.copy()
+        pred_K = search.CANON_K.copy()
+        evals, conf, fb = 0, [], []
+        sub = None
+        if len(A) and w.sum() > 0:
+            for d in range(3):
+                S = w @ pp[:, PRED_CLASSES * d:PRED_CLASSES * (d + 1)].astype(np.float64)
+                order = np.argsort(-S, kind="stable")
+                cand = sorted({0, *order[:self.top].tolist()})
+                cost = search.IrlpCost(A[:, d], B[:, d], C[:, d], V[:, d], off)
+                cc = cost(VARIANTS[cand], VARIANT_K)
+                i = int(np.argmin(cc))
+                best_n, best_c, best_K = VARIANTS[cand[i]].astype(np.int64), int(cc[i]), VARIANT_K
+                canonical = cand[i] == 0
+                for _ in range(self.refine_rounds):
+                    mv = np.repeat(best_n[None], len(MOVES), axis=0)
+                    for j, (k, sg) in enumerate(MOVES):
+                        mv[j, k] += sg
+                    mc = cost(mv, VARIANT_K)
+                    j = int(np.argmin(mc))
+                    if int(mc[j]) >= best_c:
+                        break
+                    best_n, best_c, canonical = mv[j].copy(), int(mc[j]), False
+                evals += cost.evals
+                share = float(S[order[0]] / S.sum()) if S.sum() > 0 else 1.0
+                conf.append(share)
+                if fallback and share < self.tau_pred:
+                    if sub is None:
+                        sub = search.subsample(A, B, C, V, off)
+                    n_e, K_e, c_e, ev = search.fit_axis(A, B, C, V, off, d, sub=sub)
+                    evals += ev
+                    fb.append(d)
+                    if c_e <= best_c:
+                        best_n, best_K, best_c = n_e, K_e, c_e
+                        canonical = K_e == 0 and np.array_equal(n_e, search.CANON_N[d])
+                if canonical:
+                    pred_n[d], pred_K[d] = search.CANON_N[d], 0
+                else:
+                    pred_n[d], pred_K[d] = best_n, best_K
+        if stats is not None:
+            stats.update(irlp_evals=evals, pred_confidence=conf, pred_fallback_axes=fb,
+                         proposal_predictor_s=time.perf_counter() - t0)
+        return pred_n, pred_K
+
+    # ------------------------------------------------------------ coders
+    def select_codes(self, nat, vals, soff, stats=None, fallback=False):
+        t0 = time.perf_counter()
+        _, cls, cp = self.mlp.infer(nat.features(1))
+        soff = np.asarray(soff, np.int64)
+        lens = np.diff(soff)
+        tagk = class_tagk(cls.reshape(-1))
+        tagk[lens == 0] = (1, 0)
+        n_fb = 0
+        if fallback:
+            idx = np.nonzero((cp.reshape(-1) < self.tau_coder) & (lens > 0))[0]
+            n_fb = len(idx)
+            if n_fb:
+                if vals is None:
+                    vals = nat.residual_values()
+                sv, so = gather_streams(vals, soff, idx)
+                tk, _ = search.select_codes(sv, so)
+                tagk[idx] = tk
+        if stats is not None:
+            stats.update(coder_fallback_streams=int(n_fb), coder_streams=int((lens > 0).sum()),
+                         proposal_coder_s=time.perf_counter() - t0)
+        return tagk
+
+
+class BaselineProposal:
+    """The method without the network: canonical predictor, and Rice(k) with
+    k = floor(log2(mean u)) of each stream's zigzag values (the closed-form
+    Rice parameter estimate), read from the same features."""
+
+    def fit_predictor(self, nat, A, B, C, V, off, stats=None, fallback=False):
+        return search.CANON_N.copy(), search.CANON_K.copy()
+
+    def select_codes(self, nat, vals, soff, stats=None, fallback=False):
+        F = nat.features(1)
+        mean_u = np.exp2(F[:, 23:29].astype(np.float64) * 16.0) - 1.0
+        k = np.floor(np.log2(np.maximum(mean_u, 1.0))).astype(np.int64).clip(0, 11)
+        tagk = np.zeros((k.size, 2), np.uint8)
+        tagk[:, 0] = 1
+        tagk[:, 1] = k.reshape(-1)
+        lens = np.diff(np.asarray(soff, np.int64))
+        tagk[lens == 0] = (1, 0)
+        return tagk

@@ -287,3 +287,14 @@ def batch_specs(n_meshes: int = 2048, t_min: int = 1000, t_max: int = 200000,
     fam = rng.integers(0, len(FAMILIES), size=n_meshes)
     return [(FAMILIES[int(f)], int(round(math.exp(l))), seed * 100003 + i)
             for i, (f, l) in enumerate(zip(fam, logt))]
+
+
+def learned_specs(n_meshes: int, seed: int, t_min: int = 20000, t_max: int = 500000):
+    """(family, n_tris, seed) triples for the learned-proposal meshes:
+    families uniform, n_tris log-uniform in [t_min, t_max].  Training uses
+    seed 0 and held-out evaluation seed 1, so the two sets share no mesh."""
+    rng = np.random.default_rng(1000 + seed)
+    logt = rng.uniform(math.log(t_min), math.log(t_max), size=n_meshes)
+    fam = rng.integers(0, len(FAMILIES), size=n_meshes)
+    return [(FAMILIES[int(f)], int(round(math.exp(l))), (seed + 1) * 1000003 + i)
+            for i, (f, l) in enumerate(zip(fam, logt))]
