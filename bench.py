@@ -235,15 +235,18 @@ def learned_encode(head_prep, n_heldout):
         for tag, e in (("exhaustive", ex), ("learned", le)):
             data = e.encode_prepared(prep)
             st = e.last_stats
-            r[tag] = {"bytes": len(data), "search_s": st["predictor_s"] + st["coder_s"]}
+            r[tag] = {"bytes": len(data), "search_s": st["search_s"],
+                      "predictor_s": st["predictor_s"], "coder_s": st["coder_s"]}
             V, T = ParaDeltaV5GpuDecoder(data).decode_to_host()
             outs.append((V.view(np.uint32).copy(), T))
         r["decode_equal"] = bool(np.array_equal(outs[0][0], outs[1][0])
                                  and np.array_equal(outs[0][1], outs[1][1]))
         rows.append(r)
-        log(f"learned {name}: search {r['exhaustive']['search_s'] * 1e3:.1f} -> "
-            f"{r['learned']['search_s'] * 1e3:.1f} ms, bytes {r['exhaustive']['bytes']} -> "
-            f"{r['learned']['bytes']}")
+        ex_, le_ = r["exhaustive"], r["learned"]
+        log(f"learned {name}: search {ex_['search_s'] * 1e3:.1f} ms (predictor "
+            f"{ex_['predictor_s'] * 1e3:.1f} + coders {ex_['coder_s'] * 1e3:.1f}) -> "
+            f"{le_['search_s'] * 1e3:.1f} ms ({le_['predictor_s'] * 1e3:.1f} + "
+            f"{le_['coder_s'] * 1e3:.1f}), bytes {ex_['bytes']} -> {le_['bytes']}")
 
     def agg(rs):
         be = sum(r["exhaustive"]["bytes"] for r in rs)

@@ -174,7 +174,8 @@ int stride_encoder_residuals(stride_encoder* h, const int32_t* pred_n, const int
  * 6 * n_meshlets + 1 offsets. */
 int stride_encoder_export_residuals(const stride_encoder* h, int32_t* vals,
                                     int64_t* stream_off);
-/* Per-meshlet features of the learned proposal, 32 float32 per meshlet (no
+/* Per-meshlet features of the learned proposal, 32 float32 per meshlet of
+ * sel[0..n_sel) (every meshlet when sel is NULL) (no
  * reference counterpart: they feed the MLP that replaces the search of
  * encoder/_irlp_fit.py:364-394 and paradelta_v5_dup.py:159-171).  Stream
  * statistics come from the canonical predictor a + b - c (source 0) or from
@@ -188,7 +189,16 @@ int stride_encoder_export_residuals(const stride_encoder* h, int32_t* vals,
  *   23..28 log2(1 + mean u over stream s) / 16
  *   29..31 clamp(sum(x*y) / sum(x*x), 0, 2) / 2 over the parallelogram
  *          vertices of axis d, x = a + b - 2c, y = v - c (1 when sum(x*x) = 0) */
-int stride_encoder_features(const stride_encoder* h, int32_t source, float* out);
+int stride_encoder_features(const stride_encoder* h, int32_t source, const int32_t* sel,
+                            int64_t n_sel, float* out);
+/* The IRLP fit samples of stride_encoder_para_samples for meshlets
+ * sel[0..n_sel) (all meshlets when sel is NULL), as axis-major int32
+ * columns: cols[(q * 3 + d) * n + i] for q = a, b, c, v, axis d, sample i,
+ * n = off[rows]; off gets rows + 1 offsets.  With cols NULL only off is
+ * written (to size cols).  The subsample of encoder/_irlp_fit.py:72-97 is a
+ * selection of meshlets. */
+int stride_encoder_para_columns(const stride_encoder* h, const int32_t* sel, int64_t n_sel,
+                                int32_t* cols, int64_t* off);
 /* DPR4 v3 bytes (paradelta_v5_dup.py:239-313,412-455).  f13 = center[3],
  * scale, per_coord_err, g_min[3], g_range[3] (float32 values written
  * verbatim); g_bits[3]; tagk = 12 bytes per meshlet ((tag, k) x 3 delta then

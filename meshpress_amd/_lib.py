@@ -81,7 +81,8 @@ SIGNATURES = {
     "stride_stream_costs": (C.c_int, [p, p, i64, p, p]),
     "stride_irlp_cost": (C.c_int, [p, p, p, p, p, i64, p, i32, i32, p, p]),
     "stride_irlp_cost_meshlets": (C.c_int, [p, p, p, p, p, i64, p, i32, i32, p, p]),
-    "stride_encoder_features": (C.c_int, [p, i32, p]),
+    "stride_encoder_features": (C.c_int, [p, i32, p, i64, p]),
+    "stride_encoder_para_columns": (C.c_int, [p, p, i64, p, p]),
     "stride_mlp_sizes": (C.c_int, [P(i32), P(i32), P(i32), P(i32), P(i32)]),
     "stride_mlp_gemm": (C.c_int, [p, i32, i32, p, i32, i32, i32, i32, i32, i32, p, p, i32, p, p,
                                   i32, p]),

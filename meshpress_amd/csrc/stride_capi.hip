@@ -882,11 +882,31 @@ int stride_encoder_export_residuals(const stride_encoder* h, int32_t* vals, int6
   return STRIDE_OK;
 }
 
-int stride_encoder_features(const stride_encoder* h, int32_t source, float* out) {
+static int check_selection(const stride_encoder* h, const int32_t* sel, int64_t n_sel) {
+  if (!sel) return STRIDE_OK;
+  const int64_t n_m = (int64_t)h->prep.plans.size();
+  if (n_sel < 0) return fail(STRIDE_E_ARG, "negative selection size");
+  for (int64_t i = 0; i < n_sel; ++i)
+    if (sel[i] < 0 || sel[i] >= n_m) return fail(STRIDE_E_ARG, "meshlet index out of range");
+  return STRIDE_OK;
+}
+
+int stride_encoder_features(const stride_encoder* h, int32_t source, const int32_t* sel,
+                            int64_t n_sel, float* out) {
   if (!h || !out) return fail(STRIDE_E_ARG, "null argument");
   if (source != 0 && source != 1) return fail(STRIDE_E_ARG, "source must be 0 or 1");
   if (source == 1 && !h->have_res) return fail(STRIDE_E_ARG, "residuals not computed");
-  meshlet_features(h->prep, h->codes.data(), source ? &h->res : nullptr, out, h->n_threads);
+  if (int rc = check_selection(h, sel, n_sel)) return rc;
+  meshlet_features(h->prep, h->codes.data(), source ? &h->res : nullptr, sel, n_sel, out,
+                   h->n_threads);
+  return STRIDE_OK;
+}
+
+int stride_encoder_para_columns(const stride_encoder* h, const int32_t* sel, int64_t n_sel,
+                                int32_t* cols, int64_t* off) {
+  if (!h || !off) return fail(STRIDE_E_ARG, "null argument");
+  if (int rc = check_selection(h, sel, n_sel)) return rc;
+  para_columns(h->prep, h->codes.data(), sel, n_sel, cols, off, h->n_threads);
   return STRIDE_OK;
 }
 

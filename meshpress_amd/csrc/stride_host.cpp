@@ -1028,16 +1028,45 @@ void compute_residuals(const Prep& P, const int64_t* codes, const int32_t* pred_
 // the first pass does not depend on the predictor being chosen.  Sums are
 // exact integers; the float conversions happen once at the end.
 // ---------------------------------------------------------------------------
+// Parallelogram samples (global codes of a, b, c and v for every kind-1
+// walk entry) of the selected meshlets, axis-major columns.
+void para_columns(const Prep& P, const int64_t* codes, const int32_t* sel, int64_t n_sel,
+                  int32_t* cols, int64_t* off, int n_threads) {
+  const int64_t n_rows = sel ? n_sel : (int64_t)P.plans.size();
+  off[0] = 0;
+  for (int64_t r = 0; r < n_rows; ++r) {
+    const int64_t m = sel ? sel[r] : r;
+    int64_t c = 0;
+    for (const auto& w : P.walks[m]) c += w.kind != 0;
+    off[r + 1] = off[r] + c;
+  }
+  if (!cols) return;
+  const int64_t n = off[n_rows];
+  parallel_for(n_rows, n_threads, [&](int64_t r, int) {
+    const int64_t m = sel ? sel[r] : r;
+    const Plan& pl = P.plans[m];
+    int64_t i = off[r];
+    for (const auto& w : P.walks[m]) {
+      if (!w.kind) continue;
+      const int64_t g[4] = {pl.l2g[w.a], pl.l2g[w.b], pl.l2g[w.c], pl.l2g[w.v]};
+      for (int q = 0; q < 4; ++q)
+        for (int d = 0; d < 3; ++d) cols[(q * 3 + d) * n + i] = (int32_t)codes[3 * g[q] + d];
+      ++i;
+    }
+  });
+}
+
 static inline uint64_t zz_host(int64_t r) { return r < 0 ? (uint64_t)(-r) * 2 - 1 : (uint64_t)r * 2; }
 static inline int bitlen1(uint64_t u) { return 63 - __builtin_clzll(u + 1); }   // floor(log2(u + 1))
 
-void meshlet_features(const Prep& P, const int64_t* codes, const Residuals* R, float* out,
-                      int n_threads) {
-  const int64_t n_m = (int64_t)P.plans.size();
-  parallel_for(n_m, n_threads, [&](int64_t m, int) {
+void meshlet_features(const Prep& P, const int64_t* codes, const Residuals* R, const int32_t* sel,
+                      int64_t n_sel, float* out, int n_threads) {
+  const int64_t n_rows = sel ? n_sel : (int64_t)P.plans.size();
+  parallel_for(n_rows, n_threads, [&](int64_t row, int) {
+    const int64_t m = sel ? sel[row] : row;
     const Plan& pl = P.plans[m];
     const auto& walk = P.walks[m];
-    float* f = out + (size_t)kFeatures * m;
+    float* f = out + (size_t)kFeatures * row;
     const int n_local = (int)walk.size();
     const int n_tris = (int)(pl.tris_local.size() / 3);
     const int n_strips = (int)pl.strip_len.size();

@@ -107,11 +107,18 @@ void compute_residuals(const Prep& P, const int64_t* codes, const int32_t* pred_
                        const int32_t* pred_K, Residuals& R, int n_threads);
 
 // Per-meshlet features of the learned proposal, kFeatures floats per
-// meshlet (layout in include/stride_hip.h, stride_encoder_features); stream
-// statistics from R, or from canonical residuals when R is null.
+// selected meshlet (all when sel is null; layout in include/stride_hip.h,
+// stride_encoder_features); stream statistics from R, or from canonical
+// residuals when R is null.
 constexpr int kFeatures = 32;
-void meshlet_features(const Prep& P, const int64_t* codes, const Residuals* R, float* out,
-                      int n_threads);
+void meshlet_features(const Prep& P, const int64_t* codes, const Residuals* R, const int32_t* sel,
+                      int64_t n_sel, float* out, int n_threads);
+
+// Parallelogram samples of meshlets sel[0..n_sel) (all when sel is null):
+// off gets the row offsets; cols (when not null) the codes of a, b, c, v
+// as cols[(q * 3 + d) * n + i], n = off[rows].
+void para_columns(const Prep& P, const int64_t* codes, const int32_t* sel, int64_t n_sel,
+                  int32_t* cols, int64_t* off, int n_threads);
 
 
 // tagk: per meshlet 12 bytes: (tag,k) x 3 delta then (tag,k) x 3 para.

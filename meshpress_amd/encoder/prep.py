@@ -176,13 +176,30 @@ class NativePrep:
         _lib.check(self._L.stride_encoder_export_residuals(self._h, _lib.ptr(vals), None))
         return vals
 
-    def features(self, source: int = 0) -> np.ndarray:
-        """Per-meshlet proposal features, float32 (n_meshlets, 32)
-        (stride_encoder_features): stream statistics of the canonical
-        predictor (source 0) or of the last residuals() call (source 1)."""
-        out = np.zeros((self.n_meshlets, 32), np.float32)
-        _lib.check(self._L.stride_encoder_features(self._h, int(source), _lib.ptr(out)))
+    def features(self, source: int = 0, sel=None) -> np.ndarray:
+        """Proposal features, float32 (rows, 32), of the meshlets `sel` (all
+        when None) (stride_encoder_features): stream statistics of the
+        canonical predictor (source 0) or of the last residuals() call
+        (source 1)."""
+        s = None if sel is None else np.ascontiguousarray(sel, np.int32)
+        rows = self.n_meshlets if s is None else len(s)
+        out = np.zeros((rows, 32), np.float32)
+        _lib.check(self._L.stride_encoder_features(self._h, int(source), _lib.ptr(s), rows,
+                                                   _lib.ptr(out)))
         return out
+
+    def para_columns(self, sel=None):
+        """IRLP fit samples of the meshlets `sel` (all when None) as int32
+        columns (4, 3, n) -- a, b, c, v by axis -- and int64 row offsets."""
+        s = None if sel is None else np.ascontiguousarray(sel, np.int32)
+        rows = self.n_meshlets if s is None else len(s)
+        off = np.zeros(rows + 1, np.int64)
+        _lib.check(self._L.stride_encoder_para_columns(self._h, _lib.ptr(s), rows, None,
+                                                       _lib.ptr(off)))
+        cols = np.zeros((4, 3, int(off[-1])), np.int32)
+        _lib.check(self._L.stride_encoder_para_columns(self._h, _lib.ptr(s), rows, _lib.ptr(cols),
+                                                       _lib.ptr(off)))
+        return cols, off
 
     def write(self, f13, g_bits, pred_n, pred_K, tagk) -> bytes:
         f13 = np.ascontiguousarray(np.asarray(f13, dtype=np.float32).reshape(13))

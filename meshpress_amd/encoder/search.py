@@ -143,59 +143,79 @@ def search_axis(cost, w_axis):
     return best
 
 
+def subsample_sel(n_m, n_keep=SUBSAMPLE_MESHLETS):
+    """Sorted meshlet selection of the IRLP fit (encoder/_irlp_fit.py:72-97:
+    default_rng(0).choice without replacement), or None to keep every meshlet."""
+    if n_m <= n_keep:
+        return None
+    return np.sort(np.random.default_rng(0).choice(n_m, size=n_keep, replace=False))
+
+
+def as_columns(A, B, C, V):
+    """(n, 3) sample arrays -> int32 columns (4, 3, n)."""
+    return np.ascontiguousarray(np.stack([np.asarray(x).T for x in (A, B, C, V)]), dtype=np.int32)
+
+
 def subsample(A, B, C, V, off, n_keep=SUBSAMPLE_MESHLETS):
     n_m = len(off) - 1
-    if n_m <= n_keep:
+    sel = subsample_sel(n_m, n_keep)
+    if sel is None:
         return A, B, C, V, off
-    sel = np.sort(np.random.default_rng(0).choice(n_m, size=n_keep, replace=False))
     rows = np.concatenate([np.arange(off[i], off[i + 1]) for i in sel])
     counts = np.diff(off)[sel]
     return A[rows], B[rows], C[rows], V[rows], np.concatenate(([0], np.cumsum(counts)))
 
 
-def fit_predictor(A, B, C, V, off, stats=None):
-    """(n 3x3 int64, K 3 int64): exhaustive IRLP search on the GPU."""
-    off = np.asarray(off, dtype=np.int64)
-    n_m = len(off) - 1
-    if n_m == 0 or len(A) == 0:
-        return CANON_N.copy(), CANON_K.copy()
-    As, Bs, Cs, Vs, off_s = subsample(A, B, C, V, off)
-    n_fit = np.zeros((3, 3), dtype=np.int64)
-    K_fit = np.zeros(3, dtype=np.int64)
+def _axis_cost(cols, off, d):
+    return IrlpCost(cols[0, d], cols[1, d], cols[2, d], cols[3, d], off)
+
+
+def fit_axis_cols(sub, full, d):
+    """Exhaustive fit of axis d (the per-axis body of fit_predictor).
+    sub, full: (columns (4, 3, n), offsets) of the subsample and of every
+    meshlet.  ((n0, n1, n2) int64, K, bits over all samples, evaluations)."""
+    cols, off_s = sub
+    X = np.stack([cols[0, d], cols[1, d], cols[2, d]], axis=1).astype(np.float64)
+    w = irls_l1(X, cols[3, d].astype(np.float64))
+    cost = _axis_cost(cols, off_s, d)
+    K, ns, _ = search_axis(cost, w)
+    fcost = _axis_cost(full[0], full[1], d)
+    b_fit = int(fcost(np.array(ns)[None], int(K))[0])
+    b_can = int(fcost(CANON_N[d][None], 0)[0])
+    if b_can < b_fit:
+        return CANON_N[d].copy(), 0, b_can, cost.evals + fcost.evals
+    return np.array(ns, dtype=np.int64), int(K), b_fit, cost.evals + fcost.evals
+
+
+def fit_predictor_cols(sub, full, stats=None):
+    """(n 3x3 int64, K 3 int64) from subsample and full sample columns."""
+    n_fit = CANON_N.copy()
+    K_fit = CANON_K.copy()
+    if len(full[1]) <= 1 or full[0].shape[2] == 0:
+        if stats is not None:
+            stats["irlp_evals"] = 0
+        return n_fit, K_fit
     evals = 0
     for d in range(3):
-        X = np.stack([As[:, d], Bs[:, d], Cs[:, d]], axis=1).astype(np.float64)
-        w = irls_l1(X, Vs[:, d].astype(np.float64))
-        cost = IrlpCost(As[:, d], Bs[:, d], Cs[:, d], Vs[:, d], off_s)
-        K, ns, _ = search_axis(cost, w)
-        n_fit[d] = ns
-        K_fit[d] = K
-        evals += cost.evals
-    for d in range(3):
-        full = IrlpCost(A[:, d], B[:, d], C[:, d], V[:, d], off)
-        b_fit = int(full(n_fit[d][None], int(K_fit[d]))[0])
-        b_can = int(full(CANON_N[d][None], 0)[0])
-        if b_can < b_fit:
-            n_fit[d] = CANON_N[d]
-            K_fit[d] = 0
-        evals += full.evals
+        n_fit[d], K_fit[d], _, ev = fit_axis_cols(sub, full, d)
+        evals += ev
     if stats is not None:
         stats["irlp_evals"] = evals
     return n_fit, K_fit
 
 
-def fit_axis(A, B, C, V, off, d, sub=None):
-    """Exhaustive fit of axis d alone (the per-axis body of fit_predictor):
-    ((n0, n1, n2) int64, K, bits over all samples, candidate evaluations)."""
+def fit_predictor(A, B, C, V, off, stats=None):
+    """(n 3x3 int64, K 3 int64): exhaustive IRLP search on the GPU
+    (encoder/_irlp_fit.py:364-394) over (n, 3) sample arrays."""
     off = np.asarray(off, dtype=np.int64)
-    As, Bs, Cs, Vs, off_s = sub if sub is not None else subsample(A, B, C, V, off)
-    X = np.stack([As[:, d], Bs[:, d], Cs[:, d]], axis=1).astype(np.float64)
-    w = irls_l1(X, Vs[:, d].astype(np.float64))
-    cost = IrlpCost(As[:, d], Bs[:, d], Cs[:, d], Vs[:, d], off_s)
-    K, ns, _ = search_axis(cost, w)
-    full = IrlpCost(A[:, d], B[:, d], C[:, d], V[:, d], off)
-    b_fit = int(full(np.array(ns)[None], int(K))[0])
-    b_can = int(full(CANON_N[d][None], 0)[0])
-    if b_can < b_fit:
-        return CANON_N[d].copy(), 0, b_can, cost.evals + full.evals
-    return np.array(ns, dtype=np.int64), int(K), b_fit, cost.evals + full.evals
+    As, Bs, Cs, Vs, off_s = subsample(A, B, C, V, off)
+    return fit_predictor_cols((as_columns(As, Bs, Cs, Vs), np.asarray(off_s, np.int64)),
+                              (as_columns(A, B, C, V), off), stats)
+
+
+def fit_predictor_native(nat, stats=None):
+    """fit_predictor with the samples taken from the native encoder handle."""
+    sel = subsample_sel(nat.n_meshlets)
+    full = nat.para_columns()
+    sub = full if sel is None else nat.para_columns(sel)
+    return fit_predictor_cols(sub, full, stats)

@@ -62,7 +62,6 @@ class STRIDEEncoder:
             raise ValueError(f"predictor must be 'generalized' or 'canonical', got {self.predictor!r}")
         st = self.last_stats = {}
         nat = prep["native"]
-        t0 = time.perf_counter()
         learned = None
         if self.selector != "exhaustive":
             if self.proposal is None:
@@ -70,25 +69,28 @@ class STRIDEEncoder:
                 self.proposal = LearnedProposal()
             learned = self.proposal
         fb = self.selector == "learned+fallback"
+        # search time = predictor choice + coder choice; the residual pass
+        # between them is the same work for every selector and is timed apart
+        t0 = time.perf_counter()
         if self.predictor == "canonical":
             pred_n, pred_K = search.CANON_N.copy(), search.CANON_K.copy()
+        elif learned is not None:
+            pred_n, pred_K = learned.fit_predictor(nat, stats=st, fallback=fb)
         else:
-            A, B, C, V, off = nat.para_samples()
-            if learned is not None:
-                pred_n, pred_K = learned.fit_predictor(nat, A, B, C, V, off, stats=st, fallback=fb)
-            else:
-                pred_n, pred_K = search.fit_predictor(A, B, C, V, off, stats=st)
+            pred_n, pred_K = search.fit_predictor_native(nat, stats=st)
         t1 = time.perf_counter()
-        vals, soff, max_abs = nat.residuals(pred_n, pred_K, export=learned is None)
-        if learned is not None:
-            tagk = learned.select_codes(nat, vals, soff, stats=st, fallback=fb)
-        else:
-            tagk, _ = search.select_codes(vals, soff)
+        _, soff, max_abs = nat.residuals(pred_n, pred_K, export=False)
         t2 = time.perf_counter()
+        if learned is not None:
+            tagk = learned.select_codes(nat, soff, stats=st, fallback=fb)
+        else:
+            tagk, _ = search.select_codes(nat.residual_values(), soff)
+        t3 = time.perf_counter()
         data = nat.write(header_f32(prep), prep["g_bits"], pred_n, pred_K,
                          tagk.reshape(prep["n_meshlets"], 12))
-        t3 = time.perf_counter()
-        st.update(predictor_s=t1 - t0, coder_s=t2 - t1, write_s=t3 - t2,
+        t4 = time.perf_counter()
+        st.update(predictor_s=t1 - t0, residual_s=t2 - t1, coder_s=t3 - t2, write_s=t4 - t3,
+                  search_s=(t1 - t0) + (t3 - t2),
                   pred_n=pred_n.tolist(), pred_K=pred_K.tolist(), max_abs=max_abs)
         if self.verbose:
             print(f"  dup: n_v_src={prep['n_v']:,} n_meshlets={prep['n_meshlets']:,} "

@@ -47,8 +47,9 @@ def class_tagk(cls: np.ndarray) -> np.ndarray:
     return out
 
 
-def variant_costs(A, B, C, V, off) -> np.ndarray:
-    """int64 (n_meshlets, 3, PRED_CLASSES): per-meshlet bits of every variant."""
+def variant_costs(cols, off) -> np.ndarray:
+    """int64 (n_meshlets, 3, PRED_CLASSES): per-meshlet bits of every variant;
+    cols (4, 3, n) sample columns (a, b, c, v by axis), off meshlet offsets."""
     torch = search._torch()
     from .. import _lib
     n_m = len(off) - 1
@@ -60,9 +61,9 @@ def variant_costs(A, B, C, V, off) -> np.ndarray:
     d_out = torch.empty((n_m, PRED_CLASSES), dtype=torch.int64, device="cuda")
     z = np.zeros(1, np.int32)
     for d in range(3):
-        cols = [search._dev(x[:, d] if len(x) else z, torch.int32) for x in (A, B, C, V)]
+        dcols = [search._dev(cols[q, d] if cols.shape[2] else z, torch.int32) for q in range(4)]
         _lib.check(_lib.lib().stride_irlp_cost_meshlets(
-            *[_lib.ptr(c) for c in cols], _lib.ptr(d_off), n_m, _lib.ptr(d_c), PRED_CLASSES,
+            *[_lib.ptr(c) for c in dcols], _lib.ptr(d_off), n_m, _lib.ptr(d_c), PRED_CLASSES,
             VARIANT_K, _lib.ptr(d_out), _lib.stream_ptr()))
         out[:, d, :] = d_out.cpu().numpy()
     return out
@@ -78,14 +79,14 @@ def label_prep(prep: dict, stats: dict | None = None):
     nat = prep["native"]
     n_m = nat.n_meshlets
     F0 = nat.features(0)
-    A, B, C, V, off = nat.para_samples()
+    cols, off = nat.para_columns()
     n_para = np.diff(off)
     Y0 = np.full((n_m, N_HEADS), IGNORE, np.uint8)
-    vc = variant_costs(A, B, C, V, off)
+    vc = variant_costs(cols, off)
     best = vc.argmin(axis=2).astype(np.uint8)          # first minimum = lowest j
     has = n_para > 0
     Y0[has, 0:3] = best[has]
-    pred_n, pred_K = search.fit_predictor(A, B, C, V, off)
+    pred_n, pred_K = search.fit_predictor_native(nat)
     vals, soff, _ = nat.residuals(pred_n, pred_K)
     F1 = nat.features(1)
     tagk, _ = search.select_codes(vals, soff)

@@ -5,12 +5,14 @@ LearnedProposal replaces the two searches of encode_dup:
 
 * predictor (encoder/_irlp_fit.py:364-394): the network runs on the
   canonical-pass features; per axis its per-meshlet class probabilities
-  are summed with weights n_para; the
-  canonical variant and the `top` best-voted variants are scored exactly on
-  all parallelogram samples by the IRLP cost kernel, the best one is
-  refined by `refine_rounds` rounds of +-1 moves on its three weights, and
-  with fallback an axis whose vote share is below `tau_pred` also runs the
-  exhaustive per-axis search, keeping whichever costs fewer bits;
+  are summed with weights n_para; the canonical variant and the `top`
+  best-voted variants are scored exactly by the IRLP cost kernel on the
+  parallelogram samples of the meshlets the exhaustive fit would search
+  (its 4,000-meshlet subsample), the best one is refined by
+  `refine_rounds` rounds of +-1 moves on its three weights, and with
+  fallback an axis whose vote share is below `tau_pred` also runs the
+  exhaustive per-axis search, keeping whichever costs fewer bits over every
+  meshlet;
 * coders (paradelta_v5_dup.py:159-171): the network runs again on the
   features of the residuals under the chosen predictor and each residual
   stream takes its arg-max (tag, k); with fallback a stream whose class
@@ -60,21 +62,25 @@ class LearnedProposal:
         self.tau_coder = float(tau_coder)
 
     # ------------------------------------------------------------ predictor
-    def fit_predictor(self, nat, A, B, C, V, off, stats=None, fallback=False):
+    def fit_predictor(self, nat, stats=None, fallback=False):
+        """(pred_n 3x3, pred_K 3) for the mesh behind the native handle.
+        Votes and candidate scores use the meshlets of the IRLP fit's
+        subsample (every meshlet of a mesh with at most 4,000)."""
         t0 = time.perf_counter()
-        pp, _, _ = self.mlp.infer(nat.features(0))
-        off = np.asarray(off, np.int64)
+        sel = search.subsample_sel(nat.n_meshlets)
+        cols, off = nat.para_columns(sel)
+        pp, _, _ = self.mlp.infer(nat.features(0, sel))
         w = np.diff(off).astype(np.float64)
         pred_n = search.CANON_N.copy()
         pred_K = search.CANON_K.copy()
         evals, conf, fb = 0, [], []
-        sub = None
-        if len(A) and w.sum() > 0:
+        full = None
+        if cols.shape[2] and w.sum() > 0:
             for d in range(3):
                 S = w @ pp[:, PRED_CLASSES * d:PRED_CLASSES * (d + 1)].astype(np.float64)
                 order = np.argsort(-S, kind="stable")
                 cand = sorted({0, *order[:self.top].tolist()})
-                cost = search.IrlpCost(A[:, d], B[:, d], C[:, d], V[:, d], off)
+                cost = search.IrlpCost(cols[0, d], cols[1, d], cols[2, d], cols[3, d], off)
                 cc = cost(VARIANTS[cand], VARIANT_K)
                 i = int(np.argmin(cc))
                 best_n, best_c, best_K = VARIANTS[cand[i]].astype(np.int64), int(cc[i]), VARIANT_K
@@ -92,13 +98,18 @@ class LearnedProposal:
                 share = float(S[order[0]] / S.sum()) if S.sum() > 0 else 1.0
                 conf.append(share)
                 if fallback and share < self.tau_pred:
-                    if sub is None:
-                        sub = search.subsample(A, B, C, V, off)
-                    n_e, K_e, c_e, ev = search.fit_axis(A, B, C, V, off, d, sub=sub)
-                    evals += ev
+                    # the exhaustive per-axis search; both results are then
+                    # compared over every meshlet
+                    if full is None:
+                        full = nat.para_columns() if sel is not None else (cols, off)
+                    n_e, K_e, c_e, ev = search.fit_axis_cols((cols, off), full, d)
+                    fcost = search.IrlpCost(full[0][0, d], full[0][1, d], full[0][2, d],
+                                            full[0][3, d], full[1])
+                    c_l = int(fcost(best_n[None], int(best_K))[0])
+                    evals += ev + fcost.evals
                     fb.append(d)
-                    if c_e <= best_c:
-                        best_n, best_K, best_c = n_e, K_e, c_e
+                    if c_e <= c_l:
+                        best_n, best_K = n_e, K_e
                         canonical = K_e == 0 and np.array_equal(n_e, search.CANON_N[d])
                 if canonical:
                     pred_n[d], pred_K[d] = search.CANON_N[d], 0
@@ -110,7 +121,8 @@ class LearnedProposal:
         return pred_n, pred_K
 
     # ------------------------------------------------------------ coders
-    def select_codes(self, nat, vals, soff, stats=None, fallback=False):
+    def select_codes(self, nat, soff, stats=None, fallback=False):
+        """(tag, k) per residual stream of the last residuals() call."""
         t0 = time.perf_counter()
         _, cls, cp = self.mlp.infer(nat.features(1))
         soff = np.asarray(soff, np.int64)
@@ -122,9 +134,7 @@ class LearnedProposal:
             idx = np.nonzero((cp.reshape(-1) < self.tau_coder) & (lens > 0))[0]
             n_fb = len(idx)
             if n_fb:
-                if vals is None:
-                    vals = nat.residual_values()
-                sv, so = gather_streams(vals, soff, idx)
+                sv, so = gather_streams(nat.residual_values(), soff, idx)
                 tk, _ = search.select_codes(sv, so)
                 tagk[idx] = tk
         if stats is not None:
@@ -138,10 +148,10 @@ class BaselineProposal:
     k = floor(log2(mean u)) of each stream's zigzag values (the closed-form
     Rice parameter estimate), read from the same features."""
 
-    def fit_predictor(self, nat, A, B, C, V, off, stats=None, fallback=False):
+    def fit_predictor(self, nat, stats=None, fallback=False):
         return search.CANON_N.copy(), search.CANON_K.copy()
 
-    def select_codes(self, nat, vals, soff, stats=None, fallback=False):
+    def select_codes(self, nat, soff, stats=None, fallback=False):
         F = nat.features(1)
         mean_u = np.exp2(F[:, 23:29].astype(np.float64) * 16.0) - 1.0
         k = np.floor(np.log2(np.maximum(mean_u, 1.0))).astype(np.int64).clip(0, 11)

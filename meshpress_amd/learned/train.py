@@ -161,7 +161,7 @@ def evaluate_encodes(specs, net, workers, tau_pred=0.3, tau_coder=0.5):
                 same = True
             else:
                 same = bool(np.array_equal(V.view(np.uint32), ref[0]) and np.array_equal(T, ref[1]))
-            row[name] = {"bytes": len(data), "search_s": st["predictor_s"] + st["coder_s"],
+            row[name] = {"bytes": len(data), "search_s": st["search_s"],
                          "predictor_s": st["predictor_s"], "coder_s": st["coder_s"],
                          "decode_equal": same, "pred_n": st["pred_n"], "pred_K": st["pred_K"],
                          "irlp_evals": st.get("irlp_evals"),

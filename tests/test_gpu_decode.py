@@ -43,11 +43,18 @@ def test_hand_streams(oracle):
         assert_same(V, T, Vo, To)
 
 
-@pytest.mark.parametrize("path", sorted(GOLD.glob("*_*.npz")), ids=lambda p: p.stem)
+def stream_fixtures():
+    out = []
+    for p in sorted(GOLD.glob("*_*.npz")):
+        with np.load(p) as z:
+            if "data" in z:
+                out.append(p)
+    return out
+
+
+@pytest.mark.parametrize("path", stream_fixtures(), ids=lambda p: p.stem)
 def test_golden_streams(path):
     z = np.load(path)
-    if "data" not in z:
-        pytest.skip("not a stream fixture")
     V, T = gpu_decode(z["data"].tobytes())
     assert sha(V) == z["V_sha"].tobytes().hex()
     assert sha(T.astype(np.int64)) == z["T_sha"].tobytes().hex()

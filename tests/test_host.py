@@ -115,6 +115,36 @@ def test_para_samples(oracle):
     assert np.array_equal(A, z["samples_A"]) and np.array_equal(Vs, z["samples_V"])
     assert np.array_equal(B, z["samples_B"]) and np.array_equal(C, z["samples_C"])
     assert np.array_equal(off, z["samples_off"])
+    # the same samples as axis-major columns, whole and for a meshlet selection
+    from meshpress_amd.encoder import search
+    nat = prep["native"]
+    cols, off2 = nat.para_columns()
+    assert np.array_equal(off2, off) and np.array_equal(cols, search.as_columns(A, B, C, Vs))
+    sel = np.array([0, 2, 3, nat.n_meshlets - 1])
+    sel = np.unique(sel[sel < nat.n_meshlets])
+    cs, os_ = nat.para_columns(sel)
+    rows = np.concatenate([np.arange(off[i], off[i + 1]) for i in sel])
+    assert np.array_equal(os_, np.concatenate(([0], np.cumsum(np.diff(off)[sel]))))
+    assert np.array_equal(cs, search.as_columns(A[rows], B[rows], C[rows], Vs[rows]))
+    F = nat.features(0)
+    assert np.array_equal(nat.features(0, sel), F[sel])
+    with pytest.raises(ValueError):
+        nat.para_columns(np.array([nat.n_meshlets]))
+
+
+def test_subsample_selection_matches_row_subsample():
+    from meshpress_amd.encoder import search
+    rng = np.random.default_rng(3)
+    counts = rng.integers(0, 9, 5000)
+    off = np.concatenate(([0], np.cumsum(counts)))
+    n = int(off[-1])
+    A, B, C, V = (rng.integers(0, 4096, (n, 3)).astype(np.int32) for _ in range(4))
+    As, Bs, Cs, Vs, offs = search.subsample(A, B, C, V, off)
+    sel = search.subsample_sel(5000)
+    assert len(sel) == search.SUBSAMPLE_MESHLETS and np.all(np.diff(sel) > 0)
+    rows = np.concatenate([np.arange(off[i], off[i + 1]) for i in sel])
+    assert np.array_equal(As, A[rows]) and np.array_equal(offs, np.concatenate(([0], np.cumsum(counts[sel]))))
+    assert search.subsample_sel(4000) is None
 
 
 def test_quantiser_rounds_half_even():
