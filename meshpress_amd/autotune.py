@@ -4,6 +4,7 @@ Times candidate launch shapes on a decoder (ParaDeltaV5GpuDecoder or
 BatchDecoder, anything with set_tune/get_tune/decode) with HIP events on the
 launch stream and keeps the fastest.  The search space:
 
+  mode 2 (split groups, 4 lanes per meshlet): group_size x lds_bytes
   mode 1 (lane-per-meshlet groups):  group_size x lds_bytes x grid
   mode 0 (wave per meshlet):         waves x stage_words (x persistent queue)
 
@@ -21,8 +22,13 @@ GROUP_SIZES = (16, 32, 48, 64)
 LDS_BUDGETS = (16384, 24576, 32768, 40960, 49152, 65536)
 
 
+SPLIT_GROUP_SIZES = (4, 8, 12, 16)
+
+
 def candidates(include_wave_mode: bool = True, grids=(0,)):
     out = []
+    for gs, lds, grid in itertools.product(SPLIT_GROUP_SIZES, LDS_BUDGETS, grids):
+        out.append(dict(mode=2, group_size=gs, lds_bytes=lds, grid=grid, use_queue=0))
     for gs, lds, grid in itertools.product(GROUP_SIZES, LDS_BUDGETS, grids):
         out.append(dict(mode=1, group_size=gs, lds_bytes=lds, grid=grid, use_queue=0))
     if include_wave_mode:

@@ -312,8 +312,10 @@ struct DecodeSet {
     for (int pass = 0; pass < 2; ++pass) {
       groups.clear();
       g_base.assign(n_meshes + 1, 0);
-      if (cfg.mode == 1) {
+      if (cfg.mode == 1 || cfg.mode == 2) {
         const uint64_t budget = (uint64_t)cfg.lds_bytes;
+        const bool split = cfg.mode == 2;
+        const uint32_t gmax = split ? std::min(cfg.group_size, 16) : cfg.group_size;
         for (int i = 0; i < n_meshes; ++i) {
           g_base[i] = (uint32_t)groups.size();
           const uint64_t bb = mp[i].blob_byte * 8ull;
@@ -322,7 +324,9 @@ struct DecodeSet {
           bool open = false;
           auto close = [&]() {
             if (open) {
-              cur.codes_off = (cur.n_words + 1) / 2;
+              // mode 1: packed codes after the words, in 8-byte units;
+              // mode 2: walk/codes blocks after the words, in 4-byte units
+              cur.codes_off = split ? ((cur.n_words + 3) & ~3u) : (cur.n_words + 1) / 2;
               groups.push_back(cur);
             }
             open = false;
@@ -332,10 +336,11 @@ struct DecodeSet {
             const MeshletEntry& e = tab[j];
             const uint64_t lo = (bb + e.start_bit) >> 5;
             const uint64_t hi = ((bb + e.end_bit + 31) >> 5) + 2;
-            auto need = [](uint64_t words, uint64_t nloc) {
-              return 8 * ((words + 1) / 2) + 8 * nloc;
+            auto need = [split](uint64_t words, uint64_t nloc) {
+              return split ? 4 * ((words + 3) & ~3ull) + 16 * nloc
+                           : 8 * ((words + 1) / 2) + 8 * nloc;
             };
-            if (open && (j - cur.m0 < (uint32_t)cfg.group_size) &&
+            if (open && (j - cur.m0 < gmax) &&
                 need(hi - cur.w_lo, codes + e.n_local) <= budget) {
               cur.m1 = j + 1;
               cur.n_words = (uint32_t)(hi - cur.w_lo);
@@ -431,6 +436,7 @@ struct DecodeSet {
     g.overflow = flags;
     g.lds_bytes = (uint32_t)cfg.lds_bytes;
     g.grid = (uint32_t)cfg.grid;
+    g.split = cfg.mode == 2;
     return g;
   }
 
@@ -455,7 +461,7 @@ struct DecodeSet {
 
   // Enqueue the decode of meshes [mesh0, mesh1).
   int enqueue(float* V, uint32_t* T, hipStream_t s, int mesh0, int mesh1) {
-    if (cfg.mode == 1) {
+    if (cfg.mode == 1 || cfg.mode == 2) {
       const uint32_t g0 = g_base[mesh0], g1 = g_base[mesh1];
       if (g1 > g0) HIP_TRY(launch_group(group_args(V, T, g0, g1, nullptr), s));
       const uint32_t l0 = l_base[mesh0], l1 = l_base[mesh1];
@@ -524,13 +530,14 @@ struct DecodeSet {
     if (!t) return fail(STRIDE_E_ARG, "null tune");
     bool ok_w = t->waves == 1 || t->waves == 2 || t->waves == 4;
     bool ok_s = t->stage_words == 256 || t->stage_words == 512 || t->stage_words == 1024;
-    bool ok_m = t->mode == 0 || t->mode == 1;
-    bool ok_g = t->group_size >= 1 && t->group_size <= 64;
+    bool ok_m = t->mode == 0 || t->mode == 1 || t->mode == 2;
+    bool ok_g = t->group_size >= 1 && t->group_size <= (t->mode == 2 ? 16 : 64);
     bool ok_l = t->lds_bytes >= 8192 && t->lds_bytes <= 65536 && t->lds_bytes % 8 == 0;
     if (!ok_w || !ok_s || !ok_m || !ok_g || !ok_l || t->grid < 0 || (t->use_queue && t->grid == 0))
       return fail(STRIDE_E_ARG,
-                  "tune: mode in {0,1}, waves in {1,2,4}, stage_words in {256,512,1024}, "
-                  "group_size in [1,64], lds_bytes in [8192,65536] (multiple of 8), grid >= 0, "
+                  "tune: mode in {0,1,2}, waves in {1,2,4}, stage_words in {256,512,1024}, "
+                  "group_size in [1,64] ([1,16] for mode 2), lds_bytes in [8192,65536] "
+                  "(multiple of 8), grid >= 0, "
                   "use_queue needs a persistent grid");
     const bool relayout = t->mode != cfg.mode || t->group_size != cfg.group_size ||
                           t->lds_bytes != cfg.lds_bytes;

@@ -361,6 +361,72 @@ __global__ __launch_bounds__(64) void stride_decode_group_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Split decode: one wave per group of up to 16 consecutive meshlets; lane
+// l works on meshlet l % 16 with role l / 16 (0: tokens, 1..3: axis x, y,
+// z).  LDS: the group's staged words, then per meshlet a block of
+// 4 n_local words -- walk, codes x, codes y, codes z.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void stride_decode_split_kernel(
+    const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
+    const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
+    uint32_t n_groups, float* __restrict__ V, uint32_t* __restrict__ T,
+    uint32_t* __restrict__ overflow) {
+  extern __shared__ uint32_t lds32[];
+  const int lane = threadIdx.x;
+  const int slot = lane & 15, role = lane >> 4;
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
+  for (uint32_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+    const GroupEntry G = groups[g];
+    for (uint32_t i = lane; i < G.n_words; i += 64) lds32[i] = __builtin_bswap32(words[G.w_lo + i]);
+    const uint32_t m = G.m0 + (uint32_t)slot;
+    const bool active = m < G.m1;
+    MeshletEntry e;
+    if (active) e = tab[m];
+    const uint32_t nl = active ? e.n_local : 0u;
+    uint32_t total = 0;
+    const uint32_t pre0 = wave_excl_scan(role == 0 ? nl : 0u, total);
+    const uint32_t pre = __shfl(pre0, slot, 64);       // block offset of this lane's meshlet
+    uint32_t* blk = lds32 + G.codes_off + 4 * pre;
+    __syncthreads();
+    if (active && role == 0) {
+      const MeshParams& P = meshes[e.mesh];
+      const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
+      const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
+      split_tokens(lds32, rel, e, blk, T + 3 * (P.t_base + e.t_off));
+    }
+    __syncthreads();
+    if (active && role > 0) {
+      const MeshParams& P = meshes[e.mesh];
+      const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
+      const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
+      int32_t* codes = reinterpret_cast<int32_t*>(blk + nl * (uint32_t)role);
+      if (!split_axis(lds32, G.n_words, rel, e, P, role - 1, blk, codes) && overflow)
+        overflow[g] = 1u;
+    }
+    __syncthreads();
+    // dequantise, one meshlet at a time, coalesced xyz dwords
+    for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
+      const MeshletEntry ej = tab[G.m0 + j];
+      const MeshParams& Pj = meshes[ej.mesh];
+      const uint32_t pj = __shfl(pre0, (int)j, 64);
+      const int32_t* cb = reinterpret_cast<const int32_t*>(lds32 + G.codes_off + 4 * pj);
+      float* vo = V + 3 * (Pj.v_base + ej.v_off);
+      const uint32_t n = ej.n_local, nv3 = 3 * n;
+      for (uint32_t i = lane; i < nv3; i += 64) {
+        const uint32_t vtx = i / 3, a = i - 3 * vtx;
+        const int32_t code = cb[n * (a + 1) + vtx];
+        const float gmin = a == 0 ? Pj.g_min[0] : (a == 1 ? Pj.g_min[1] : Pj.g_min[2]);
+        const float grng = a == 0 ? Pj.g_range[0] : (a == 1 ? Pj.g_range[1] : Pj.g_range[2]);
+        const int gb = a == 0 ? Pj.g_bits[0] : (a == 1 ? Pj.g_bits[1] : Pj.g_bits[2]);
+        const float ce = a == 0 ? Pj.center[0] : (a == 1 ? Pj.center[1] : Pj.center[2]);
+        vo[i] = dequant(code, gmin, grng, gb, Pj.scale, ce);
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_scan(const uint8_t* buf, const MeshletEntry* seeds, const MeshParams* meshes,
@@ -384,8 +450,12 @@ static hipError_t launch_decode_t(const DecodeLaunch& d, hipStream_t stream) {
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream) {
   if (d.n_groups == 0) return hipSuccess;
   const uint32_t grid = d.grid ? std::min(d.grid, d.n_groups) : d.n_groups;
-  hipLaunchKernelGGL(stride_decode_group_kernel, dim3(grid), dim3(64), d.lds_bytes, stream, d.buf,
-                     d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
+  if (d.split)
+    hipLaunchKernelGGL(stride_decode_split_kernel, dim3(grid), dim3(64), d.lds_bytes, stream,
+                       d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
+  else
+    hipLaunchKernelGGL(stride_decode_group_kernel, dim3(grid), dim3(64), d.lds_bytes, stream,
+                       d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
   return hipGetLastError();
 }
 

@@ -233,6 +233,149 @@ STRIDE_DEV bool decode_lane(const uint32_t* __restrict__ st, uint32_t n_words,
 }
 
 
+// ---------------------------------------------------------------------------
+// Split decode: four lanes per meshlet.  A token lane decodes the strip
+// tokens (reuse FIFO, triangles) and records the walk -- the local ids in
+// first-seen order with their parallelogram references -- in LDS; then one
+// lane per axis decodes that axis's two residual substreams along the walk
+// and runs its prediction chain.  The serial path of a wave is the token
+// loop plus one axis chain instead of the token loop with all three axes
+// folded in.
+//
+// Walk entry i: kRootMark | v for a strip-root vertex (anchor or delta),
+// v | a << 8 | b << 16 | c << 24 for a parallelogram vertex (the pre-pass
+// rejects references (255, 255, 255), so the two forms never collide).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kRootMark = 0xFFFFFF00u;
+
+STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
+                             uint32_t* __restrict__ walk, uint32_t* __restrict__ tout) {
+  const uint32_t n_local = e.n_local, n_tris = e.n_tris;
+  const uint32_t n_strips = e.n_strips & kStripMask;
+  const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
+  uint32_t pc = rel + 48;
+  int ib = 1;
+  while ((1u << ib) < n_local + 1) ++ib;
+  Mtf fifo;
+  fifo.clear();
+  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;     // seen local ids
+  uint32_t nw = 0, q0 = 0, q1 = 0, q2 = 0;
+  uint32_t root_j = 3, steps_left = 0, tc = 0;
+  const uint32_t v_off = e.v_off;
+  const uint32_t n_tok = n_tris + 2 * n_strips;
+  for (uint32_t it = 0; it < n_tok; ++it) {
+    uint64_t W = win64(st, pc);
+    const bool start = root_j == 3 && steps_left == 0;
+    const uint32_t L = (uint32_t)(W >> 48);
+    W = start ? (W << 16) : W;
+    steps_left = start ? L - 1 : steps_left;
+    root_j = start ? 0u : root_j;
+    const bool is_root = root_j < 3;
+    const uint32_t ec = is_root ? 0u : (uint32_t)(W >> 63);
+    W = is_root ? W : (W << 1);
+    const uint32_t flag = (uint32_t)(W >> 63);
+    W <<= 1;
+    const uint32_t val = flag ? (uint32_t)(W >> (64 - ib)) : (uint32_t)(W >> 59);
+    pc += (start ? 16u : 0u) + (is_root ? 0u : 1u) + 1u + (flag ? (uint32_t)ib : 5u);
+    const bool hit = flag == 0;
+    uint32_t j = hit ? ((fifo.n - 1 - (val & 15u)) & 15u) : fifo.n;
+    const uint32_t v = hit ? fifo.rank(j) : (val & 0xFFu);
+    if (raw_in_fifo && !hit) {
+      const int pf = fifo.find(v);
+      j = pf >= 0 ? (uint32_t)pf : j;
+    }
+    fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
+    fifo.to_front(j, v);
+    // first sighting -> walk entry
+    const uint32_t wq = v >> 6;
+    const uint64_t bit = 1ull << (v & 63);
+    const uint64_t sw = wq == 0 ? s0 : (wq == 1 ? s1 : (wq == 2 ? s2 : s3));
+    const bool fresh = (sw & bit) == 0;
+    s0 |= wq == 0 ? bit : 0ull;
+    s1 |= wq == 1 ? bit : 0ull;
+    s2 |= wq == 2 ? bit : 0ull;
+    s3 |= wq == 3 ? bit : 0ull;
+    // parallelogram refs (a, b, c): ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
+    const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
+    const uint32_t entry = is_root ? (kRootMark | v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
+    if (fresh) walk[nw++ & 255u] = entry;
+    uint32_t ta, tb;
+    if (is_root) {
+      ta = q0;
+      tb = root_j == 1 ? v : q1;
+      q0 = root_j == 0 ? v : q0;
+      q1 = root_j == 1 ? v : q1;
+      q2 = root_j == 2 ? v : q2;
+      ++root_j;
+    } else {
+      ta = ec ? q0 : q1;
+      tb = q2;
+      q0 = ec ? q0 : q1;
+      q1 = q2;
+      q2 = v;
+      --steps_left;
+    }
+    if (root_j == 3 || !is_root) {
+      uint32_t* o = tout + 3 * (tc & (kMaxTris - 1));
+      o[0] = v_off + ta;
+      o[1] = v_off + tb;
+      o[2] = v_off + v;
+      ++tc;
+    }
+  }
+}
+
+STRIDE_DEV uint32_t pick3(int d, uint32_t x0, uint32_t x1, uint32_t x2) {
+  return d == 0 ? x0 : (d == 1 ? x1 : x2);
+}
+
+// Axis lane d: anchor, the delta and parallelogram substreams of axis d,
+// and the chain along the walk; codes[v] receives the code of local id v.
+// Returns false when a code leaves int32.
+STRIDE_DEV bool split_axis(const uint32_t* __restrict__ st, uint32_t n_words, uint32_t rel,
+                           const MeshletEntry& e, const MeshParams& P, int d,
+                           const uint32_t* __restrict__ walk, int32_t* __restrict__ codes) {
+  const uint32_t n_local = e.n_local;
+  const uint32_t gb0 = (uint32_t)P.g_bits[0], gb1 = (uint32_t)P.g_bits[1],
+                 gb2 = (uint32_t)P.g_bits[2];
+  const uint32_t gb = pick3(d, gb0, gb1, gb2);
+  const uint32_t r0 = rel + (e.resid_bit - e.start_bit);
+  const uint32_t ap = r0 + (d > 0 ? gb0 : 0u) + (d > 1 ? gb1 : 0u);
+  const int64_t anchor = (int64_t)(win64(st, ap) >> (64 - gb));
+  // (tag, k) of delta-d at tag byte pair d, of para-d at pair 3 + d
+  const uint32_t tb = r0 + gb0 + gb1 + gb2;
+  const uint64_t TD = win64(st, tb + 16u * (uint32_t)d);
+  const uint64_t TP = win64(st, tb + 16u * (uint32_t)(3 + d));
+  const bool deg = (uint32_t)(TD >> 56) != 1u, peg = (uint32_t)(TP >> 56) != 1u;
+  const int dk = (int)((TD >> 48) & 31u), pk = (int)((TP >> 48) & 31u);
+  uint32_t pd = rel + (pick3(d, e.sub_bit[0], e.sub_bit[1], e.sub_bit[2]) - e.start_bit);
+  uint32_t pp = rel + (pick3(d, e.sub_bit[3], e.sub_bit[4], e.sub_bit[5]) - e.start_bit);
+  const int32_t n0 = (int32_t)pick3(d, (uint32_t)P.pred_n[0], (uint32_t)P.pred_n[3], (uint32_t)P.pred_n[6]);
+  const int32_t n1 = (int32_t)pick3(d, (uint32_t)P.pred_n[1], (uint32_t)P.pred_n[4], (uint32_t)P.pred_n[7]);
+  const int32_t n2 = (int32_t)pick3(d, (uint32_t)P.pred_n[2], (uint32_t)P.pred_n[5], (uint32_t)P.pred_n[8]);
+  const int32_t K = (int32_t)pick3(d, (uint32_t)P.pred_K[0], (uint32_t)P.pred_K[1], (uint32_t)P.pred_K[2]);
+  bool ok = true;
+  int64_t prev = anchor;
+  codes[walk[0] & 0xFFu] = (int32_t)anchor;
+  ok &= fits32(anchor);
+  for (uint32_t i = 1; i < n_local; ++i) {
+    const uint32_t w = walk[i];
+    const bool root = (w & 0xFFFFFF00u) == kRootMark;
+    uint32_t pos = root ? pd : pp;
+    const int64_t r = codeword(st, n_words, pos, root ? deg : peg, root ? dk : pk, true);
+    pd = root ? pos : pd;
+    pp = root ? pp : pos;
+    const uint32_t ia = root ? 0u : (w >> 8) & 0xFFu, ib = root ? 0u : (w >> 16) & 0xFFu,
+                   ic = root ? 0u : w >> 24;
+    const int64_t pred = irlp64(codes[ia], codes[ib], codes[ic], n0, n1, n2, K);
+    const int64_t c = (root ? prev : pred) + r;
+    ok &= fits32(c);
+    codes[w & 0xFFu] = (int32_t)c;
+    prev = c;
+  }
+  return ok;
+}
+
 // Index pre-pass for one meshlet (one lane): parses the whole meshlet once,
 // derives the side-table fields the stream does not store, and validates
 // every field against the meshlet's bit range [start, end).

@@ -3,8 +3,11 @@
 // the lane decoder (decode_lane) run here on the CPU, under the sanitizers
 // when built with them, and their output is compared with the oracle.
 //
-//   lane_check <stream.dpr4> <verts.f32 out> <tris.u32 out>
+//   lane_check <stream.dpr4> <verts.f32 out> <tris.u32 out> [split]
 //
+// With "split" the split decoder runs instead: every meshlet's token lane
+// (split_tokens), then its three axis lanes (split_axis), as the kernel's
+// phases do.
 // Exit status: 0 ok, 2 stream rejected by the pre-pass, 3 code overflow
 // (the kernels route such groups to the wave-per-meshlet decoder).
 #include <cstdio>
@@ -82,20 +85,32 @@ int main(int argc, char** argv) {
     nv += s.n_local;
     nt += s.n_tris;
   }
+  const bool split = argc > 4 && std::strcmp(argv[4], "split") == 0;
   std::vector<uint64_t> codes(nv, kUnseen);
   std::vector<uint32_t> tris(3 * nt, 0xFFFFFFFFu);
+  std::vector<int32_t> codes3(3 * nv, 0);
   int rc = 0;
   for (uint32_t m = 0; m < n_m; ++m) {
     const MeshletEntry& e = tab[m];
     // one group = the whole stream staged: rel is the absolute start bit
-    if (!decode_lane(st.data(), (uint32_t)n_words, e.start_bit, e, P, codes.data() + e.v_off,
-                     tris.data() + 3ull * e.t_off))
+    if (split) {
+      std::vector<uint32_t> blk(4 * (size_t)e.n_local, 0xDEADBEEFu);
+      split_tokens(st.data(), e.start_bit, e, blk.data(), tris.data() + 3ull * e.t_off);
+      for (int d = 0; d < 3; ++d) {
+        int32_t* cd = reinterpret_cast<int32_t*>(blk.data() + (size_t)e.n_local * (d + 1));
+        if (!split_axis(st.data(), (uint32_t)n_words, e.start_bit, e, P, d, blk.data(), cd))
+          rc = 3;
+        for (uint32_t v = 0; v < e.n_local; ++v) codes3[3 * ((size_t)e.v_off + v) + d] = cd[v];
+      }
+    } else if (!decode_lane(st.data(), (uint32_t)n_words, e.start_bit, e, P,
+                            codes.data() + e.v_off, tris.data() + 3ull * e.t_off)) {
       rc = 3;
+    }
   }
   std::vector<float> V(3 * nv);
   for (uint64_t i = 0; i < nv; ++i)
     for (int a = 0; a < 3; ++a) {
-      const int32_t c = field21(codes[i], a);
+      const int32_t c = split ? codes3[3 * i + a] : field21(codes[i], a);
       const uint64_t mx = (1ull << P.g_bits[a]) - 1;
       const double vn = mx == 0 ? (double)P.g_min[a]
                                 : (double)P.g_min[a] + ((double)c / (double)mx) * (double)P.g_range[a];

@@ -77,6 +77,10 @@ TUNES = [dict(mode=0, waves=w, stage_words=s) for w in (1, 2, 4) for s in (256, 
          dict(mode=1, group_size=64, lds_bytes=65536, grid=5),
          dict(mode=1, group_size=7, lds_bytes=16384),
          dict(mode=1, group_size=1, lds_bytes=8192),
+         dict(mode=2, group_size=16, lds_bytes=40960),
+         dict(mode=2, group_size=16, lds_bytes=65536, grid=7),
+         dict(mode=2, group_size=5, lds_bytes=16384),
+         dict(mode=2, group_size=1, lds_bytes=8192),
          # too small for most meshlets: exercises the wave-per-meshlet fallback list
          dict(mode=1, group_size=32, lds_bytes=8192, stage_words=256)]
 

@@ -211,20 +211,23 @@ def lane_check(tmp_path_factory):
     return exe
 
 
-def run_lane_check(exe, data, tmp_path):
+def run_lane_check(exe, data, tmp_path, split=False):
     import subprocess
     src = tmp_path / "s.dpr4"
     src.write_bytes(data)
     fv, ft = tmp_path / "v.f32", tmp_path / "t.u32"
-    r = subprocess.run([str(exe), str(src), str(fv), str(ft)], capture_output=True, text=True)
+    r = subprocess.run([str(exe), str(src), str(fv), str(ft)] + (["split"] if split else []),
+                       capture_output=True, text=True)
     return r.returncode, np.fromfile(fv, np.float32).reshape(-1, 3) if r.returncode in (0, 3) else None, \
         np.fromfile(ft, np.uint32).reshape(-1, 3) if r.returncode in (0, 3) else None
 
 
-def test_lane_decoder_matches_oracle(lane_check, coracle, tmp_path):
-    """The gfx950 kernels' pre-pass and per-lane decoder, compiled for the
-    host, reproduce the oracle bit-for-bit on every golden stream, the
-    hand-built streams and freshly encoded small meshes."""
+@pytest.mark.parametrize("split", [False, True], ids=["lane", "split"])
+def test_lane_decoder_matches_oracle(lane_check, coracle, tmp_path, split):
+    """The gfx950 kernels' pre-pass and per-lane decoders (one lane per
+    meshlet, and the split token/axis lanes), compiled for the host,
+    reproduce the oracle bit-for-bit on every golden stream, the hand-built
+    streams and freshly encoded small meshes."""
     from tests.test_oracle import hand_stream_fifo, hand_stream_small
     streams = [np.load(p)["data"].tobytes() for p in sorted((ROOT / "tests" / "golden").glob("*_*.npz"))
                if "data" in np.load(p)]
@@ -233,7 +236,7 @@ def test_lane_decoder_matches_oracle(lane_check, coracle, tmp_path):
         streams.append(encode_with_oracle(V, T, pred=([[33, 32, -33], [31, 34, -33], [32, 32, -32]],
                                                       [5, 5, 5]))[1])
     for data in streams:
-        rc, Vl, Tl = run_lane_check(lane_check, data, tmp_path)
+        rc, Vl, Tl = run_lane_check(lane_check, data, tmp_path, split)
         assert rc == 0
         Vo, To = coracle.decode(data)
         assert np.array_equal(Tl.astype(np.int64), To)
