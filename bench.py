@@ -274,6 +274,8 @@ def main():
     ap.add_argument("--learned-heldout", type=int, default=12,
                     help="held-out meshes in the learned-encode comparison (0: skip)")
     ap.add_argument("--quick", action="store_true", help="headline workload only")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the decoders' default launch shape (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
@@ -309,7 +311,8 @@ def main():
         V, T = workloads.icosphere_l8()
     else:
         sys.exit(f"unknown workload {name}")
-    dec, data, head, head_prep = single_mesh(name, V, T, bits, a.steps, a.warmup)
+    dec, data, head, head_prep = single_mesh(name, V, T, bits, a.steps, a.warmup,
+                                             tune=not a.no_autotune)
     details = {}
     cpu = None
     if not a.no_cpu:

@@ -220,6 +220,46 @@ struct Mtf {
   }
 };
 
+// The newest-first reuse FIFO as two 64-bit halves with cheap masks: the
+// move-to-front of rank j keeps the bytes above j and shifts bytes 0..j up
+// by one, new = (old & ~M) | ((old << 8) & M) | v with M = bytes [0, j].
+struct MtfLite {
+  uint64_t lo, hi;
+  uint32_t n;
+  STRIDE_DEV void clear() { lo = 0; hi = 0; n = 0; }
+  STRIDE_DEV uint32_t rank(uint32_t j) const {
+    const uint64_t w = j < 8 ? lo : hi;
+    return (uint32_t)(w >> (8 * (j & 7))) & 0xFFu;
+  }
+  STRIDE_DEV void to_front(uint32_t j, uint32_t v) {
+    // bytes [0, j] of the low half: all ones >> (56 - 8 min(j, 7))
+    const uint32_t jl = j < 7 ? j : 7u;
+    const uint64_t ml = ~0ull >> (56u - 8u * jl);
+    // bytes [8, j] of the high half (none when j < 8)
+    const uint32_t jh = j < 8 ? 0u : (j - 8 < 7 ? j - 8 : 7u);
+    const uint64_t mh = j < 8 ? 0ull : (~0ull >> (56u - 8u * jh));
+    const uint64_t sl = (lo << 8) | (uint64_t)(v & 0xFFu);
+    const uint64_t sh = (hi << 8) | (lo >> 56);
+    lo = (lo & ~ml) | (sl & ml);
+    hi = (hi & ~mh) | (sh & mh);
+  }
+  // rank of v among the n entries, or -1
+  STRIDE_DEV int find(uint32_t v) const {
+    const uint64_t ones = 0x0101010101010101ull, low7 = 0x7F7F7F7F7F7F7F7Full;
+    const uint64_t pat = (uint64_t)(v & 0xFFu) * ones;
+    const uint64_t t0 = lo ^ pat, t1 = hi ^ pat;
+    uint64_t z0 = ~(((t0 & low7) + low7) | t0 | low7);
+    uint64_t z1 = ~(((t1 & low7) + low7) | t1 | low7);
+    const uint64_t v0 = n >= 8 ? ~0ull : (n ? (~0ull >> (64 - 8 * n)) : 0ull);
+    const uint64_t v1 = n <= 8 ? 0ull : (n >= 16 ? ~0ull : (~0ull >> (64 - 8 * (n - 8))));
+    z0 &= v0;
+    z1 &= v1;
+    if (z0) return __builtin_ctzll(z0) >> 3;
+    if (z1) return 8 + (__builtin_ctzll(z1) >> 3);
+    return -1;
+  }
+};
+
 struct Seen256 {
   uint64_t s[4];
   STRIDE_DEV void clear() { s[0] = s[1] = s[2] = s[3] = 0; }

@@ -387,12 +387,14 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
     const uint32_t pre0 = wave_excl_scan(role == 0 ? nl : 0u, total);
     const uint32_t pre = __shfl(pre0, slot, 64);       // block offset of this lane's meshlet
     uint32_t* blk = lds32 + G.codes_off + 4 * pre;
+    // zero the walk/code blocks: the x-code block is the token lanes' seen set
+    for (uint32_t i = lane; i < 4 * total; i += 64) lds32[G.codes_off + i] = 0u;
     __syncthreads();
     if (active && role == 0) {
       const MeshParams& P = meshes[e.mesh];
       const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
       const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
-      split_tokens(lds32, rel, e, blk, T + 3 * (P.t_base + e.t_off));
+      split_tokens(lds32, rel, e, blk, blk + nl, T + 3 * (P.t_base + e.t_off));
     }
     __syncthreads();
     if (active && role > 0) {
@@ -404,22 +406,24 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
         overflow[g] = 1u;
     }
     __syncthreads();
-    // dequantise, one meshlet at a time, coalesced xyz dwords
+    // dequantise, one meshlet at a time, one vertex (x, y, z) per lane
     for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
       const MeshletEntry ej = tab[G.m0 + j];
       const MeshParams& Pj = meshes[ej.mesh];
       const uint32_t pj = __shfl(pre0, (int)j, 64);
       const int32_t* cb = reinterpret_cast<const int32_t*>(lds32 + G.codes_off + 4 * pj);
       float* vo = V + 3 * (Pj.v_base + ej.v_off);
-      const uint32_t n = ej.n_local, nv3 = 3 * n;
-      for (uint32_t i = lane; i < nv3; i += 64) {
-        const uint32_t vtx = i / 3, a = i - 3 * vtx;
-        const int32_t code = cb[n * (a + 1) + vtx];
-        const float gmin = a == 0 ? Pj.g_min[0] : (a == 1 ? Pj.g_min[1] : Pj.g_min[2]);
-        const float grng = a == 0 ? Pj.g_range[0] : (a == 1 ? Pj.g_range[1] : Pj.g_range[2]);
-        const int gb = a == 0 ? Pj.g_bits[0] : (a == 1 ? Pj.g_bits[1] : Pj.g_bits[2]);
-        const float ce = a == 0 ? Pj.center[0] : (a == 1 ? Pj.center[1] : Pj.center[2]);
-        vo[i] = dequant(code, gmin, grng, gb, Pj.scale, ce);
+      const uint32_t n = ej.n_local;
+      for (uint32_t v = lane; v < n; v += 64) {
+        const float x = dequant(cb[n + v], Pj.g_min[0], Pj.g_range[0], Pj.g_bits[0], Pj.scale,
+                                Pj.center[0]);
+        const float y = dequant(cb[2 * n + v], Pj.g_min[1], Pj.g_range[1], Pj.g_bits[1], Pj.scale,
+                                Pj.center[1]);
+        const float z = dequant(cb[3 * n + v], Pj.g_min[2], Pj.g_range[2], Pj.g_bits[2], Pj.scale,
+                                Pj.center[2]);
+        vo[3 * v] = x;
+        vo[3 * v + 1] = y;
+        vo[3 * v + 2] = z;
       }
     }
     __syncthreads();

@@ -248,37 +248,48 @@ STRIDE_DEV bool decode_lane(const uint32_t* __restrict__ st, uint32_t n_words,
 // ---------------------------------------------------------------------------
 constexpr uint32_t kRootMark = 0xFFFFFF00u;
 
+// 32 bits of the staged stream starting at bit p.
+STRIDE_DEV uint32_t win32(const uint32_t* __restrict__ st, uint32_t p) {
+  const uint32_t w = p >> 5, sh = p & 31u;
+  return (st[w] << sh) | ((st[w + 1] >> 1) >> (31u - sh));
+}
+
+// Token lane.  `seen` is n_local zeroed words of scratch (the x-axis code
+// block, which the axis lanes overwrite afterwards).
 STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
-                             uint32_t* __restrict__ walk, uint32_t* __restrict__ tout) {
+                             uint32_t* __restrict__ walk, uint32_t* __restrict__ seen,
+                             uint32_t* __restrict__ tout) {
   const uint32_t n_local = e.n_local, n_tris = e.n_tris;
   const uint32_t n_strips = e.n_strips & kStripMask;
   const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
   uint32_t pc = rel + 48;
-  int ib = 1;
+  uint32_t ib = 1;
   while ((1u << ib) < n_local + 1) ++ib;
-  Mtf fifo;
+  MtfLite fifo;
   fifo.clear();
-  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;     // seen local ids
   uint32_t nw = 0, q0 = 0, q1 = 0, q2 = 0;
-  uint32_t root_j = 3, steps_left = 0, tc = 0;
+  uint32_t root_j = 3, steps_left = 0;
   const uint32_t v_off = e.v_off;
   const uint32_t n_tok = n_tris + 2 * n_strips;
+  uint32_t* o = tout;
   for (uint32_t it = 0; it < n_tok; ++it) {
-    uint64_t W = win64(st, pc);
+    // strip length (16 bits) + edge bit + flag + id (<= 9 bits): one 32-bit window
+    uint32_t W = win32(st, pc);
     const bool start = root_j == 3 && steps_left == 0;
-    const uint32_t L = (uint32_t)(W >> 48);
+    steps_left = start ? (W >> 16) - 1 : steps_left;
     W = start ? (W << 16) : W;
-    steps_left = start ? L - 1 : steps_left;
     root_j = start ? 0u : root_j;
     const bool is_root = root_j < 3;
-    const uint32_t ec = is_root ? 0u : (uint32_t)(W >> 63);
+    const uint32_t ec = is_root ? 0u : (W >> 31);
     W = is_root ? W : (W << 1);
-    const uint32_t flag = (uint32_t)(W >> 63);
-    W <<= 1;
-    const uint32_t val = flag ? (uint32_t)(W >> (64 - ib)) : (uint32_t)(W >> 59);
-    pc += (start ? 16u : 0u) + (is_root ? 0u : 1u) + 1u + (flag ? (uint32_t)ib : 5u);
+    const uint32_t flag = W >> 31;
+    const uint32_t val = flag ? ((W << 1) >> (32 - ib)) : ((W >> 26) & 31u);
+    pc += (start ? 16u : 0u) + (is_root ? 1u : 2u) + (flag ? ib : 5u);
+    // reuse FIFO: a hit names FIFO index val (oldest first), a raw id is
+    // appended (the oldest entry leaving a full FIFO); either way the value
+    // becomes the most recent entry
     const bool hit = flag == 0;
-    uint32_t j = hit ? ((fifo.n - 1 - (val & 15u)) & 15u) : fifo.n;
+    uint32_t j = hit ? ((fifo.n - 1 - val) & 15u) : fifo.n;
     const uint32_t v = hit ? fifo.rank(j) : (val & 0xFFu);
     if (raw_in_fifo && !hit) {
       const int pf = fifo.find(v);
@@ -286,42 +297,29 @@ STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, cons
     }
     fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
     fifo.to_front(j, v);
-    // first sighting -> walk entry
-    const uint32_t wq = v >> 6;
-    const uint64_t bit = 1ull << (v & 63);
-    const uint64_t sw = wq == 0 ? s0 : (wq == 1 ? s1 : (wq == 2 ? s2 : s3));
-    const bool fresh = (sw & bit) == 0;
-    s0 |= wq == 0 ? bit : 0ull;
-    s1 |= wq == 1 ? bit : 0ull;
-    s2 |= wq == 2 ? bit : 0ull;
-    s3 |= wq == 3 ? bit : 0ull;
-    // parallelogram refs (a, b, c): ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
+    // first sighting -> walk entry; parallelogram refs (a, b, c):
+    // ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
+    const bool fresh = seen[v] == 0u;
+    seen[v] = 1u;
     const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
     const uint32_t entry = is_root ? (kRootMark | v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
-    if (fresh) walk[nw++ & 255u] = entry;
-    uint32_t ta, tb;
-    if (is_root) {
-      ta = q0;
-      tb = root_j == 1 ? v : q1;
-      q0 = root_j == 0 ? v : q0;
-      q1 = root_j == 1 ? v : q1;
-      q2 = root_j == 2 ? v : q2;
-      ++root_j;
-    } else {
-      ta = ec ? q0 : q1;
-      tb = q2;
-      q0 = ec ? q0 : q1;
-      q1 = q2;
-      q2 = v;
-      --steps_left;
-    }
-    if (root_j == 3 || !is_root) {
-      uint32_t* o = tout + 3 * (tc & (kMaxTris - 1));
+    if (fresh) walk[nw & 255u] = entry;
+    nw += fresh ? 1u : 0u;
+    // triangle (a, b, v) and the next previous triangle
+    const bool emit = !is_root || root_j == 2;
+    const uint32_t ta = is_root ? q0 : ra;
+    const uint32_t tb = is_root ? q1 : q2;
+    if (emit) {
       o[0] = v_off + ta;
       o[1] = v_off + tb;
       o[2] = v_off + v;
-      ++tc;
+      o += 3;
     }
+    q0 = is_root ? (root_j == 0 ? v : q0) : ra;
+    q1 = is_root ? (root_j == 1 ? v : q1) : q2;
+    q2 = is_root ? (root_j == 2 ? v : q2) : v;
+    root_j += is_root ? 1u : 0u;
+    steps_left -= is_root ? 0u : 1u;
   }
 }
 

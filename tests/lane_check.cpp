@@ -94,8 +94,9 @@ int main(int argc, char** argv) {
     const MeshletEntry& e = tab[m];
     // one group = the whole stream staged: rel is the absolute start bit
     if (split) {
-      std::vector<uint32_t> blk(4 * (size_t)e.n_local, 0xDEADBEEFu);
-      split_tokens(st.data(), e.start_bit, e, blk.data(), tris.data() + 3ull * e.t_off);
+      std::vector<uint32_t> blk(4 * (size_t)e.n_local, 0u);
+      split_tokens(st.data(), e.start_bit, e, blk.data(), blk.data() + e.n_local,
+                   tris.data() + 3ull * e.t_off);
       for (int d = 0; d < 3; ++d) {
         int32_t* cd = reinterpret_cast<int32_t*>(blk.data() + (size_t)e.n_local * (d + 1));
         if (!split_axis(st.data(), (uint32_t)n_words, e.start_bit, e, P, d, blk.data(), cd))
