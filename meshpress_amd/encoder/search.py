@@ -153,7 +153,8 @@ def subsample_sel(n_m, n_keep=SUBSAMPLE_MESHLETS):
 
 def as_columns(A, B, C, V):
     """(n, 3) sample arrays -> int32 columns (4, 3, n)."""
-    return np.ascontiguousarray(np.stack([np.asarray(x).T for x in (A, B, C, V)]), dtype=np.int32)
+    return np.ascontiguousarray(np.stack([np.asarray(x).reshape(-1, 3).T for x in (A, B, C, V)]),
+                                dtype=np.int32)
 
 
 def subsample(A, B, C, V, off, n_keep=SUBSAMPLE_MESHLETS):

@@ -262,3 +262,12 @@ def test_lane_prepass_rejects_corruption(lane_check, coracle, tmp_path):
             raise AssertionError(f"trial {trial}: pre-pass accepted a stream the oracle rejects")
         assert np.array_equal(Tl.astype(np.int64), To)
         assert np.array_equal(Vl.view(np.uint32), Vo.view(np.uint32))
+
+
+def test_fit_predictor_without_samples_is_canonical():
+    """A mesh without parallelogram vertices keeps the canonical predictor
+    (no GPU work is needed to decide that)."""
+    from meshpress_amd.encoder import search
+    z = np.zeros(0, np.int32)
+    n, K = search.fit_predictor(z, z, z, z, np.zeros(2, np.int64))
+    assert np.array_equal(n, search.CANON_N) and np.array_equal(K, search.CANON_K)
