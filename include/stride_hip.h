@@ -84,10 +84,16 @@ int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes,
 int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
 
 /* Launch configuration (autotuned).
- *   mode 1 (default): groups of up to group_size consecutive meshlets per
+ *   mode 2 (default; group_size 16, lds_bytes 40960): split groups of up to
+ *     16 consecutive meshlets per wave, four lanes per meshlet (one decodes
+ *     the strip tokens into a walk list in LDS, then one per axis decodes
+ *     that axis's residual substreams and prediction chain); lds_bytes of
+ *     LDS per group (staged bits + 16 bytes per vertex).
+ *   mode 1: groups of up to group_size (<= 64) consecutive meshlets per
  *     wave, one lane per meshlet, lds_bytes of LDS per group (staged bits +
- *     packed codes); meshlets that do not fit, or whose codes leave the
- *     packed range, are decoded by mode 0.
+ *     packed codes).
+ *   In modes 1 and 2, meshlets that do not fit a group, or whose codes leave
+ *   the packed range, are decoded by mode 0.
  *   mode 0: one wave per meshlet: waves per workgroup {1,2,4}, LDS stage
  *     words per meshlet {256,512,1024}, use_queue = persistent grid pulling
  *     meshlets from a device atomic counter.

@@ -10,14 +10,14 @@
 namespace stride {
 
 struct DecodeConfig {
-  int mode = 1;           // 0 = one wave per meshlet, 1 = groups (one lane per meshlet),
+  int mode = 2;           // 0 = one wave per meshlet, 1 = groups (one lane per meshlet),
                           // 2 = split groups (token lane + three axis lanes per meshlet)
   int waves = 1;          // mode 0: waves per workgroup (1, 2, 4); one meshlet per wave
   int stage_words = 512;  // mode 0: LDS words staged per meshlet (256, 512, 1024)
   int grid = 0;           // 0 = one workgroup per unit of work, else persistent grid
   int use_queue = 0;      // mode 0: persistent grid pulls meshlets from an atomic counter
-  int group_size = 64;    // mode 1: meshlets per group (lanes), 1..64; mode 2: 1..16
-  int lds_bytes = 49152;  // mode 1: LDS per group (stage words + packed codes)
+  int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; mode 2: 1..16
+  int lds_bytes = 40960;  // modes 1, 2: LDS per group (stage words + codes / walk blocks)
 };
 
 struct DecodeLaunch {
