@@ -17,6 +17,7 @@
 #include "stride_device.h"
 #include "stride_err.h"
 #include "stride_host.h"
+#include "stride_lane.h"
 
 using namespace stride;
 
@@ -315,6 +316,7 @@ struct DecodeSet {
       if (cfg.mode == 1 || cfg.mode == 2) {
         const uint64_t budget = (uint64_t)cfg.lds_bytes;
         const bool split = cfg.mode == 2;
+        const bool nar = narrow();
         const uint32_t gmax = split ? std::min(cfg.group_size, 16) : cfg.group_size;
         for (int i = 0; i < n_meshes; ++i) {
           g_base[i] = (uint32_t)groups.size();
@@ -336,19 +338,20 @@ struct DecodeSet {
             const MeshletEntry& e = tab[j];
             const uint64_t lo = (bb + e.start_bit) >> 5;
             const uint64_t hi = ((bb + e.end_bit + 31) >> 5) + 2;
-            auto need = [split](uint64_t words, uint64_t nloc) {
-              return split ? 4 * ((words + 3) & ~3ull) + 16 * nloc
-                           : 8 * ((words + 1) / 2) + 8 * nloc;
+            // bytes of LDS: staged words + per-meshlet code storage ("codes"
+            // counts vertices in mode 1, block words in mode 2)
+            const uint64_t unit = split ? split_block_words(e.n_local, nar) : e.n_local;
+            auto need = [split](uint64_t words, uint64_t cu) {
+              return split ? 4 * ((words + 3) & ~3ull) + 4 * cu : 8 * ((words + 1) / 2) + 8 * cu;
             };
-            if (open && (j - cur.m0 < gmax) &&
-                need(hi - cur.w_lo, codes + e.n_local) <= budget) {
+            if (open && (j - cur.m0 < gmax) && need(hi - cur.w_lo, codes + unit) <= budget) {
               cur.m1 = j + 1;
               cur.n_words = (uint32_t)(hi - cur.w_lo);
-              codes += e.n_local;
+              codes += unit;
               continue;
             }
             close();
-            if (need(hi - lo, e.n_local) > budget) {   // too large for any group
+            if (need(hi - lo, unit) > budget) {   // too large for any group
               legacy[j] = 1;
               continue;
             }
@@ -357,7 +360,7 @@ struct DecodeSet {
             cur.m1 = j + 1;
             cur.w_lo = (uint32_t)lo;
             cur.n_words = (uint32_t)(hi - lo);
-            codes = e.n_local;
+            codes = unit;
             open = true;
           }
           close();
@@ -437,7 +440,16 @@ struct DecodeSet {
     g.lds_bytes = (uint32_t)cfg.lds_bytes;
     g.grid = (uint32_t)cfg.grid;
     g.split = cfg.mode == 2;
+    g.narrow = narrow();
     return g;
+  }
+
+  // every mesh has at most 16 bits per axis: the split kernel keeps uint16 codes
+  bool narrow() const {
+    for (const MeshParams& p : mp)
+      for (int d = 0; d < 3; ++d)
+        if (p.g_bits[d] > 16) return false;
+    return true;
   }
 
   DecodeLaunch legacy_args(float* V, uint32_t* T, uint32_t i0, uint32_t i1) const {

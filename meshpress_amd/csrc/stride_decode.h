@@ -46,6 +46,7 @@ struct GroupLaunch {
   uint32_t lds_bytes;
   uint32_t grid;          // 0 = one workgroup per group
   int split;              // mode 2 kernel
+  int narrow;             // mode 2: uint16 codes (every mesh has g_bits <= 16)
 };
 
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream);

@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "stride_bits.h"
 #include "stride_lane.h"
@@ -363,14 +364,17 @@ __global__ __launch_bounds__(64) void stride_decode_group_kernel(
 // ---------------------------------------------------------------------------
 // Split decode: one wave per group of up to 16 consecutive meshlets; lane
 // l works on meshlet l % 16 with role l / 16 (0: tokens, 1..3: axis x, y,
-// z).  LDS: the group's staged words, then per meshlet a block of
-// 4 n_local words -- walk, codes x, codes y, codes z.
+// z).  LDS: the group's staged words, then per meshlet a block with the
+// walk (n_local words) and the x, y, z code arrays (int32, or uint16 when
+// NARROW: every mesh of the set has at most 16 bits per axis).
 // ---------------------------------------------------------------------------
+template <bool NARROW>
 __global__ __launch_bounds__(64) void stride_decode_split_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
     const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
     uint32_t n_groups, float* __restrict__ V, uint32_t* __restrict__ T,
     uint32_t* __restrict__ overflow) {
+  using CodeT = typename std::conditional<NARROW, uint16_t, int32_t>::type;
   extern __shared__ uint32_t lds32[];
   const int lane = threadIdx.x;
   const int slot = lane & 15, role = lane >> 4;
@@ -384,24 +388,26 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
     if (active) e = tab[m];
     const uint32_t nl = active ? e.n_local : 0u;
     uint32_t total = 0;
-    const uint32_t pre0 = wave_excl_scan(role == 0 ? nl : 0u, total);
+    const uint32_t pre0 = wave_excl_scan(role == 0 ? split_block_words(nl, NARROW) : 0u, total);
     const uint32_t pre = __shfl(pre0, slot, 64);       // block offset of this lane's meshlet
-    uint32_t* blk = lds32 + G.codes_off + 4 * pre;
-    // zero the walk/code blocks: the x-code block is the token lanes' seen set
-    for (uint32_t i = lane; i < 4 * total; i += 64) lds32[G.codes_off + i] = 0u;
+    uint32_t* blk = lds32 + G.codes_off + pre;
+    CodeT* codes0 = reinterpret_cast<CodeT*>(blk + nl);
+    const uint32_t cstride = split_code_stride(nl, NARROW);
+    // zero the blocks: the x-code array is the token lanes' seen set
+    for (uint32_t i = lane; i < total; i += 64) lds32[G.codes_off + i] = 0u;
     __syncthreads();
     if (active && role == 0) {
       const MeshParams& P = meshes[e.mesh];
       const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
       const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
-      split_tokens(lds32, rel, e, blk, blk + nl, T + 3 * (P.t_base + e.t_off));
+      split_tokens(lds32, rel, e, blk, codes0, T + 3 * (P.t_base + e.t_off));
     }
     __syncthreads();
     if (active && role > 0) {
       const MeshParams& P = meshes[e.mesh];
       const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
       const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
-      int32_t* codes = reinterpret_cast<int32_t*>(blk + nl * (uint32_t)role);
+      CodeT* codes = codes0 + cstride * (uint32_t)(role - 1);
       if (!split_axis(lds32, G.n_words, rel, e, P, role - 1, blk, codes) && overflow)
         overflow[g] = 1u;
     }
@@ -411,16 +417,16 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
       const MeshletEntry ej = tab[G.m0 + j];
       const MeshParams& Pj = meshes[ej.mesh];
       const uint32_t pj = __shfl(pre0, (int)j, 64);
-      const int32_t* cb = reinterpret_cast<const int32_t*>(lds32 + G.codes_off + 4 * pj);
+      const uint32_t n = ej.n_local, cs = split_code_stride(n, NARROW);
+      const CodeT* cb = reinterpret_cast<const CodeT*>(lds32 + G.codes_off + pj + n);
       float* vo = V + 3 * (Pj.v_base + ej.v_off);
-      const uint32_t n = ej.n_local;
       for (uint32_t v = lane; v < n; v += 64) {
-        const float x = dequant(cb[n + v], Pj.g_min[0], Pj.g_range[0], Pj.g_bits[0], Pj.scale,
+        const float x = dequant((int32_t)cb[v], Pj.g_min[0], Pj.g_range[0], Pj.g_bits[0], Pj.scale,
                                 Pj.center[0]);
-        const float y = dequant(cb[2 * n + v], Pj.g_min[1], Pj.g_range[1], Pj.g_bits[1], Pj.scale,
-                                Pj.center[1]);
-        const float z = dequant(cb[3 * n + v], Pj.g_min[2], Pj.g_range[2], Pj.g_bits[2], Pj.scale,
-                                Pj.center[2]);
+        const float y = dequant((int32_t)cb[cs + v], Pj.g_min[1], Pj.g_range[1], Pj.g_bits[1],
+                                Pj.scale, Pj.center[1]);
+        const float z = dequant((int32_t)cb[2 * cs + v], Pj.g_min[2], Pj.g_range[2], Pj.g_bits[2],
+                                Pj.scale, Pj.center[2]);
         vo[3 * v] = x;
         vo[3 * v + 1] = y;
         vo[3 * v + 2] = z;
@@ -454,9 +460,12 @@ static hipError_t launch_decode_t(const DecodeLaunch& d, hipStream_t stream) {
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream) {
   if (d.n_groups == 0) return hipSuccess;
   const uint32_t grid = d.grid ? std::min(d.grid, d.n_groups) : d.n_groups;
-  if (d.split)
-    hipLaunchKernelGGL(stride_decode_split_kernel, dim3(grid), dim3(64), d.lds_bytes, stream,
+  if (d.split && d.narrow)
+    hipLaunchKernelGGL(stride_decode_split_kernel<true>, dim3(grid), dim3(64), d.lds_bytes, stream,
                        d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
+  else if (d.split)
+    hipLaunchKernelGGL(stride_decode_split_kernel<false>, dim3(grid), dim3(64), d.lds_bytes,
+                       stream, d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
   else
     hipLaunchKernelGGL(stride_decode_group_kernel, dim3(grid), dim3(64), d.lds_bytes, stream,
                        d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);

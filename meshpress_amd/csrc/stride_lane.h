@@ -242,11 +242,24 @@ STRIDE_DEV bool decode_lane(const uint32_t* __restrict__ st, uint32_t n_words,
 // loop plus one axis chain instead of the token loop with all three axes
 // folded in.
 //
+// Per meshlet the LDS block holds the walk (n_local words) and the three
+// code arrays: int32 (3 n_local words) or uint16 (split_code_words(n)
+// words, each axis array padded to an even count).
+//
 // Walk entry i: kRootMark | v for a strip-root vertex (anchor or delta),
 // v | a << 8 | b << 16 | c << 24 for a parallelogram vertex (the pre-pass
 // rejects references (255, 255, 255), so the two forms never collide).
 // ---------------------------------------------------------------------------
 constexpr uint32_t kRootMark = 0xFFFFFF00u;
+
+// 32-bit words of one meshlet's split block (walk + three code arrays).
+STRIDE_DEV uint32_t split_block_words(uint32_t n_local, bool narrow) {
+  return narrow ? n_local + 3 * ((n_local + 1) / 2) : 4 * n_local;
+}
+// elements between the starts of two consecutive axis code arrays
+STRIDE_DEV uint32_t split_code_stride(uint32_t n_local, bool narrow) {
+  return narrow ? 2 * ((n_local + 1) / 2) : n_local;
+}
 
 // 32 bits of the staged stream starting at bit p.
 STRIDE_DEV uint32_t win32(const uint32_t* __restrict__ st, uint32_t p) {
@@ -254,10 +267,11 @@ STRIDE_DEV uint32_t win32(const uint32_t* __restrict__ st, uint32_t p) {
   return (st[w] << sh) | ((st[w + 1] >> 1) >> (31u - sh));
 }
 
-// Token lane.  `seen` is n_local zeroed words of scratch (the x-axis code
-// block, which the axis lanes overwrite afterwards).
+// Token lane.  `seen` is n_local zeroed entries of scratch (the x-axis code
+// array, which the axis lanes overwrite afterwards).
+template <class SeenT>
 STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
-                             uint32_t* __restrict__ walk, uint32_t* __restrict__ seen,
+                             uint32_t* __restrict__ walk, SeenT* __restrict__ seen,
                              uint32_t* __restrict__ tout) {
   const uint32_t n_local = e.n_local, n_tris = e.n_tris;
   const uint32_t n_strips = e.n_strips & kStripMask;
@@ -299,8 +313,8 @@ STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, cons
     fifo.to_front(j, v);
     // first sighting -> walk entry; parallelogram refs (a, b, c):
     // ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
-    const bool fresh = seen[v] == 0u;
-    seen[v] = 1u;
+    const bool fresh = seen[v] == 0;
+    seen[v] = 1;
     const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
     const uint32_t entry = is_root ? (kRootMark | v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
     if (fresh) walk[nw & 255u] = entry;
@@ -327,12 +341,21 @@ STRIDE_DEV uint32_t pick3(int d, uint32_t x0, uint32_t x1, uint32_t x2) {
   return d == 0 ? x0 : (d == 1 ? x1 : x2);
 }
 
+// Code storage of the split decoder: int32, or uint16 when every axis of
+// the mesh has at most 16 bits (codes of a well-formed stream lie in
+// [0, 2^g_bits - 1]).
+template <class CodeT>
+STRIDE_DEV bool code_fits(int64_t c) {
+  return sizeof(CodeT) == 2 ? (c >= 0 && c <= 0xFFFF) : fits32(c);
+}
+
 // Axis lane d: anchor, the delta and parallelogram substreams of axis d,
 // and the chain along the walk; codes[v] receives the code of local id v.
-// Returns false when a code leaves int32.
+// Returns false when a code leaves the range of CodeT.
+template <class CodeT>
 STRIDE_DEV bool split_axis(const uint32_t* __restrict__ st, uint32_t n_words, uint32_t rel,
                            const MeshletEntry& e, const MeshParams& P, int d,
-                           const uint32_t* __restrict__ walk, int32_t* __restrict__ codes) {
+                           const uint32_t* __restrict__ walk, CodeT* __restrict__ codes) {
   const uint32_t n_local = e.n_local;
   const uint32_t gb0 = (uint32_t)P.g_bits[0], gb1 = (uint32_t)P.g_bits[1],
                  gb2 = (uint32_t)P.g_bits[2];
@@ -354,8 +377,8 @@ STRIDE_DEV bool split_axis(const uint32_t* __restrict__ st, uint32_t n_words, ui
   const int32_t K = (int32_t)pick3(d, (uint32_t)P.pred_K[0], (uint32_t)P.pred_K[1], (uint32_t)P.pred_K[2]);
   bool ok = true;
   int64_t prev = anchor;
-  codes[walk[0] & 0xFFu] = (int32_t)anchor;
-  ok &= fits32(anchor);
+  codes[walk[0] & 0xFFu] = (CodeT)anchor;
+  ok &= code_fits<CodeT>(anchor);
   for (uint32_t i = 1; i < n_local; ++i) {
     const uint32_t w = walk[i];
     const bool root = (w & 0xFFFFFF00u) == kRootMark;
@@ -365,10 +388,11 @@ STRIDE_DEV bool split_axis(const uint32_t* __restrict__ st, uint32_t n_words, ui
     pp = root ? pp : pos;
     const uint32_t ia = root ? 0u : (w >> 8) & 0xFFu, ib = root ? 0u : (w >> 16) & 0xFFu,
                    ic = root ? 0u : w >> 24;
-    const int64_t pred = irlp64(codes[ia], codes[ib], codes[ic], n0, n1, n2, K);
+    const int64_t pred = irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1,
+                                n2, K);
     const int64_t c = (root ? prev : pred) + r;
-    ok &= fits32(c);
-    codes[w & 0xFFu] = (int32_t)c;
+    ok &= code_fits<CodeT>(c);
+    codes[w & 0xFFu] = (CodeT)c;
     prev = c;
   }
   return ok;

@@ -85,7 +85,8 @@ int main(int argc, char** argv) {
     nv += s.n_local;
     nt += s.n_tris;
   }
-  const bool split = argc > 4 && std::strcmp(argv[4], "split") == 0;
+  const bool split = argc > 4 && std::strncmp(argv[4], "split", 5) == 0;
+  const bool narrow = argc > 4 && std::strcmp(argv[4], "split16") == 0;
   std::vector<uint64_t> codes(nv, kUnseen);
   std::vector<uint32_t> tris(3 * nt, 0xFFFFFFFFu);
   std::vector<int32_t> codes3(3 * nv, 0);
@@ -93,12 +94,23 @@ int main(int argc, char** argv) {
   for (uint32_t m = 0; m < n_m; ++m) {
     const MeshletEntry& e = tab[m];
     // one group = the whole stream staged: rel is the absolute start bit
-    if (split) {
-      std::vector<uint32_t> blk(4 * (size_t)e.n_local, 0u);
-      split_tokens(st.data(), e.start_bit, e, blk.data(), blk.data() + e.n_local,
-                   tris.data() + 3ull * e.t_off);
+    if (split && narrow) {
+      std::vector<uint32_t> blk(split_block_words(e.n_local, true), 0u);
+      uint16_t* c16 = reinterpret_cast<uint16_t*>(blk.data() + e.n_local);
+      const uint32_t cs = split_code_stride(e.n_local, true);
+      split_tokens(st.data(), e.start_bit, e, blk.data(), c16, tris.data() + 3ull * e.t_off);
       for (int d = 0; d < 3; ++d) {
-        int32_t* cd = reinterpret_cast<int32_t*>(blk.data() + (size_t)e.n_local * (d + 1));
+        if (!split_axis(st.data(), (uint32_t)n_words, e.start_bit, e, P, d, blk.data(), c16 + cs * d))
+          rc = 3;
+        for (uint32_t v = 0; v < e.n_local; ++v)
+          codes3[3 * ((size_t)e.v_off + v) + d] = c16[cs * d + v];
+      }
+    } else if (split) {
+      std::vector<uint32_t> blk(split_block_words(e.n_local, false), 0u);
+      int32_t* c32 = reinterpret_cast<int32_t*>(blk.data() + e.n_local);
+      split_tokens(st.data(), e.start_bit, e, blk.data(), c32, tris.data() + 3ull * e.t_off);
+      for (int d = 0; d < 3; ++d) {
+        int32_t* cd = c32 + (size_t)e.n_local * d;
         if (!split_axis(st.data(), (uint32_t)n_words, e.start_bit, e, P, d, blk.data(), cd))
           rc = 3;
         for (uint32_t v = 0; v < e.n_local; ++v) codes3[3 * ((size_t)e.v_off + v) + d] = cd[v];

@@ -208,3 +208,22 @@ def test_icosphere_l8_full_size(coracle, bits):
     l2g = np.concatenate([p["local_to_global"] for p in plans])
     assert np.array_equal(codes, prep["global_codes"][l2g])
     assert len(Tg) == len(T) and int(Tg.max()) < len(Vg)
+
+
+@pytest.mark.parametrize("bits", [17, 20, 24])
+def test_wide_codes(coracle, bits):
+    """Meshes quantised to more than 16 bits per axis take the int32 code
+    path of the split decoder (and the 21-bit packed path of mode 1 where
+    the codes fit); both equal the oracle."""
+    from meshpress_amd import workloads
+    from meshpress_amd.encoder import STRIDEEncoder
+    from meshpress_amd.utils.types import Model
+    V, T = workloads.displaced_torus(60, 20, 1.0, 0.3, 0.05, 4, 9)
+    enc = STRIDEEncoder(precision_error=1.0 / ((1 << bits) - 1.5), precision_mode="bbox_frac")
+    data = enc.encode(Model(V, T)).data
+    assert data[50] >= 17
+    Vo, To = coracle.decode(data)
+    for tune in (dict(mode=2, group_size=16, lds_bytes=40960), dict(mode=2, group_size=3, lds_bytes=8192),
+                 dict(mode=1, group_size=64, lds_bytes=49152), dict(mode=0, waves=1, stage_words=512)):
+        Vg, Tg = gpu_decode(data, **tune)
+        assert_same(Vg, Tg, Vo, To)

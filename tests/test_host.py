@@ -212,17 +212,19 @@ def lane_check(tmp_path_factory):
 
 
 def run_lane_check(exe, data, tmp_path, split=False):
+    """split: False (one lane per meshlet), True or "split" (int32 codes),
+    "split16" (uint16 codes)."""
     import subprocess
     src = tmp_path / "s.dpr4"
     src.write_bytes(data)
     fv, ft = tmp_path / "v.f32", tmp_path / "t.u32"
-    r = subprocess.run([str(exe), str(src), str(fv), str(ft)] + (["split"] if split else []),
-                       capture_output=True, text=True)
+    mode = [] if not split else (["split"] if split is True else [split])
+    r = subprocess.run([str(exe), str(src), str(fv), str(ft)] + mode, capture_output=True, text=True)
     return r.returncode, np.fromfile(fv, np.float32).reshape(-1, 3) if r.returncode in (0, 3) else None, \
         np.fromfile(ft, np.uint32).reshape(-1, 3) if r.returncode in (0, 3) else None
 
 
-@pytest.mark.parametrize("split", [False, True], ids=["lane", "split"])
+@pytest.mark.parametrize("split", [False, "split", "split16"], ids=["lane", "split", "split16"])
 def test_lane_decoder_matches_oracle(lane_check, coracle, tmp_path, split):
     """The gfx950 kernels' pre-pass and per-lane decoders (one lane per
     meshlet, and the split token/axis lanes), compiled for the host,
