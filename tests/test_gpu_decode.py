@@ -208,6 +208,16 @@ def test_icosphere_l8_full_size(coracle, bits):
     l2g = np.concatenate([p["local_to_global"] for p in plans])
     assert np.array_equal(codes, prep["global_codes"][l2g])
     assert len(Tg) == len(T) and int(Tg.max()) < len(Vg)
+    # every decoded vertex lies within half a quantisation step of its source
+    # vertex (plus float32 rounding of the output)
+    src = np.asarray(V, np.float64)[l2g]
+    mx = (1 << prep["g_bits"].astype(np.int64)) - 1
+    half = 0.5 * prep["g_range"].astype(np.float64) / mx * prep["scale"]
+    err = np.abs(Vg.astype(np.float64) - src)
+    assert np.all(err <= half * (1 + 1e-6) + np.abs(src) * 2.0 ** -23 + 1e-12)
+    # triangles name the same source vertices as the input (as sets per face)
+    got = np.sort(l2g[Tg.astype(np.int64)], axis=1)
+    assert np.array_equal(np.unique(got, axis=0), np.unique(np.sort(T, axis=1), axis=0))
 
 
 @pytest.mark.parametrize("bits", [17, 20, 24])

@@ -85,3 +85,22 @@ def test_encoder_c1_generalized(oracle, coracle):
     _, _, codes = coracle.decode(out.data, with_codes=True)
     l2g = np.concatenate([pl["local_to_global"] for pl in prep["plans"]]).astype(np.int64)
     assert np.array_equal(codes, prep["global_codes"][l2g])
+
+
+def test_compressed_model_formulas_and_size():
+    """STRIDEEncoder.encode returns CompressedModel with the reference's
+    bpv/bpt formulas (encoder/implementation/stride.py:45-53); its bytes
+    equal encode_prepared on the same prep; the size lands in the reference
+    test's 20 < bpv < 60 band (tests/test_stride_encoder.py:45-49) for the
+    65,536-triangle sphere."""
+    from meshpress_amd import workloads
+    from meshpress_amd.encoder import STRIDEEncoder
+    from meshpress_amd.utils.types import Model
+    V, T = workloads.uv_sphere_c1()
+    enc = STRIDEEncoder(max_verts=256, precision_error=0.0005)
+    cm = enc.encode(Model(V, T))
+    assert cm.bits_per_vertex == len(cm.data) * 8 / len(V)
+    assert cm.bits_per_triangle == len(cm.data) * 8 / len(T)
+    assert 20 < cm.bits_per_vertex < 60
+    again = enc.encode_prepared(enc.prepare(Model(V, T)))
+    assert again == cm.data

@@ -335,3 +335,38 @@ def test_roundtrip_invariant(oracle, coracle, pred):
         Vc, Tc, Cc = coracle.decode(data, with_codes=True)
         assert np.array_equal(Cc, C) and np.array_equal(Tc, Td)
         assert np.array_equal(Vc.view(np.uint32), Vd.view(np.uint32))
+
+
+from hypothesis import given, settings, strategies as st  # noqa: E402
+
+_ops = st.lists(st.tuples(st.integers(0, 2), st.integers(0, (1 << 24) - 1), st.integers(0, 11)),
+                min_size=1, max_size=60)
+
+
+@settings(max_examples=200, deadline=None)
+@given(_ops)
+def test_bitcodec_roundtrip_hypothesis(ops):
+    """Fixed-width, Rice(k) and Exp-Golomb(k) fields written back to back
+    (MSB first, no alignment) read back exactly, and the stream length is
+    the sum of the closed-form field lengths."""
+    from oracle import stride_oracle as O
+    w = O.BitWriter()
+    bits = 0
+    for kind, u, k in ops:
+        if kind == 0:
+            n = max(1, u.bit_length())
+            w.put(u, n)
+            bits += n
+        elif kind == 1:
+            w.put_rice(u, k)
+            bits += (u >> k) + 1 + k
+        else:
+            w.put_eg(u, k)
+            lb = ((u >> k) + 1).bit_length() - 1
+            bits += 2 * lb + 1 + k
+    data = w.tobytes()
+    assert len(data) == (bits + 7) // 8
+    r = O.BitReader(data)
+    for kind, u, k in ops:
+        got = r.get(max(1, u.bit_length())) if kind == 0 else (r.rice(k) if kind == 1 else r.eg(k))
+        assert got == u
