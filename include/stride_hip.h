@@ -84,7 +84,7 @@ int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes,
 int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
 
 /* Launch configuration (autotuned).
- *   mode 2 (default; group_size 16, lds_bytes 40960): split groups of up to
+ *   mode 2 (default; group_size 16, lds_bytes 24576): split groups of up to
  *     16 consecutive meshlets per wave, four lanes per meshlet (one decodes
  *     the strip tokens into a walk list in LDS, then one per axis decodes
  *     that axis's residual substreams and prediction chain); lds_bytes of

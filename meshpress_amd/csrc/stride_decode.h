@@ -17,7 +17,7 @@ struct DecodeConfig {
   int grid = 0;           // 0 = one workgroup per unit of work, else persistent grid
   int use_queue = 0;      // mode 0: persistent grid pulls meshlets from an atomic counter
   int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; mode 2: 1..16
-  int lds_bytes = 40960;  // modes 1, 2: LDS per group (stage words + codes / walk blocks)
+  int lds_bytes = 24576;  // modes 1, 2: LDS per group (stage words + codes / walk blocks)
 };
 
 struct DecodeLaunch {

@@ -209,12 +209,14 @@ def test_icosphere_l8_full_size(coracle, bits):
     assert np.array_equal(codes, prep["global_codes"][l2g])
     assert len(Tg) == len(T) and int(Tg.max()) < len(Vg)
     # every decoded vertex lies within half a quantisation step of its source
-    # vertex (plus float32 rounding of the output)
+    # vertex, plus the float32 rounding of the stored header constants
+    # (g_min, g_range, scale, center) and of the output
     src = np.asarray(V, np.float64)[l2g]
     mx = (1 << prep["g_bits"].astype(np.int64)) - 1
     half = 0.5 * prep["g_range"].astype(np.float64) / mx * prep["scale"]
     err = np.abs(Vg.astype(np.float64) - src)
-    assert np.all(err <= half * (1 + 1e-6) + np.abs(src) * 2.0 ** -23 + 1e-12)
+    slack = half * 1e-3 + 2.0 ** -21 * (np.abs(src) + prep["scale"])
+    assert np.all(err <= half + slack), float((err - half).max())
     # triangles name the same source vertices as the input (as sets per face)
     got = np.sort(l2g[Tg.astype(np.int64)], axis=1)
     assert np.array_equal(np.unique(got, axis=0), np.unique(np.sort(T, axis=1), axis=0))
