@@ -88,7 +88,8 @@ int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
  *     16 consecutive meshlets per wave, four lanes per meshlet (one decodes
  *     the strip tokens into a walk list in LDS, then one per axis decodes
  *     that axis's residual substreams and prediction chain); lds_bytes of
- *     LDS per group (staged bits + 16 bytes per vertex).
+ *     LDS per group (staged bits + 10 bytes per vertex with uint16 codes,
+ *     used when every axis has at most 16 bits, else 16).
  *   mode 1: groups of up to group_size (<= 64) consecutive meshlets per
  *     wave, one lane per meshlet, lds_bytes of LDS per group (staged bits +
  *     packed codes).
