@@ -4,7 +4,8 @@ Times candidate launch shapes on a decoder (ParaDeltaV5GpuDecoder or
 BatchDecoder, anything with set_tune/get_tune/decode) with HIP events on the
 launch stream and keeps the fastest.  The search space:
 
-  mode 2 (split groups, 4 lanes per meshlet): group_size x lds_bytes
+  mode 2 (split groups, 4 lanes per meshlet): group_size x lds_bytes, and
+                                     persistent grids with a device work queue
   mode 1 (lane-per-meshlet groups):  group_size x lds_bytes x grid
   mode 0 (wave per meshlet):         waves x stage_words (x persistent queue)
 
@@ -23,12 +24,16 @@ LDS_BUDGETS = (16384, 24576, 32768, 40960, 49152, 65536)
 
 
 SPLIT_GROUP_SIZES = (4, 8, 12, 16)
+QUEUE_GRIDS = (1024, 1536, 3072)
 
 
 def candidates(include_wave_mode: bool = True, grids=(0,)):
     out = []
     for gs, lds, grid in itertools.product(SPLIT_GROUP_SIZES, LDS_BUDGETS, grids):
         out.append(dict(mode=2, group_size=gs, lds_bytes=lds, grid=grid, use_queue=0))
+    # persistent grids pulling groups from the device work queue
+    for g in QUEUE_GRIDS:
+        out.append(dict(mode=2, group_size=16, lds_bytes=24576, grid=g, use_queue=1))
     for gs, lds, grid in itertools.product(GROUP_SIZES, LDS_BUDGETS, grids):
         out.append(dict(mode=1, group_size=gs, lds_bytes=lds, grid=grid, use_queue=0))
     if include_wave_mode:

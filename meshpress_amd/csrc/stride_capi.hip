@@ -379,6 +379,8 @@ struct DecodeSet {
       HIP_TRY(hipMalloc(&d_flags, 4 * n_groups));
       hipError_t e = hipMemset(d_flags, 0, 4 * n_groups);
       GroupLaunch g = group_args(d_v_own, d_t_own, 0, n_groups, d_flags);
+      g.queue = nullptr;   // validation: one workgroup per group
+      g.grid = 0;
       if (e == hipSuccess) e = launch_group(g, nullptr);
       if (e == hipSuccess) e = hipDeviceSynchronize();
       std::vector<uint32_t> flags(n_groups, 0);
@@ -441,6 +443,7 @@ struct DecodeSet {
     g.grid = (uint32_t)cfg.grid;
     g.split = cfg.mode == 2;
     g.narrow = narrow();
+    g.queue = (cfg.mode == 2 && cfg.use_queue && cfg.grid > 0) ? d_queue : nullptr;
     return g;
   }
 
@@ -475,7 +478,14 @@ struct DecodeSet {
   int enqueue(float* V, uint32_t* T, hipStream_t s, int mesh0, int mesh1) {
     if (cfg.mode == 1 || cfg.mode == 2) {
       const uint32_t g0 = g_base[mesh0], g1 = g_base[mesh1];
-      if (g1 > g0) HIP_TRY(launch_group(group_args(V, T, g0, g1, nullptr), s));
+      if (g1 > g0) {
+        GroupLaunch g = group_args(V, T, g0, g1, nullptr);
+        if (g.queue) {   // a fill kernel, not a memset, so hipGraph capture stays clean
+          hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_queue, 0u);
+          HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(launch_group(g, s));
+      }
       const uint32_t l0 = l_base[mesh0], l1 = l_base[mesh1];
       if (l1 > l0) {
         DecodeConfig c = cfg;

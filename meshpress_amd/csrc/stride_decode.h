@@ -47,6 +47,7 @@ struct GroupLaunch {
   uint32_t grid;          // 0 = one workgroup per group
   int split;              // mode 2 kernel
   int narrow;             // mode 2: uint16 codes (every mesh has g_bits <= 16)
+  uint32_t* queue;        // mode 2: zeroed work-queue counter (persistent grid), or null
 };
 
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream);

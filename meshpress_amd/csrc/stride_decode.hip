@@ -373,13 +373,22 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
     const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
     uint32_t n_groups, float* __restrict__ V, uint32_t* __restrict__ T,
-    uint32_t* __restrict__ overflow) {
+    uint32_t* __restrict__ overflow, uint32_t* __restrict__ queue) {
   using CodeT = typename std::conditional<NARROW, uint16_t, int32_t>::type;
   extern __shared__ uint32_t lds32[];
   const int lane = threadIdx.x;
   const int slot = lane & 15, role = lane >> 4;
   const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
-  for (uint32_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
+  // groups come from the device work queue (a counter zeroed before the
+  // launch) or, without one, in grid-stride order; every wave leaves once
+  // the index passes n_groups
+  auto next = [&](uint32_t cur) -> uint32_t {
+    if (!queue) return cur + gridDim.x;
+    uint32_t t = 0;
+    if (lane == 0) t = atomicAdd(queue, 1u);
+    return __builtin_amdgcn_readfirstlane(t);
+  };
+  for (uint32_t g = queue ? next(0) : blockIdx.x; g < n_groups; g = next(g)) {
     const GroupEntry G = groups[g];
     for (uint32_t i = lane; i < G.n_words; i += 64) lds32[i] = __builtin_bswap32(words[G.w_lo + i]);
     const uint32_t m = G.m0 + (uint32_t)slot;
@@ -462,10 +471,12 @@ hipError_t launch_group(const GroupLaunch& d, hipStream_t stream) {
   const uint32_t grid = d.grid ? std::min(d.grid, d.n_groups) : d.n_groups;
   if (d.split && d.narrow)
     hipLaunchKernelGGL(stride_decode_split_kernel<true>, dim3(grid), dim3(64), d.lds_bytes, stream,
-                       d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
+                       d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow,
+                       d.queue);
   else if (d.split)
     hipLaunchKernelGGL(stride_decode_split_kernel<false>, dim3(grid), dim3(64), d.lds_bytes,
-                       stream, d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);
+                       stream, d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow,
+                       d.queue);
   else
     hipLaunchKernelGGL(stride_decode_group_kernel, dim3(grid), dim3(64), d.lds_bytes, stream,
                        d.buf, d.tab, d.meshes, d.groups, d.n_groups, d.V, d.T, d.overflow);

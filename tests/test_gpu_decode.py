@@ -79,6 +79,8 @@ TUNES = [dict(mode=0, waves=w, stage_words=s) for w in (1, 2, 4) for s in (256, 
          dict(mode=1, group_size=1, lds_bytes=8192),
          dict(mode=2, group_size=16, lds_bytes=40960),
          dict(mode=2, group_size=16, lds_bytes=65536, grid=7),
+         dict(mode=2, group_size=16, lds_bytes=24576, grid=64, use_queue=1),
+         dict(mode=2, group_size=4, lds_bytes=16384, grid=3, use_queue=1),
          dict(mode=2, group_size=5, lds_bytes=16384),
          dict(mode=2, group_size=1, lds_bytes=8192),
          # too small for most meshlets: exercises the wave-per-meshlet fallback list
@@ -175,7 +177,8 @@ def test_batch_decode_matches_single(coracle):
     want = [coracle.decode(b) for b in blobs]
     bd = BatchDecoder(blobs)
     for tune in (dict(), dict(mode=0, waves=4, stage_words=512, grid=128, use_queue=1),
-                 dict(mode=1, group_size=16, lds_bytes=16384, grid=0, use_queue=0)):
+                 dict(mode=1, group_size=16, lds_bytes=16384, grid=0, use_queue=0),
+                 dict(mode=2, group_size=16, lds_bytes=24576, grid=96, use_queue=1)):
         if tune:
             bd.set_tune(**tune)
         for mode in ("launch", "per_mesh", "graph", "graph", "graph"):
