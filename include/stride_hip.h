@@ -67,6 +67,17 @@ typedef struct stride_decoder stride_decoder;
 
 int stride_decoder_create(const uint8_t* data, size_t len, stride_decoder** out);
 void stride_decoder_destroy(stride_decoder* h);
+/* DPRB v4 side-table blob (scripts/dump_stride_dup_blob.py:88-152; the
+ * native decode bench's input, bench_cpp/stride_dup_decode_bench.cu:523-595):
+ * the DPR4 meshlet body plus per-meshlet tables (bit offsets, n_local, n_tris,
+ * n_strips, v_off, t_off, anchor offset, n_kind0, substream offsets).
+ * stride_dprb_from_dpr4 derives the tables on the host with the pre-pass
+ * parse; with out == NULL only *out_len is set.  stride_decoder_create_dprb
+ * decodes a DPRB blob and rejects it (STRIDE_E_CORRUPT) unless every table
+ * entry equals what the pre-pass derives from its body. */
+int stride_dprb_from_dpr4(const uint8_t* data, size_t len, uint8_t* out, size_t cap,
+                          size_t* out_len);
+int stride_decoder_create_dprb(const uint8_t* blob, size_t len, stride_decoder** out);
 /* n_v = sum of per-meshlet n_local (duplicated boundary vertices included),
  * n_t = triangles, n_meshlets. */
 int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,

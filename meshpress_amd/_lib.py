@@ -52,6 +52,8 @@ SIGNATURES = {
     "stride_debug_pyset_order": (C.c_int, [p, i32, p]),
     "stride_decoder_create": (C.c_int, [p, sz, P(p)]),
     "stride_decoder_destroy": (None, [p]),
+    "stride_dprb_from_dpr4": (C.c_int, [p, sz, p, sz, P(sz)]),
+    "stride_decoder_create_dprb": (C.c_int, [p, sz, P(p)]),
     "stride_decoder_sizes": (C.c_int, [p, P(u64), P(u64), P(u32)]),
     "stride_decode": (C.c_int, [p, p, p, p]),
     "stride_decode_to_host": (C.c_int, [p, p, p]),

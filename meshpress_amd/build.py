@@ -22,12 +22,13 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libstride_hip.so"
+BENCH = PKG / "stride_bench"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = "gfx950"
 
 HIP_SOURCES = ["stride_decode.hip", "stride_search.hip", "stride_capi.hip", "stride_mlp.hip"]
-CXX_SOURCES = ["stride_host.cpp"]
+CXX_SOURCES = ["stride_host.cpp", "stride_dprb.cpp"]
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -67,6 +68,11 @@ def build(jobs: int = 8, verbose: bool = False) -> Path:
     _run([HIPCC, "-shared", f"--offload-arch={ARCH}", *[str(o) for o in objs], "-o", str(tmp),
           "-pthread"])
     os.replace(tmp, LIB)
+    # native decode bench over the C ABI (host code only, links the library)
+    cli_tmp = BENCH.with_suffix(".tmp")
+    _run([HIPCC, "-O2", "-std=c++17", str(CSRC / "stride_bench_cli.cpp"), "-o", str(cli_tmp),
+          f"-L{PKG}", "-lstride_hip", "-Wl,-rpath,$ORIGIN"])
+    os.replace(cli_tmp, BENCH)
     return LIB
 
 

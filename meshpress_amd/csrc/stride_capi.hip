@@ -646,6 +646,53 @@ int stride_decoder_create(const uint8_t* data, size_t len, stride_decoder** out)
 
 void stride_decoder_destroy(stride_decoder* h) { delete h; }
 
+int stride_dprb_from_dpr4(const uint8_t* data, size_t len, uint8_t* out, size_t cap,
+                          size_t* out_len) {
+  if (!out_len || (!data && len)) return fail(STRIDE_E_ARG, "null argument");
+  std::vector<uint8_t> blob;
+  std::string err;
+  int code = 0;
+  if (dprb_from_dpr4(data, len, blob, err, code)) return fail(code, err);
+  *out_len = blob.size();
+  if (out) {
+    if (cap < blob.size()) return fail(STRIDE_E_ARG, "output buffer too small");
+    std::memcpy(out, blob.data(), blob.size());
+  }
+  return STRIDE_OK;
+}
+
+int stride_decoder_create_dprb(const uint8_t* blob, size_t len, stride_decoder** out) {
+  if (!out) return fail(STRIDE_E_ARG, "null out");
+  *out = nullptr;
+  if (!blob && len) return fail(STRIDE_E_ARG, "null blob");
+  std::vector<uint8_t> dpr4;
+  DprbTables t;
+  std::string err;
+  int code = 0;
+  if (dprb_to_dpr4(blob, len, dpr4, t, err, code)) return fail(code, err);
+  auto h = std::make_unique<stride_decoder>();
+  const uint8_t* blobs[1] = {dpr4.data()};
+  size_t lens[1] = {dpr4.size()};
+  int rc = h->set.build(blobs, lens, 1);
+  if (rc) return rc;
+  // the blob's tables must be exactly what the pre-pass derived
+  const DecodeSet& s = h->set;
+  if (s.n_m != t.n_m || s.n_v != t.n_v || s.n_t != t.n_t)
+    return fail(STRIDE_E_CORRUPT, "DPRB counts disagree with the meshlets");
+  for (uint32_t m = 0; m < s.n_m; ++m) {
+    const MeshletEntry& e = s.tab[m];
+    bool ok = e.start_bit - t.body_start_bit == t.rel_off[m] && e.n_local == t.n_local[m] &&
+              e.n_tris == t.n_tris[m] && (e.n_strips & kStripMask) == t.n_strips[m] &&
+              e.v_off == t.v_off[m] && e.t_off == t.t_off[m] && e.n_kind0 == t.n_kind0[m] &&
+              e.resid_bit - t.body_start_bit == t.resid_off[m];
+    for (int j = 1; j < 6 && ok; ++j)
+      ok = (uint16_t)(e.sub_bit[j] - e.resid_bit) == t.subs[5 * m + j - 1];
+    if (!ok) return fail(STRIDE_E_CORRUPT, "DPRB table disagrees with meshlet " + std::to_string(m));
+  }
+  *out = h.release();
+  return STRIDE_OK;
+}
+
 int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
                          uint32_t* n_meshlets) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");

@@ -121,6 +121,20 @@ void para_columns(const Prep& P, const int64_t* codes, const int32_t* sel, int64
                   int32_t* cols, int64_t* off, int n_threads);
 
 
+// DPRB v4 side-table blob <-> DPR4 v3 (stride_dprb.cpp).  Return 0 or a
+// negative status (the STRIDE_E_* values) with a message in err.
+struct DprbTables {
+  uint32_t n_m = 0, n_v = 0, n_t = 0;
+  uint64_t body_start_bit = 0;                  // of the rebuilt DPR4 stream
+  std::vector<uint64_t> rel_off, resid_off;     // body-relative bit offsets
+  std::vector<uint32_t> n_local, n_tris, n_strips, v_off, t_off, n_kind0;
+  std::vector<uint16_t> subs;                   // 5 per meshlet
+};
+int dprb_from_dpr4(const uint8_t* data, size_t len, std::vector<uint8_t>& out, std::string& err,
+                   int& code);
+int dprb_to_dpr4(const uint8_t* blob, size_t len, std::vector<uint8_t>& dpr4, DprbTables& t,
+                 std::string& err, int& code);
+
 // tagk: per meshlet 12 bytes: (tag,k) x 3 delta then (tag,k) x 3 para.
 std::vector<uint8_t> write_dup(const Prep& P, const Residuals& R, const Header& H,
                                const uint8_t* tagk);

@@ -2,7 +2,8 @@
 
 `ParaDeltaV5GpuDecoder(data)` keeps the reference's class name, module path
 and keyword, and decodes the DPR4 v3 bytes that `STRIDEEncoder.encode()`
-returns, on MI355X through libstride_hip.so:
+returns (or the DPRB side-table blob of utils/dprb.py, told apart by its
+magic), on MI355X through libstride_hip.so:
 
 * construction uploads the stream, runs the index pre-pass kernel that
   derives the per-meshlet side table, validates every meshlet and allocates
@@ -48,8 +49,8 @@ class ParaDeltaV5GpuDecoder:
         h = C.c_void_p()
         buf = np.frombuffer(self.data, dtype=np.uint8)
         _lib.check(L.stride_init(self.device.index))
-        _lib.check(L.stride_decoder_create(_lib.ptr(buf) if len(buf) else None, len(buf),
-                                           C.byref(h)))
+        create = L.stride_decoder_create_dprb if self.data[:4] == b"DPRB" else L.stride_decoder_create
+        _lib.check(create(_lib.ptr(buf) if len(buf) else None, len(buf), C.byref(h)))
         self._h = h
         nv, nt, nm = C.c_uint64(), C.c_uint64(), C.c_uint32()
         _lib.check(L.stride_decoder_sizes(h, C.byref(nv), C.byref(nt), C.byref(nm)))

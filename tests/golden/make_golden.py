@@ -19,7 +19,11 @@ parts no closed form pins:
     sample cache (the cache would be written into the read-only reference
     tree);
   * _pick_best_k (paradelta_v5_dup.py:159-171) and encoder_cost_batch
-    (_irlp_fit.py:186-226) on seeded random residuals.
+    (_irlp_fit.py:186-226) on seeded random residuals;
+  * the DPRB v4 side-table blob of scripts/dump_stride_dup_blob.py:88-152,
+    built from encode_dup(return_meta=True)'s own per-meshlet meta (anchor
+    offsets, n_kind0, substream offsets) with the layout that script packs
+    (`python tests/golden/make_golden.py dprb` writes only these).
 
 Fixtures are data only: inputs (bytes, sample arrays) and expected outputs.
 Large outputs are stored as SHA-256 digests.
@@ -181,5 +185,58 @@ def main():
     print("coder_choice", list(zip(tags, ks))[:10])
 
 
+def dprb_goldens():
+    """DPRB blobs from the reference encoder's meta, for two small meshes."""
+    import struct
+    from meshpress_amd.encoder.prep import prepare
+    sys.path.insert(0, str(REF))
+    from encoder.paradelta_v5_dup import encode_dup  # noqa: E402
+    for name, V, T in meshes():
+        if name not in ("torus_30x10", "uv_c1"):
+            continue
+        p = prepare(V, T)
+        prep = dict(p)
+        prep.pop("native")
+        plans = p["native"].export_plans()
+        for pl in plans:
+            pl.pop("walk")
+        prep["plans"] = plans
+        data, meta = encode_dup(prep, predictor="canonical", return_meta=True)
+        n_m = int(prep["n_meshlets"])
+        head = 86 + 4 * n_m                       # DPR4 header + offset table, bytes
+        body = data[head:]
+        offs = [struct.unpack(">I", data[86 + 4 * i:90 + 4 * i])[0] - 8 * head for i in range(n_m)]
+        hdr = {k: data[a:b] for k, a, b in (("f", 6, 50),)}
+        f11 = struct.unpack(">11f", hdr["f"])
+        g_bits = list(data[50:53])
+        pred_n = [struct.unpack(">h", data[53 + 2 * i:55 + 2 * i])[0] for i in range(9)]
+        pred_K = list(data[71:74])
+        n_local = np.array([len(pl["local_to_global"]) for pl in plans], np.uint32)
+        n_tris = np.array([pl["n_tris_m"] for pl in plans], np.uint32)
+        n_strips = np.array([pl["n_strips"] for pl in plans], np.uint32)
+        v_off = np.concatenate([[0], np.cumsum(n_local[:-1])]).astype(np.uint32)
+        t_off = np.concatenate([[0], np.cumsum(n_tris[:-1])]).astype(np.uint32)
+        out = bytearray()
+        out += struct.pack("<II", 0x42525044, 4)
+        out += struct.pack("<III", n_m, int(n_local.sum()), int(n_tris.sum()))
+        out += struct.pack("<I", len(body)) + struct.pack("<II", 0, 0)
+        out += struct.pack("<11f", *f11) + struct.pack("<III", *g_bits)
+        out += struct.pack("<9h", *pred_n) + struct.pack("<3B", *pred_K) + b"\x00\x00\x00"
+        out += body
+        out += np.array(offs, np.uint64).tobytes()
+        for a in (n_local, n_tris, n_strips, v_off, t_off):
+            out += a.tobytes()
+        out += np.asarray(meta["resid_offs_rel"], np.uint64).tobytes()
+        out += np.asarray(meta["n_kind0"], np.uint32).tobytes()
+        out += np.asarray(meta["axis_sub_offs"], np.uint16).tobytes()
+        np.savez_compressed(HERE / f"dprb_{name}.npz", dpr4=np.frombuffer(data, np.uint8),
+                            dprb=np.frombuffer(bytes(out), np.uint8))
+        print("dprb", name, len(data), len(out), flush=True)
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["dprb"]:
+        dprb_goldens()
+    else:
+        main()
+        dprb_goldens()

@@ -242,3 +242,43 @@ def test_wide_codes(coracle, bits):
                  dict(mode=1, group_size=64, lds_bytes=49152), dict(mode=0, waves=1, stage_words=512)):
         Vg, Tg = gpu_decode(data, **tune)
         assert_same(Vg, Tg, Vo, To)
+
+
+def test_dprb_blobs_decode_like_dpr4(coracle):
+    """A DPRB blob decodes to exactly the DPR4 stream's output; a blob whose
+    tables disagree with its body is rejected before any decode."""
+    from meshpress_amd.utils.dprb import to_dprb
+    for name in ("dprb_torus_30x10", "dprb_uv_c1"):
+        z = np.load(GOLD / f"{name}.npz")
+        Vo, To = coracle.decode(z["dpr4"].tobytes())
+        V, T = gpu_decode(z["dprb"].tobytes())
+        assert_same(V, T, Vo, To)
+    blob = bytearray(to_dprb(np.load(GOLD / "dprb_torus_30x10.npz")["dpr4"].tobytes()))
+    n_m = int.from_bytes(blob[8:12], "little")
+    body = int.from_bytes(blob[20:24], "little")
+    tab = 112 + body + 8 * n_m                   # n_local table
+    blob[tab] ^= 1
+    with pytest.raises(ValueError, match="DPRB"):
+        gpu_decode(bytes(blob))
+
+
+def test_native_bench_cli(tmp_path):
+    """The native bench (C ABI only, no Python in the timed path) decodes a
+    DPR4 stream and its DPRB blob and reports the event-timed rate."""
+    import json
+    import subprocess
+    from meshpress_amd import build
+    if not build.BENCH.exists():
+        pytest.skip("native bench not built")
+    z = np.load(GOLD / "dprb_uv_c1.npz")
+    for fmt, key in (("dpr4", "dpr4"), ("dprb", "dprb")):
+        f = tmp_path / f"c1.{fmt}"
+        f.write_bytes(z[key].tobytes())
+        r = subprocess.run([str(build.BENCH), str(f), "--iters", "5", "--warmup", "1"],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        out = json.loads(r.stdout)
+        assert out["format"] == fmt and out["n_tris"] == 65536 and out["ms"] > 0
+    r = subprocess.run([str(build.BENCH), str(tmp_path / "c1.dpr4"), "--mode", "7"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 1 and "set_tune" in r.stderr

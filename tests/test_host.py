@@ -273,3 +273,30 @@ def test_fit_predictor_without_samples_is_canonical():
     z = np.zeros(0, np.int32)
     n, K = search.fit_predictor(z, z, z, z, np.zeros(2, np.int64))
     assert np.array_equal(n, search.CANON_N) and np.array_equal(K, search.CANON_K)
+
+
+@pytest.mark.parametrize("name", ["torus_30x10", "uv_c1"])
+def test_dprb_writer_matches_reference_blob(name):
+    """The DPRB blob derived from the DPR4 bytes alone equals the blob built
+    from the reference encoder's own per-meshlet meta (anchor offsets,
+    n_kind0, substream offsets; scripts/dump_stride_dup_blob.py:88-152)."""
+    from meshpress_amd.utils.dprb import to_dprb
+    z = np.load(ROOT / "tests" / "golden" / f"dprb_{name}.npz")
+    dpr4 = z["dpr4"].tobytes()
+    assert to_dprb(dpr4) == z["dprb"].tobytes()
+    # the stream is the reference encoder's output for the same mesh
+    assert dpr4 == np.load(ROOT / "tests" / "golden" / f"dup_{name}.npz")["data"].tobytes()
+
+
+def test_dprb_writer_rejects_bad_streams():
+    from meshpress_amd.utils.dprb import to_dprb
+    z = np.load(ROOT / "tests" / "golden" / "dprb_torus_30x10.npz")
+    dpr4 = bytearray(z["dpr4"].tobytes())
+    with pytest.raises(ValueError, match="magic"):
+        to_dprb(b"\0" * 100)
+    with pytest.raises(IndexError):
+        to_dprb(bytes(dpr4[:60]))
+    bad = bytearray(dpr4)
+    bad[86 + 4 * 0 + 3] ^= 0x01          # first meshlet offset off by one bit
+    with pytest.raises(ValueError):
+        to_dprb(bytes(bad))
