@@ -225,6 +225,25 @@ int stride_encoder_write(const stride_encoder* h, const float* f13, const int32_
                          const int32_t* pred_n, const int32_t* pred_K, const uint8_t* tagk,
                          uint8_t* out, size_t cap, size_t* len);
 
+/* PDLT v5 writer over the same plans (encode_from_prepared_v5,
+ * encoder/paradelta_v5.py:144-283, with the float32 chain and meshlet
+ * layout of encoder/paradelta_v5_nb.py:20-539).  Three calls:
+ *   _v5_prepare: vn = normalised positions (n_v x 3 f64, as passed to
+ *     stride_encoder_create), f13 / g_bits as for stride_encoder_write.
+ *     Runs the plain-parallelogram pass; returns the boundary-table size and
+ *     the number of LIN3 fit samples;
+ *   _v5_samples: the samples as four n_samples x 3 f64 arrays (a, b, c
+ *     reconstructions and the true position), for the least-squares fit
+ *     (_fit_linear3, encoder/paradelta_codec.py:626-639) done by the caller;
+ *   _write_v5: the LIN3 pass with the fitted float32 weights and the bytes
+ *     (out == NULL: only *len). */
+int stride_encoder_v5_prepare(stride_encoder* h, const double* vn, const float* f13,
+                              const int32_t* g_bits, int64_t* n_boundary, int64_t* n_samples);
+int stride_encoder_v5_samples(const stride_encoder* h, double* A, double* B, double* C,
+                              double* T);
+int stride_encoder_write_v5(const stride_encoder* h, const float* lin3, uint8_t* out, size_t cap,
+                            size_t* len);
+
 /* ------------------------------------------------------------------------ */
 /* GPU search kernels (device pointers). */
 

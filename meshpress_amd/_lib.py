@@ -79,6 +79,9 @@ SIGNATURES = {
     "stride_encoder_residuals": (C.c_int, [p, p, p, P(i64), P(i64)]),
     "stride_encoder_export_residuals": (C.c_int, [p, p, p]),
     "stride_encoder_write": (C.c_int, [p, p, p, p, p, p, p, sz, P(sz)]),
+    "stride_encoder_v5_prepare": (C.c_int, [p, p, p, p, P(i64), P(i64)]),
+    "stride_encoder_v5_samples": (C.c_int, [p, p, p, p, p]),
+    "stride_encoder_write_v5": (C.c_int, [p, p, p, sz, P(sz)]),
     "stride_select_codes": (C.c_int, [p, p, i64, p, p, p]),
     "stride_stream_costs": (C.c_int, [p, p, i64, p, p]),
     "stride_irlp_cost": (C.c_int, [p, p, p, p, p, i64, p, i32, i32, p, p]),

@@ -595,6 +595,10 @@ struct stride_batch {
 struct stride_encoder {
   Prep prep;
   std::vector<int64_t> codes;
+  // PDLT v5 state (stride_encoder_v5_prepare)
+  std::vector<double> vn5, rows5;
+  V5Header h5;
+  bool have_v5 = false;
   Residuals res;
   bool have_res = false;
   int32_t pred_n[9];
@@ -1013,6 +1017,57 @@ int stride_encoder_write(const stride_encoder* h, const float* f13, const int32_
   H.per_coord_err = f13[4];
   for (int i = 0; i < 9; ++i) H.pred_n[i] = pred_n[i];
   std::vector<uint8_t> bytes = write_dup(h->prep, h->res, H, tagk);
+  *len = bytes.size();
+  if (out) {
+    if (cap < bytes.size()) return fail(STRIDE_E_ARG, "output buffer too small");
+    std::memcpy(out, bytes.data(), bytes.size());
+  }
+  return STRIDE_OK;
+}
+
+// ------------------------------------------------------------ PDLT v5 writer
+int stride_encoder_v5_prepare(stride_encoder* h, const double* vn, const float* f13,
+                              const int32_t* g_bits, int64_t* n_boundary, int64_t* n_samples) {
+  if (!h || !vn || !f13 || !g_bits) return fail(STRIDE_E_ARG, "null argument");
+  V5Header H;
+  for (int i = 0; i < 3; ++i) {
+    H.center[i] = f13[i];
+    H.g_min[i] = f13[5 + i];
+    H.g_range[i] = f13[8 + i];
+    H.g_bits[i] = g_bits[i];
+    if (g_bits[i] < 1 || g_bits[i] > 30) return fail(STRIDE_E_ARG, "g_bits outside [1, 30]");
+  }
+  H.scale = f13[3];
+  H.per_coord_err = f13[4];
+  h->h5 = H;
+  h->vn5.assign(vn, vn + 3 * h->prep.n_v);
+  v5_samples(h->prep, h->codes.data(), h->vn5.data(), H, h->rows5, h->n_threads);
+  h->have_v5 = true;
+  if (n_boundary) *n_boundary = (int64_t)h->prep.boundary.size();
+  if (n_samples) *n_samples = (int64_t)(h->rows5.size() / 12);
+  return STRIDE_OK;
+}
+
+int stride_encoder_v5_samples(const stride_encoder* h, double* A, double* B, double* C,
+                              double* T) {
+  if (!h || !A || !B || !C || !T) return fail(STRIDE_E_ARG, "null argument");
+  if (!h->have_v5) return fail(STRIDE_E_ARG, "stride_encoder_v5_prepare not called");
+  const size_t n = h->rows5.size() / 12;
+  double* dst[4] = {A, B, C, T};
+  for (size_t i = 0; i < n; ++i)
+    for (int q = 0; q < 4; ++q)
+      for (int d = 0; d < 3; ++d) dst[q][3 * i + d] = h->rows5[12 * i + 3 * q + d];
+  return STRIDE_OK;
+}
+
+int stride_encoder_write_v5(const stride_encoder* h, const float* lin3, uint8_t* out, size_t cap,
+                            size_t* len) {
+  if (!h || !lin3 || !len) return fail(STRIDE_E_ARG, "null argument");
+  if (!h->have_v5) return fail(STRIDE_E_ARG, "stride_encoder_v5_prepare not called");
+  if (h->prep.boundary.size() > 0xFFFFFFFFull || h->prep.n_v > 0xFFFFFFFFll)
+    return fail(STRIDE_E_ARG, "mesh too large for 32-bit PDLT fields");
+  std::vector<uint8_t> bytes = write_v5(h->prep, h->codes.data(), h->vn5.data(), h->h5, lin3,
+                                        h->n_threads);
   *len = bytes.size();
   if (out) {
     if (cap < bytes.size()) return fail(STRIDE_E_ARG, "output buffer too small");

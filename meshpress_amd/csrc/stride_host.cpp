@@ -906,6 +906,8 @@ bool build_prep(const int64_t* tris, int64_t n_t, const double* vn, int64_t n_v,
                    [&](int64_t x, int64_t y) { return mk[x] < mk[y]; });
   std::vector<int32_t> ref(n_v, -1);
   for (size_t i = 0; i < bord.size(); ++i) ref[bnd[bord[i]]] = (int32_t)i;
+  out.boundary.resize(bord.size());
+  for (size_t i = 0; i < bord.size(); ++i) out.boundary[i] = bnd[bord[i]];
   last.clear(); last.shrink_to_fit();
 
   out.plans.resize(n_m);
@@ -1284,6 +1286,257 @@ std::vector<uint8_t> write_dup(const Prep& P, const Residuals& R, const Header& 
   w.put(total_local, 32);
   w.put((uint64_t)P.n_t, 32);
   w.put((uint64_t)n_m, 32);
+  const uint64_t region = w.bit_pos() + 32ULL * (uint64_t)n_m;
+  for (int64_t m = 0; m < n_m; ++m) w.put(region + rel[m], 32);
+  std::vector<uint8_t> out = w.finish();
+  out.insert(out.end(), body_bytes.begin(), body_bytes.end());
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// PDLT v5 writer (encoder/paradelta_v5.py:144-283; the per-meshlet chain and
+// bit layout of encoder/paradelta_v5_nb.py:20-539).
+// ---------------------------------------------------------------------------
+namespace {
+
+// g_min + code / (2^bits - 1) * g_range in f64 (paradelta_codec.py:167-175).
+inline double dequant64(int64_t code, const V5Header& H, int d) {
+  const int64_t mx = ((int64_t)1 << H.g_bits[d]) - 1;
+  if (mx == 0) return (double)H.g_min[d];
+  return (double)H.g_min[d] + (double)code / (double)mx * (double)H.g_range[d];
+}
+
+// The float32 closed-loop chain of one meshlet (interior_pass_strip_nb).
+// w3 == nullptr: plain parallelogram a + b - c, fit samples appended to
+// rows; otherwise LIN3 with the stored weights, codes written (n_int x 3).
+void v5_chain(const Prep& P, int64_t m, const int64_t* gcodes, const double* vn,
+              const V5Header& H, const float* w3, int64_t* codes, std::vector<double>* rows) {
+  const Plan& pl = P.plans[m];
+  const int n_bnd = pl.n_bnd;
+  const int n_local = (int)pl.l2g.size();
+  float recon[256][3];
+  for (int l = 0; l < n_bnd; ++l)
+    for (int d = 0; d < 3; ++d) recon[l][d] = (float)dequant64(gcodes[3 * pl.l2g[l] + d], H, d);
+  float fb[3] = {0.f, 0.f, 0.f};
+  if (n_bnd > 0) {
+    float s[3] = {0.f, 0.f, 0.f};
+    for (int l = 0; l < n_bnd; ++l)
+      for (int d = 0; d < 3; ++d) s[d] = s[d] + recon[l][d];
+    const float nb = (float)n_bnd;
+    for (int d = 0; d < 3; ++d) fb[d] = s[d] / nb;
+  }
+  const float delta = (float)(2.0 * (double)H.per_coord_err);
+  int i = 0;
+  for (const WalkEntry& e : P.walks[m]) {
+    const int v = e.v;
+    if (v < n_bnd) continue;
+    const int64_t g = pl.l2g[v];
+    float pred[3];
+    for (int d = 0; d < 3; ++d) {
+      if (e.kind == 0) {
+        pred[d] = fb[d];
+      } else if (!w3) {
+        pred[d] = (recon[e.a][d] + recon[e.b][d]) - recon[e.c][d];
+      } else {
+        const float t0 = w3[0] * recon[e.a][d];
+        const float t1 = w3[1] * recon[e.b][d];
+        const float t2 = w3[2] * recon[e.c][d];
+        pred[d] = (t0 + t1) + t2;
+      }
+    }
+    for (int d = 0; d < 3; ++d) {
+      const float tv = (float)vn[3 * g + d];
+      const float q = (tv - pred[d]) / delta;
+      const int64_t c = (int64_t)std::nearbyint(q);   // half to even, as np.round
+      if (codes) codes[3 * i + d] = c;
+      recon[v][d] = pred[d] + (float)c * delta;
+    }
+    if (rows && e.kind == 1) {
+      for (int d = 0; d < 3; ++d) rows->push_back((double)recon[e.a][d]);
+      for (int d = 0; d < 3; ++d) rows->push_back((double)recon[e.b][d]);
+      for (int d = 0; d < 3; ++d) rows->push_back((double)recon[e.c][d]);
+      for (int d = 0; d < 3; ++d) rows->push_back(vn[3 * g + d]);
+    }
+    ++i;
+  }
+  (void)n_local;
+}
+
+inline uint64_t zz64(int64_t v) { return v < 0 ? ((uint64_t)(-(v + 1)) << 1) + 1 : ((uint64_t)v << 1); }
+
+// Lowest Rice cost over k = 0..k_max (lowest k on ties).
+int best_rice(const uint64_t* u, size_t n, int k_max, int64_t& bits) {
+  int best_k = 0;
+  bits = INT64_MAX;
+  for (int k = 0; k <= k_max; ++k) {
+    int64_t b = (int64_t)n * (1 + k);
+    for (size_t i = 0; i < n; ++i) b += (int64_t)(u[i] >> k);
+    if (b < bits) { bits = b; best_k = k; }
+  }
+  return best_k;
+}
+
+int best_eg(const uint64_t* u, size_t n, int k_max, int64_t& bits) {
+  int best_k = 0;
+  bits = INT64_MAX;
+  for (int k = 0; k <= k_max; ++k) {
+    int64_t b = 0;
+    for (size_t i = 0; i < n; ++i) b += 2 * (63 - __builtin_clzll((u[i] >> k) + 1)) + 1 + k;
+    if (b < bits) { bits = b; best_k = k; }
+  }
+  return best_k;
+}
+
+void v5_meshlet(BitWriter& w, const Plan& pl, const std::vector<int32_t>& ref, const int64_t* codes) {
+  const int n_bnd = pl.n_bnd;
+  const int n_local = (int)pl.l2g.size();
+  const int n_int = n_local - n_bnd;
+  const int n_tris = (int)pl.tris_local.size() / 3;
+  w.put((uint64_t)n_bnd, 16);
+  w.put((uint64_t)n_int, 16);
+  w.put((uint64_t)n_tris, 16);
+  w.put(pl.strip_len.size(), 16);
+  if (n_bnd > 0) {
+    w.put((uint64_t)ref[pl.l2g[0]], 32);
+    if (n_bnd > 1) {
+      std::vector<uint64_t> diffs(n_bnd - 1);
+      for (int i = 0; i + 1 < n_bnd; ++i)
+        diffs[i] = (uint64_t)(ref[pl.l2g[i + 1]] - ref[pl.l2g[i]] - 1);
+      int64_t bits;
+      const int k = best_rice(diffs.data(), diffs.size(), 11, bits);
+      w.put((uint64_t)k, 8);
+      for (uint64_t x : diffs) w.put_rice(x, k);
+    }
+  }
+  const int ib = idx_bits_for(n_local);
+  int fifo[kFifoSize];
+  int fn = 0;
+  const uint8_t* tl = pl.tris_local.data();
+  size_t pos = 0;
+  for (size_t s = 0; s < pl.strip_len.size(); ++s) {
+    const int L = pl.strip_len[s];
+    const int32_t* st = pl.strip_tris.data() + pos;
+    pos += L;
+    w.put((uint64_t)L, 16);
+    int root[3];
+    root_orient(tl, st[0], L > 1 ? st[1] : -1, root);
+    for (int j = 0; j < 3; ++j) emit_vert(w, root[j], fifo, fn, ib);
+    int prev[3] = {root[0], root[1], root[2]};
+    for (int i = 1; i < L; ++i) {
+      int a, b, c, nv, np[3];
+      const int ec = strip_step(prev, tl + 3 * st[i], a, b, c, nv, np);
+      w.put((uint64_t)ec, 1);
+      emit_vert(w, nv, fifo, fn, ib);
+      prev[0] = np[0]; prev[1] = np[1]; prev[2] = np[2];
+    }
+  }
+  if (n_int == 0) return;
+  std::vector<uint64_t> u(n_int);
+  for (int d = 0; d < 3; ++d) {
+    int64_t mn = codes[d], mx = codes[d];
+    for (int i = 0; i < n_int; ++i) {
+      mn = std::min(mn, codes[3 * i + d]);
+      mx = std::max(mx, codes[3 * i + d]);
+      u[i] = zz64(codes[3 * i + d]);
+    }
+    const int64_t rng = mx - mn;
+    const int bw = rng > 0 ? 64 - __builtin_clzll((uint64_t)rng + 1) : 1;
+    int64_t best = (mn < -32768 || mn > 32767) ? INT64_MAX : 8 + 16 + 8 + (int64_t)n_int * bw;
+    int64_t rb, eb;
+    const int rk = best_rice(u.data(), u.size(), 11, rb);
+    const int ek = best_eg(u.data(), u.size(), 7, eb);
+    int tag = 0;
+    if (16 + rb < best) { best = 16 + rb; tag = 1; }
+    if (16 + eb < best) { best = 16 + eb; tag = 2; }
+    w.put((uint64_t)tag, 8);
+    if (tag == 0) {
+      w.put((uint64_t)mn & 0xFFFF, 16);
+      w.put((uint64_t)bw, 8);
+      for (int i = 0; i < n_int; ++i) w.put((uint64_t)(codes[3 * i + d] - mn), bw);
+    } else {
+      const int k = tag == 1 ? rk : ek;
+      w.put((uint64_t)k, 8);
+      for (int i = 0; i < n_int; ++i) {
+        if (tag == 1) w.put_rice(u[i], k);
+        else w.put_eg(u[i], k);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+void v5_samples(const Prep& P, const int64_t* codes, const double* vn, const V5Header& H,
+                std::vector<double>& rows, int n_threads) {
+  const int64_t n_m = (int64_t)P.plans.size();
+  std::vector<std::vector<double>> per(n_m);
+  parallel_for(n_m, n_threads, [&](int64_t m, int) { v5_chain(P, m, codes, vn, H, nullptr, nullptr, &per[m]); });
+  size_t n = 0;
+  for (auto& r : per) n += r.size();
+  rows.clear();
+  rows.reserve(n);
+  for (auto& r : per) rows.insert(rows.end(), r.begin(), r.end());
+}
+
+std::vector<uint8_t> write_v5(const Prep& P, const int64_t* codes, const double* vn,
+                              const V5Header& H, const float* lin3, int n_threads) {
+  const int64_t n_m = (int64_t)P.plans.size();
+  std::vector<int64_t> coff(n_m + 1, 0);
+  for (int64_t m = 0; m < n_m; ++m)
+    coff[m + 1] = coff[m] + 3 * (int64_t)(P.plans[m].l2g.size() - P.plans[m].n_bnd);
+  std::vector<int64_t> rc(coff[n_m]);
+  parallel_for(n_m, n_threads, [&](int64_t m, int) {
+    v5_chain(P, m, codes, vn, H, lin3, rc.data() + coff[m], nullptr);
+  });
+  int64_t max_abs = 0;
+  for (int64_t c : rc) max_abs = std::max(max_abs, c < 0 ? -c : c);
+  std::vector<int32_t> ref(P.n_v, -1);
+  for (size_t i = 0; i < P.boundary.size(); ++i) ref[P.boundary[i]] = (int32_t)i;
+  BitWriter body;
+  std::vector<uint64_t> rel(n_m);
+  for (int64_t m = 0; m < n_m; ++m) {
+    rel[m] = body.bit_pos();
+    v5_meshlet(body, P.plans[m], ref, rc.data() + coff[m]);
+  }
+  std::vector<uint8_t> body_bytes = body.finish();
+
+  BitWriter w;
+  w.put(0x504C4454u, 32);
+  w.put(5, 8);
+  w.put(max_abs > 32767 ? 1 : 0, 8);
+  auto f32 = [&](float f) {
+    uint32_t x;
+    std::memcpy(&x, &f, 4);
+    w.put(x, 32);
+  };
+  for (int d = 0; d < 3; ++d) f32(H.center[d]);
+  f32(H.scale);
+  f32(H.per_coord_err);
+  for (int d = 0; d < 3; ++d) f32(H.g_min[d]);
+  for (int d = 0; d < 3; ++d) f32(H.g_range[d]);
+  for (int d = 0; d < 3; ++d) w.put((uint64_t)H.g_bits[d], 8);
+  const uint64_t n_b = P.boundary.size();
+  w.put((uint64_t)P.n_v, 32);
+  w.put((uint64_t)P.n_t, 32);
+  w.put(n_b, 32);
+  w.put((uint64_t)n_m, 32);
+  for (int j = 0; j < 3; ++j) f32(lin3[j]);
+  if (n_b > 0) {
+    std::vector<uint64_t> u(n_b - 1);
+    for (int d = 0; d < 3; ++d) {
+      w.put((uint64_t)codes[3 * P.boundary[0] + d], H.g_bits[d]);
+      if (n_b > 1) {
+        for (uint64_t i = 0; i + 1 < n_b; ++i)
+          u[i] = zz64(codes[3 * P.boundary[i + 1] + d] - codes[3 * P.boundary[i] + d]);
+        int64_t bits;
+        const int k = best_rice(u.data(), u.size(), 11, bits);
+        w.put((uint64_t)k, 8);
+        for (uint64_t x : u) w.put_rice(x, k);
+      }
+    }
+  }
+  const int pad = (int)((8 - (w.bit_pos() & 7)) & 7);
+  if (pad) w.put(0, pad);
   const uint64_t region = w.bit_pos() + 32ULL * (uint64_t)n_m;
   for (int64_t m = 0; m < n_m; ++m) w.put(region + rel[m], 32);
   std::vector<uint8_t> out = w.finish();

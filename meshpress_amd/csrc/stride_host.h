@@ -43,6 +43,10 @@ struct Prep {
   int64_t n_v = 0, n_t = 0;
   std::vector<Plan> plans;
   std::vector<std::vector<WalkEntry>> walks;
+  // global ids of the vertices shared by >= 2 meshlets, in boundary-table
+  // order (sorted, then stable Morton order); plan boundary locals are
+  // sorted by their position here
+  std::vector<int64_t> boundary;
   std::string error;
 };
 
@@ -134,6 +138,21 @@ int dprb_from_dpr4(const uint8_t* data, size_t len, std::vector<uint8_t>& out, s
                    int& code);
 int dprb_to_dpr4(const uint8_t* blob, size_t len, std::vector<uint8_t>& dpr4, DprbTables& t,
                  std::string& err, int& code);
+
+// PDLT v5 writer (encoder/paradelta_v5.py:144-283 and
+// encoder/paradelta_v5_nb.py:20-539): LIN3 predictor with a float32
+// closed-loop chain, global boundary table, Rice-coded boundary refs.
+struct V5Header {
+  float center[3], scale, per_coord_err, g_min[3], g_range[3];
+  int g_bits[3];
+};
+// Fit samples of the plain-parallelogram pass (w3 == nullptr) as rows of
+// (a, b, c, true) x 3 doubles, or the residual codes (n_int x 3 per
+// meshlet, meshlet order) of the LIN3 pass.
+void v5_samples(const Prep& P, const int64_t* codes, const double* vn, const V5Header& H,
+                std::vector<double>& rows, int n_threads);
+std::vector<uint8_t> write_v5(const Prep& P, const int64_t* codes, const double* vn,
+                              const V5Header& H, const float* lin3, int n_threads);
 
 // tagk: per meshlet 12 bytes: (tag,k) x 3 delta then (tag,k) x 3 para.
 std::vector<uint8_t> write_dup(const Prep& P, const Residuals& R, const Header& H,

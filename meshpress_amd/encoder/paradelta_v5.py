@@ -1,0 +1,46 @@
+"""PDLT v5 writer (reference: encoder/paradelta_v5.py:144-283).
+
+`encode_from_prepared_v5(prep)` keeps the reference's name and prep-dict
+input.  The two float32 chain passes and the bit writer run natively
+(stride_encoder_v5_prepare / stride_encoder_write_v5); the LIN3 weights are
+fitted here with the reference's least-squares expression
+(encoder/paradelta_codec.py:626-639), so the weights match it exactly for
+the same samples.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .prep import header_f32, prepare
+
+MAGIC = 0x504C4454
+VERSION_V5 = 5
+
+
+def fit_linear3(A, B, C, T) -> np.ndarray:
+    """(w_a, w_b, w_c) minimising ||T - (w_a A + w_b B + w_c C)||^2 over all
+    axes, as float32; (1, 1, -1) without samples."""
+    if len(A) == 0:
+        return np.array([1.0, 1.0, -1.0], dtype=np.float32)
+    M = np.stack([np.asarray(A).ravel(), np.asarray(B).ravel(), np.asarray(C).ravel()], axis=1)
+    w, *_ = np.linalg.lstsq(M, np.asarray(T).ravel(), rcond=None)
+    return w.astype(np.float32)
+
+
+def encode_from_prepared_v5(prep: dict, *, verbose: bool = False, return_info: bool = False):
+    """PDLT v5 bytes for a prep dict of encoder.prep.prepare."""
+    nat = prep["native"]
+    n_b, n_s = nat.v5_prepare(prep["vn"], header_f32(prep), prep["g_bits"])
+    lin3 = fit_linear3(*nat.v5_samples())
+    data = nat.write_v5(lin3)
+    if verbose:
+        print(f"  v5 lin3 samples={n_s}  w=({float(lin3[0]):.4f}, {float(lin3[1]):.4f}, "
+              f"{float(lin3[2]):.4f})  n_boundary={n_b:,}  bytes={len(data):,}")
+    if return_info:
+        return data, {"lin3": lin3, "n_boundary": n_b, "n_samples": n_s}
+    return data
+
+
+def encode_v5(verts, tris, **kw) -> bytes:
+    """prepare() + encode_from_prepared_v5 for a vertex / triangle array pair."""
+    return encode_from_prepared_v5(prepare(verts, tris, **kw))

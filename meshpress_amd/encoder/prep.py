@@ -201,6 +201,33 @@ class NativePrep:
                                                        _lib.ptr(off)))
         return cols, off
 
+    def v5_prepare(self, vn, f13, g_bits):
+        """Plain-parallelogram pass of the PDLT v5 writer; (n_boundary, n_samples)."""
+        vn = np.ascontiguousarray(vn, dtype=np.float64)
+        f13 = np.ascontiguousarray(np.asarray(f13, dtype=np.float32).reshape(13))
+        gb = np.ascontiguousarray(np.asarray(g_bits, dtype=np.int32).reshape(3))
+        nb, ns = C.c_int64(), C.c_int64()
+        _lib.check(self._L.stride_encoder_v5_prepare(self._h, _lib.ptr(vn), _lib.ptr(f13),
+                                                     _lib.ptr(gb), C.byref(nb), C.byref(ns)))
+        self._v5_samples = int(ns.value)
+        return int(nb.value), int(ns.value)
+
+    def v5_samples(self):
+        """LIN3 fit samples: a, b, c reconstructions and true positions,
+        float64 (n_samples, 3) each."""
+        out = [np.zeros((self._v5_samples, 3), np.float64) for _ in range(4)]
+        _lib.check(self._L.stride_encoder_v5_samples(self._h, *[_lib.ptr(a) for a in out]))
+        return out
+
+    def write_v5(self, lin3) -> bytes:
+        w = np.ascontiguousarray(np.asarray(lin3, dtype=np.float32).reshape(3))
+        n = C.c_size_t()
+        _lib.check(self._L.stride_encoder_write_v5(self._h, _lib.ptr(w), None, 0, C.byref(n)))
+        out = np.zeros(int(n.value), np.uint8)
+        _lib.check(self._L.stride_encoder_write_v5(self._h, _lib.ptr(w), _lib.ptr(out), len(out),
+                                                   C.byref(n)))
+        return out.tobytes()
+
     def write(self, f13, g_bits, pred_n, pred_K, tagk) -> bytes:
         f13 = np.ascontiguousarray(np.asarray(f13, dtype=np.float32).reshape(13))
         gb = np.ascontiguousarray(np.asarray(g_bits, dtype=np.int32).reshape(3))
@@ -249,7 +276,7 @@ def prepare(verts, tris, *, max_verts=256, max_tris=256, precision_error=0.0005,
         "center": center, "scale": scale, "per_coord_err": per_coord_err,
         "g_min": g_min, "g_range": g_range, "g_bits": g_bits,
         "n_v": n_v, "n_t": n_t, "n_meshlets": native.n_meshlets,
-        "global_codes": codes, "native": native,
+        "global_codes": codes, "vn": vn, "native": native,
         "max_verts": max_verts, "max_tris": max_tris, "precision_error": precision_error,
     }
 

@@ -20,6 +20,12 @@ parts no closed form pins:
     tree);
   * _pick_best_k (paradelta_v5_dup.py:159-171) and encoder_cost_batch
     (_irlp_fit.py:186-226) on seeded random residuals;
+  * decode_paradelta_v5(bytes) -> V float32, T int64 on PDLT v5 streams of
+    this repository's writer (encoder/paradelta_v5.py:286-462; the
+    reference's v5 writer is Numba code and cannot run here), including
+    streams that force each residual coder and explicit partitions with
+    meshlets that have no interior or no boundary vertices
+    (`python tests/golden/make_golden.py pdlt` writes only these);
   * the DPRB v4 side-table blob of scripts/dump_stride_dup_blob.py:88-152,
     built from encode_dup(return_meta=True)'s own per-meshlet meta (anchor
     offsets, n_kind0, substream offsets) with the layout that script packs
@@ -234,9 +240,56 @@ def dprb_goldens():
         print("dprb", name, len(data), len(out), flush=True)
 
 
+def pdlt_cases():
+    """(name, data) PDLT v5 streams of the oracle writer."""
+    from meshpress_amd.encoder.prep import prepare
+    from oracle import pdlt_oracle as PO
+    out = []
+    ms = {n: (v, t) for n, v, t in meshes()}
+    for name in ("uv_12x24", "torus_q16", "hf_24", "one_tri", "uv_c1"):
+        V, T = ms[name]
+        kw = dict(precision_error=1.0 / 65534.5, precision_mode="bbox_frac") \
+            if name == "torus_q16" else {}
+        p = prepare(V, T, **kw)
+        prep = dict(p)
+        prep["plans"] = p["native"].export_plans()
+        out.append((name, PO.encode_v5(prep)))
+        if name in ("hf_24", "torus_q16"):
+            out.append((name + "_tags", PO.encode_v5(prep, force_tags=lambda m: [
+                m % 3, (m + 1) % 3, (m + 2) % 3])))
+    # explicit partitions: a fan whose middle meshlet is all boundary, plus a
+    # separate triangle (a meshlet without boundary vertices)
+    V = np.array([[0.0, 0, 0], [1, 0, 0.1], [1, 1, 0.2], [0, 1, 0.3], [-1, 1, 0.1], [-1, 0, 0.2],
+                  [5, 5, 5], [6, 5, 5.5], [5, 6, 4.5]])
+    T = np.array([[0, 1, 2], [0, 2, 3], [0, 3, 4], [0, 4, 5], [6, 7, 8]])
+    p = prepare(V, T)
+    prep = dict(p)
+    prep["plans"] = PO.plans_from_partition(T, [[0, 1], [2], [3], [4]], p["global_codes"])
+    out.append(("fan_parts", PO.encode_v5(prep)))
+    return out
+
+
+def pdlt_goldens():
+    sys.path.insert(0, str(REF))
+    from encoder.paradelta_v5 import decode_paradelta_v5  # noqa: E402
+    for name, data in pdlt_cases():
+        Vd, Td = decode_paradelta_v5(data)
+        rec = {"data": np.frombuffer(data, np.uint8),
+               "V_sha": np.frombuffer(bytes.fromhex(sha(Vd)), np.uint8),
+               "T_sha": np.frombuffer(bytes.fromhex(sha(Td.astype(np.int64))), np.uint8)}
+        if len(Vd) <= 5000:
+            rec["V"] = Vd
+            rec["T"] = Td.astype(np.int64)
+        np.savez_compressed(HERE / f"pdlt_{name}.npz", **rec)
+        print("pdlt", name, len(data), Vd.shape, Td.shape, flush=True)
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["dprb"]:
         dprb_goldens()
+    elif sys.argv[1:] == ["pdlt"]:
+        pdlt_goldens()
     else:
         main()
         dprb_goldens()
+        pdlt_goldens()
