@@ -62,7 +62,19 @@ int stride_debug_pyset_order(const int32_t* in, int32_t n, int32_t* out);
  * (encoder/paradelta_v5_dup.py:316-464).  Creation uploads the bytes, runs
  * the index pre-pass kernel (the side table the reference bench reads from a
  * DPRB blob, scripts/dump_stride_dup_blob.py:88-152) and validates every
- * meshlet; decode is then one kernel launch. */
+ * meshlet; decode is then one kernel launch.
+ *
+ * A stream starting with the PDLT magic (0x504C4454) is decoded as PDLT v5
+ * (the format the reference class decodes, utils/paradelta_v5_cuda.py:34-577;
+ * CPU decoder encoder/paradelta_v5.py:286-462): creation parses the header
+ * and the boundary table on the host and runs the meshlet pre-pass; decode
+ * is a boundary-vertex kernel plus one wave-group kernel.  Output equals
+ * decode_paradelta_v5 bit for bit (float64 reconstruction): vertices are the
+ * boundary table, then each meshlet's interior vertices; indices are global.
+ * Limits (STRIDE_E_UNSUPPORTED): n_bnd + n_int <= 256 per meshlet, boundary
+ * refs inside the boundary table, residual codes and coder parameters within
+ * 32-bit decoding, g_bits <= 32, streams below 512 MiB.  Tune fields used:
+ * group_size (1..16) and lds_bytes (4096..65536). */
 typedef struct stride_decoder stride_decoder;
 
 int stride_decoder_create(const uint8_t* data, size_t len, stride_decoder** out);
@@ -78,8 +90,8 @@ void stride_decoder_destroy(stride_decoder* h);
 int stride_dprb_from_dpr4(const uint8_t* data, size_t len, uint8_t* out, size_t cap,
                           size_t* out_len);
 int stride_decoder_create_dprb(const uint8_t* blob, size_t len, stride_decoder** out);
-/* n_v = sum of per-meshlet n_local (duplicated boundary vertices included),
- * n_t = triangles, n_meshlets. */
+/* n_v = sum of per-meshlet n_local (duplicated boundary vertices included;
+ * PDLT v5: n_boundary + sum of n_int), n_t = triangles, n_meshlets. */
 int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
                          uint32_t* n_meshlets);
 /* Decoded world-space positions (float32 xyz, n_v x 3) and uint32 indices
@@ -87,6 +99,14 @@ int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
 int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
 /* Synchronous decode into caller-owned host buffers. */
 int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris);
+/* PDLT v5 only: normalised positions (float32 of the float64 reconstruction,
+ * before scale and center), as the reference's decode() returns them
+ * (utils/paradelta_v5_cuda.py:686-712); STRIDE_E_ARG for DPR4. */
+int stride_decode_normalized(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
+/* Stream format: *version = 3 (DPR4 v3) or 5 (PDLT v5); for PDLT v5 also the
+ * LIN3 weights and the boundary-table size (zeros for DPR4). */
+int stride_decoder_format(const stride_decoder* h, int32_t* version, float* lin3,
+                          uint32_t* n_boundary);
 /* Bytes resident on the device: compressed stream, derived side table. */
 int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes,
                          uint64_t* side_bytes);

@@ -27,7 +27,8 @@ ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 HIPCC = str(ROCM / "bin" / "hipcc")
 ARCH = "gfx950"
 
-HIP_SOURCES = ["stride_decode.hip", "stride_search.hip", "stride_capi.hip", "stride_mlp.hip"]
+HIP_SOURCES = ["stride_decode.hip", "stride_search.hip", "stride_capi.hip", "stride_mlp.hip",
+               "pdlt_decode.hip"]
 CXX_SOURCES = ["stride_host.cpp", "stride_dprb.cpp"]
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",

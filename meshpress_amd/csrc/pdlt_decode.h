@@ -1,0 +1,40 @@
+// PDLT v5 decode set: the device side of the gfx950 PDLT v5 decoder
+// (pdlt_decode.hip), used by the C ABI glue (stride_capi.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <vector>
+
+#include "pdlt_host.h"
+
+namespace stride {
+
+struct PdltSet {
+  PdltHost H;
+  std::vector<PdltGroup> groups;
+  uint32_t lds_bytes = 0;               // dynamic LDS of the launch
+  int group_size = 16;
+  int lds_budget = 32768;
+  // device
+  uint32_t* d_words = nullptr;
+  PdltEntry* d_tab = nullptr;
+  PdltGroup* d_groups = nullptr;
+  double* d_bnd = nullptr;              // boundary table, normalised f64 (n_b x 3)
+  float* d_v_own = nullptr;
+  uint32_t* d_t_own = nullptr;
+
+  ~PdltSet();
+  // Parses, validates and uploads.  STRIDE_OK or a STRIDE_E_* status with
+  // the message set.
+  int build(const uint8_t* data, size_t len);
+  int set_layout(int group_size, int lds_budget);
+  // world = 1: world coordinates; 0: normalised.
+  int decode(float* d_verts, uint32_t* d_tris, int world, hipStream_t stream);
+  int own_outputs();
+  size_t side_bytes() const;
+};
+
+}  // namespace stride

@@ -1,0 +1,190 @@
+// gfx950 PDLT v5 decoder: host set-up (header, boundary table, pre-pass,
+// wave groups) and the decode kernels.
+//
+// One wave decodes a group of up to 16 consecutive meshlets whose stream
+// bits are staged in LDS.  Lane l serves meshlet slot l & 15 in role l >> 4:
+//   phase A  role 0 (token lane): boundary refs, strips -> walk, triangles;
+//   phase B  all roles: boundary reconstructions gathered into LDS;
+//   phase C  roles 1..3 (axis lanes): boundary centroid and the float64
+//            LIN3 chain of one axis along the walk;
+//   phase D  all 64 lanes: interior vertices written out.
+// Boundary-table vertices are written by a separate element-wise kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/stride_hip.h"
+#include "pdlt_decode.h"
+#include "stride_err.h"
+
+namespace stride {
+
+// ---------------------------------------------------------------------------
+// Kernels
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void pdlt_boundary_kernel(const double* __restrict__ bnd,
+                                                            uint64_t n3, PdltParams P,
+                                                            float* __restrict__ V) {
+  for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n3;
+       i += (uint64_t)gridDim.x * 256) {
+    V[i] = pdlt_out(bnd[i], P, (int)(i % 3));
+  }
+}
+
+__global__ __launch_bounds__(64) void pdlt_decode_kernel(
+    const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
+    const PdltGroup* __restrict__ groups, const double* __restrict__ bnd, PdltParams P,
+    float* __restrict__ V, uint32_t* __restrict__ T) {
+  extern __shared__ uint64_t lds64[];
+  uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
+  const int lane = threadIdx.x;
+  const int slot = lane & 15, role = lane >> 4;
+  const PdltGroup G = groups[blockIdx.x];
+  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
+  const uint32_t m = G.m0 + (uint32_t)slot;
+  const bool active = m < G.m1;
+  PdltEntry e = {};
+  if (active) e = tab[m];
+  const uint32_t nl = e.n_bnd + e.n_int;
+  uint32_t* blk = lds + G.blk_base + e.blk_off;
+  double* recon = reinterpret_cast<double*>(blk);
+  uint32_t* refs = blk + 6 * nl;
+  uint32_t* walk = refs + e.n_bnd;
+  uint32_t* seen = walk + e.n_int;
+  uint32_t* n_emit = seen + 8;
+  const int64_t rel = -32 * (int64_t)G.w_lo;
+  __syncthreads();
+  if (active && role == 0) pdlt_tokens(lds, rel, e, refs, walk, seen, n_emit, T + 3ull * e.t_off);
+  __syncthreads();
+  // boundary reconstructions: the four lanes of a slot gather its refs
+  if (active) {
+    const uint32_t nb3 = 3 * e.n_bnd;
+    for (uint32_t base = (uint32_t)role; base < nb3; base += 32) {
+      double x[8];
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t idx = base + 4 * u;
+        if (idx < nb3) {
+          const uint32_t i = idx / 3, d = idx - 3 * i;
+          x[u] = bnd[3ull * refs[i] + d];
+        }
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < 8; ++u) {
+        const uint32_t idx = base + 4 * u;
+        if (idx < nb3) {
+          const uint32_t i = idx / 3, d = idx - 3 * i;
+          recon[d * nl + i] = x[u];
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (active && role > 0)
+    pdlt_axis(lds, rel, e, role - 1, P, recon + (uint32_t)(role - 1) * nl, walk, *n_emit);
+  __syncthreads();
+  // interior vertices, one meshlet at a time, 64 coordinates per step
+  for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
+    const PdltEntry ej = tab[G.m0 + j];
+    const uint32_t nlj = ej.n_bnd + ej.n_int;
+    const uint32_t* bj = lds + G.blk_base + ej.blk_off;
+    const double* rj = reinterpret_cast<const double*>(bj);
+    const uint32_t* sj = bj + 6 * nlj + ej.n_bnd + ej.n_int;
+    float* vo = V + 3ull * ej.int_base;
+    const uint32_t n3 = 3 * ej.n_int;
+    for (uint32_t t = lane; t < n3; t += 64) {
+      const uint32_t i = t / 3, d = t - 3 * i, v = ej.n_bnd + i;
+      const bool reached = ((sj[v >> 5] >> (v & 31)) & 1u) != 0;
+      vo[t] = pdlt_out(reached ? rj[d * nlj + v] : 0.0, P, (int)d);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Host side
+// ---------------------------------------------------------------------------
+#define PDLT_HIP(expr)                                                                \
+  do {                                                                                \
+    hipError_t _e = (expr);                                                           \
+    if (_e != hipSuccess)                                                             \
+      return set_error(STRIDE_E_HIP, std::string(#expr ": ") + hipGetErrorString(_e)); \
+  } while (0)
+
+PdltSet::~PdltSet() {
+  (void)hipFree(d_words);
+  (void)hipFree(d_tab);
+  (void)hipFree(d_groups);
+  (void)hipFree(d_bnd);
+  (void)hipFree(d_v_own);
+  (void)hipFree(d_t_own);
+}
+
+size_t PdltSet::side_bytes() const {
+  return H.tab.size() * sizeof(PdltEntry) + groups.size() * sizeof(PdltGroup) +
+         H.bnd.size() * sizeof(double);
+}
+
+int PdltSet::build(const uint8_t* d, size_t len) {
+  std::string err;
+  int rc = pdlt_parse(d, len, H, err);
+  if (rc) return set_error(rc, err);
+  PDLT_HIP(hipMalloc(&d_words, H.words.size() * 4));
+  PDLT_HIP(hipMemcpy(d_words, H.words.data(), H.words.size() * 4, hipMemcpyHostToDevice));
+  if (!H.bnd.empty()) {
+    PDLT_HIP(hipMalloc(&d_bnd, H.bnd.size() * sizeof(double)));
+    PDLT_HIP(hipMemcpy(d_bnd, H.bnd.data(), H.bnd.size() * sizeof(double), hipMemcpyHostToDevice));
+  }
+  if (!H.tab.empty()) PDLT_HIP(hipMalloc(&d_tab, H.tab.size() * sizeof(PdltEntry)));
+  return set_layout(group_size, lds_budget);
+}
+
+int PdltSet::set_layout(int gs, int budget) {
+  std::string err;
+  std::vector<PdltGroup> gr;
+  uint32_t lds = 0;
+  std::vector<PdltEntry> tab = H.tab;
+  int rc = pdlt_layout(tab, gs, budget, gr, lds, err);
+  if (rc) return set_error(rc, err);
+  H.tab.swap(tab);
+  groups.swap(gr);
+  lds_bytes = lds;
+  group_size = gs;
+  lds_budget = budget;
+  (void)hipFree(d_groups);
+  d_groups = nullptr;
+  if (!H.tab.empty())
+    PDLT_HIP(hipMemcpy(d_tab, H.tab.data(), H.tab.size() * sizeof(PdltEntry), hipMemcpyHostToDevice));
+  if (!groups.empty()) {
+    PDLT_HIP(hipMalloc(&d_groups, groups.size() * sizeof(PdltGroup)));
+    PDLT_HIP(hipMemcpy(d_groups, groups.data(), groups.size() * sizeof(PdltGroup), hipMemcpyHostToDevice));
+  }
+  return STRIDE_OK;
+}
+
+int PdltSet::own_outputs() {
+  if (!d_v_own && H.n_v) PDLT_HIP(hipMalloc(&d_v_own, 12 * H.n_v));
+  if (!d_t_own && H.n_t) PDLT_HIP(hipMalloc(&d_t_own, 12 * H.n_t));
+  return STRIDE_OK;
+}
+
+int PdltSet::decode(float* V, uint32_t* T, int world, hipStream_t s) {
+  if ((!V && H.n_v) || (!T && H.n_t)) return set_error(STRIDE_E_ARG, "null output pointer");
+  PdltParams p = H.params;
+  p.world = world ? 1 : 0;
+  if (H.n_b) {
+    const uint64_t n3 = 3ull * H.n_b;
+    const uint32_t grid = (uint32_t)std::min<uint64_t>((n3 + 255) / 256, 8192);
+    hipLaunchKernelGGL(pdlt_boundary_kernel, dim3(grid), dim3(256), 0, s, d_bnd, n3, p, V);
+    PDLT_HIP(hipGetLastError());
+  }
+  if (!groups.empty()) {
+    hipLaunchKernelGGL(pdlt_decode_kernel, dim3((uint32_t)groups.size()), dim3(64), lds_bytes, s,
+                       d_words, d_tab, d_groups, d_bnd, p, V, T);
+    PDLT_HIP(hipGetLastError());
+  }
+  return STRIDE_OK;
+}
+
+}  // namespace stride

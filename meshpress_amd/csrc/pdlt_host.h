@@ -1,0 +1,254 @@
+// Host-side PDLT v5 set-up shared by the decoder (pdlt_decode.hip) and the
+// CPU harness of its kernel logic: global header, boundary table, meshlet
+// pre-pass and the wave-group layout.  Host code only.
+#pragma once
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/stride_hip.h"
+#include "pdlt_lane.h"
+
+namespace stride {
+
+struct PdltHost {
+  // global header (encoder/paradelta_v5.py:290-311)
+  uint32_t code_width = 0;
+  float center[3], scale = 0.f, pce = 0.f, g_min[3], g_range[3], lin3[3];
+  int32_t g_bits[3];
+  uint32_t hdr_n_v = 0, hdr_n_t = 0;
+  uint32_t n_b = 0, n_m = 0;
+  uint64_t n_v = 0, n_t = 0;            // decoded counts
+  size_t stream_bytes = 0;
+  std::vector<uint32_t> words;          // the stream as little-endian words, zero padded
+  std::vector<double> bnd;              // boundary table, normalised f64 (n_b x 3)
+  std::vector<PdltEntry> tab;
+  PdltParams params{};
+};
+
+namespace pdlt_detail {
+
+inline uint32_t be32(const uint8_t* p) {
+  return ((uint32_t)p[0] << 24) | ((uint32_t)p[1] << 16) | ((uint32_t)p[2] << 8) | p[3];
+}
+inline float bef32(const uint8_t* p) {
+  const uint32_t u = be32(p);
+  float f;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+inline int scan_status(uint32_t e, uint32_t m, std::string& err) {
+  const std::string at = " (meshlet " + std::to_string(m) + ")";
+  switch (e) {
+    case kV5ErrSize: err = "meshlet with more than 256 vertices" + at; return STRIDE_E_UNSUPPORTED;
+    case kV5ErrOverrun: err = "meshlet runs past the end of the stream" + at; return STRIDE_E_TRUNCATED;
+    case kV5ErrFifo: err = "reuse FIFO index past the FIFO fill" + at; return STRIDE_E_CORRUPT;
+    case kV5ErrIndex: err = "local vertex id >= n_local" + at; return STRIDE_E_CORRUPT;
+    case kV5ErrStrips: err = "strip lengths do not add up to n_tris" + at; return STRIDE_E_CORRUPT;
+    case kV5ErrTag: err = "bad residual tag" + at; return STRIDE_E_CORRUPT;
+    case kV5ErrRef: err = "boundary ref outside the boundary table" + at; return STRIDE_E_UNSUPPORTED;
+    case kV5ErrRange:
+      err = "residual code or coder parameter outside 32-bit decoding" + at;
+      return STRIDE_E_UNSUPPORTED;
+    default: err = "malformed meshlet" + at; return STRIDE_E_CORRUPT;
+  }
+}
+
+template <class F>
+void par_for(int64_t n, F&& fn) {
+  const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+  if (n < 256 || nt == 1) {
+    for (int64_t i = 0; i < n; ++i) fn(i);
+    return;
+  }
+  std::atomic<int64_t> next{0};
+  std::vector<std::thread> th;
+  for (int t = 0; t < nt; ++t)
+    th.emplace_back([&]() {
+      for (;;) {
+        const int64_t s = next.fetch_add(64);
+        if (s >= n) break;
+        for (int64_t i = s; i < std::min(n, s + 64); ++i) fn(i);
+      }
+    });
+  for (auto& x : th) x.join();
+}
+
+}  // namespace pdlt_detail
+
+// Parses and validates a PDLT v5 stream.  Returns STRIDE_OK or a STRIDE_E_*
+// status with the message in err.  Accepted streams decode with
+// decode_paradelta_v5 (encoder/paradelta_v5.py:286-462) to the same output
+// the kernels produce; the meshlet offsets must match the sequential layout
+// that decoder reads.
+inline int pdlt_parse(const uint8_t* d, size_t len, PdltHost& H, std::string& err) {
+  using namespace pdlt_detail;
+  if (len < 5) { err = "stream shorter than the PDLT v5 header"; return STRIDE_E_TRUNCATED; }
+  const uint32_t magic = be32(d);
+  if (magic != kPdltMagic) {
+    char b[64];
+    std::snprintf(b, sizeof b, "bad magic 0x%08X", magic);
+    err = b;
+    return STRIDE_E_MAGIC;
+  }
+  if (d[4] != kPdltVersion) {
+    char b[64];
+    std::snprintf(b, sizeof b, "expected v5, got v%u", (unsigned)d[4]);
+    err = b;
+    return STRIDE_E_VERSION;
+  }
+  if (len < kPdltHeaderBytes) { err = "stream shorter than the PDLT v5 header"; return STRIDE_E_TRUNCATED; }
+  if (len > 0x1FFFFF00ull) { err = "streams above 512 MiB are not supported"; return STRIDE_E_UNSUPPORTED; }
+  H.stream_bytes = len;
+  H.code_width = d[5];
+  for (int i = 0; i < 3; ++i) H.center[i] = bef32(d + 6 + 4 * i);
+  H.scale = bef32(d + 18);
+  H.pce = bef32(d + 22);
+  for (int i = 0; i < 3; ++i) H.g_min[i] = bef32(d + 26 + 4 * i);
+  for (int i = 0; i < 3; ++i) H.g_range[i] = bef32(d + 38 + 4 * i);
+  for (int i = 0; i < 3; ++i) H.g_bits[i] = d[50 + i];
+  H.hdr_n_v = be32(d + 53);
+  H.hdr_n_t = be32(d + 57);
+  H.n_b = be32(d + 61);
+  H.n_m = be32(d + 65);
+  for (int i = 0; i < 3; ++i) H.lin3[i] = bef32(d + 69 + 4 * i);
+  for (int i = 0; i < 3; ++i)
+    if (H.g_bits[i] > 32) { err = "g_bits above 32 are not supported"; return STRIDE_E_UNSUPPORTED; }
+  const uint64_t end_all = 8ull * len;
+  const uint32_t n_b = H.n_b, n_m = H.n_m;
+  // every boundary difference takes at least one bit, every offset 32
+  if ((uint64_t)n_b > end_all + 1 || 32ull * n_m > end_all) {
+    err = "stream shorter than its boundary and offset tables";
+    return STRIDE_E_TRUNCATED;
+  }
+  const size_t nw = (len + 3) / 4 + 4;
+  H.words.assign(nw, 0);
+  std::memcpy(H.words.data(), d, len);
+  GlobalWords src{H.words.data(), (uint32_t)nw};
+
+  // boundary table (encoder/paradelta_v5.py:313-332)
+  H.bnd.assign(3ull * n_b, 0.0);
+  BitReader<GlobalWords> r;
+  r.init(src, 8 * kPdltHeaderBytes);
+  {
+    std::vector<int64_t> code(n_b);
+    for (int a = 0; a < 3 && n_b > 0; ++a) {
+      code[0] = r.bits(H.g_bits[a]);
+      if (n_b > 1) {
+        const uint32_t k = r.bits(8);
+        if (k > 32) { err = "boundary coder parameter k > 32"; return STRIDE_E_UNSUPPORTED; }
+        for (uint32_t i = 1; i < n_b; ++i) {
+          uint64_t q;
+          const uint32_t e = v5_scan_unary(r, end_all, k >= 32 ? 1ull : (1ull << (62 - k)), q);
+          if (e == kV5ErrOverrun || r.pos > end_all) {
+            err = "boundary table runs past the end of the stream";
+            return STRIDE_E_TRUNCATED;
+          }
+          if (e) { err = "boundary table value outside 64-bit decoding"; return STRIDE_E_UNSUPPORTED; }
+          const uint64_t u = (q << k) | r.bits((int)k);
+          code[i] = code[i - 1] + (int64_t)((u >> 1) ^ (0ull - (u & 1)));
+        }
+      }
+      if (r.pos > end_all) { err = "boundary table runs past the end of the stream"; return STRIDE_E_TRUNCATED; }
+      const uint64_t mx = (H.g_bits[a] ? (1ull << H.g_bits[a]) : 1ull) - 1;
+      // g_min + code / mx * g_range in f64 (paradelta_v5.py:322-332)
+      for (uint32_t i = 0; i < n_b; ++i)
+        H.bnd[3ull * i + a] = mx == 0 ? (double)H.g_min[a]
+                                      : (double)H.g_min[a] + (double)code[i] / (double)mx * (double)H.g_range[a];
+    }
+  }
+  const uint64_t pos = (r.pos + 7) & ~7ull;
+  if (pos + 32ull * n_m > end_all) {
+    err = "stream shorter than its meshlet offset table";
+    return STRIDE_E_TRUNCATED;
+  }
+  std::vector<uint32_t> off(n_m);
+  for (uint32_t m = 0; m < n_m; ++m) off[m] = be32(d + pos / 8 + 4ull * m);
+  const uint64_t region = pos + 32ull * n_m;
+  if (n_m > 0 && off[0] != region) {
+    err = "meshlet offsets disagree with the meshlet layout";
+    return STRIDE_E_CORRUPT;
+  }
+
+  // pre-pass (the parse of paradelta_v5.py:345-446)
+  H.tab.assign(n_m, PdltEntry{});
+  std::vector<uint32_t> status(n_m, 0);
+  par_for(n_m, [&](int64_t m) {
+    if (off[m] >= end_all) {
+      status[m] = kV5ErrOverrun;
+      return;
+    }
+    status[m] = pdlt_scan(src, off[m], end_all, n_b, H.tab[m]);
+  });
+  uint64_t iv = n_b, tv = 0;
+  for (uint32_t m = 0; m < n_m; ++m) {
+    if (status[m]) return scan_status(status[m], m, err);
+    if (m + 1 < n_m && H.tab[m].end_bit != off[m + 1]) {
+      err = "meshlet offsets disagree with the meshlet layout";
+      return STRIDE_E_CORRUPT;
+    }
+    H.tab[m].int_base = (uint32_t)iv;
+    H.tab[m].t_off = (uint32_t)tv;
+    iv += H.tab[m].n_int;
+    tv += H.tab[m].n_tris;
+    if (iv > 0xFFFFFFFFull || 3 * tv > 0xFFFFFFFFull) {
+      err = "more than 2^32 output values";
+      return STRIDE_E_UNSUPPORTED;
+    }
+  }
+  H.n_v = iv;
+  H.n_t = tv;
+  PdltParams& P = H.params;
+  for (int i = 0; i < 3; ++i) P.w[i] = (double)H.lin3[i];
+  P.delta = 2.0 * (double)H.pce;
+  for (int i = 0; i < 3; ++i) P.center[i] = (double)H.center[i];
+  P.scale = (double)H.scale;
+  P.world = 1;
+  return STRIDE_OK;
+}
+
+// Greedy wave groups: up to gs consecutive meshlets whose staged words and
+// LDS blocks fit `budget` bytes (a meshlet alone always forms a group).
+// Sets each entry's blk_off; lds = the largest group's LDS bytes.
+inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vector<PdltGroup>& groups,
+                       uint32_t& lds, std::string& err) {
+  if (gs < 1 || gs > 16) { err = "group_size must be in [1, 16]"; return STRIDE_E_ARG; }
+  if (budget < 4096 || budget > 65536) { err = "lds_bytes must be in [4096, 65536]"; return STRIDE_E_ARG; }
+  groups.clear();
+  uint32_t max_lds = 0;
+  const uint32_t n_m = (uint32_t)tab.size();
+  uint32_t m = 0;
+  while (m < n_m) {
+    PdltGroup g{};
+    g.m0 = m;
+    g.w_lo = tab[m].start_bit >> 5;
+    uint32_t blk = 0, words = 0, j = m;
+    while (j < n_m && (int)(j - m) < gs) {
+      // the words holding [start, end) plus one word read ahead, even count
+      const uint32_t w = ((((tab[j].end_bit - 1) >> 5) + 2 - g.w_lo) + 1) & ~1u;
+      const uint32_t b = pdlt_block_words(tab[j].n_bnd, tab[j].n_int);
+      if (j > m && 4ull * (w + blk + b) > (uint64_t)budget) break;
+      tab[j].blk_off = blk;
+      blk += b;
+      words = w;
+      ++j;
+    }
+    g.m1 = j;
+    g.n_words = words;
+    g.blk_base = words;
+    max_lds = std::max(max_lds, 4 * (words + blk));
+    groups.push_back(g);
+    m = j;
+  }
+  if (max_lds > 65536) { err = "a meshlet needs more than 64 KiB of LDS"; return STRIDE_E_UNSUPPORTED; }
+  lds = max_lds;
+  return STRIDE_OK;
+}
+
+}  // namespace stride

@@ -1,0 +1,443 @@
+// PDLT v5 (ParaDelta v5) decode logic shared by the gfx950 kernels
+// (pdlt_decode.hip) and the CPU harness that checks it against the oracle
+// (compiled as host C++ there).
+//
+// Stream layout (encoder/paradelta_v5.py:144-283, paradelta_v5_nb.py:328-539):
+//   global header (81 bytes, big endian), boundary table (per axis: first
+//   code, then Rice-coded zigzag differences), pad to a byte, n_meshlets
+//   absolute bit offsets (u32), then the meshlets, bit packed back to back:
+//     n_bnd, n_int, n_tris, n_strips     4 x u16
+//     boundary refs                       u32 first, then u8 k and
+//                                         Rice(k) of (ref[i+1] - ref[i] - 1)
+//     strips                              u16 length, 3 root tokens, then
+//                                         (length - 1) x (edge bit, token)
+//     residual codes (if n_int > 0)       per axis: u8 tag; 0: i16 min, u8
+//                                         width, n_int fixed-width values;
+//                                         1 / 2: u8 k, n_int Rice / Exp-Golomb
+//                                         zigzag values
+// Tokens are those of the DPR4 stream: '0' + 5-bit reuse-FIFO index (oldest
+// first) or '1' + raw local id of idx_bits(n_local) bits.
+//
+// Decode (encoder/paradelta_v5.py:286-462): boundary locals take the
+// dequantised boundary-table position of their ref; each interior local is
+// predicted, in first-use order along the strips, from the boundary
+// centroid (strip-root vertices) or from LIN3 weights on the parallelogram
+// (a, b, c) of the strip step, and corrected by code * 2 * per_coord_err,
+// all in float64.
+#pragma once
+
+#include <cstdint>
+
+#include "stride_bits.h"
+#include "stride_lane.h"
+
+namespace stride {
+
+constexpr uint32_t kPdltMagic = 0x504C4454u;   // 'PLDT' as written (MAGIC)
+constexpr uint32_t kPdltVersion = 5u;
+constexpr uint32_t kPdltHeaderBytes = 81;
+
+// Per-meshlet side table from the pre-pass.  Bit offsets are absolute in
+// the stream.
+struct PdltEntry {
+  uint32_t start_bit;    // meshlet header
+  uint32_t strips_bit;   // first strip length
+  uint32_t axis_bit[3];  // residual tag byte of each axis (0 when n_int == 0)
+  uint32_t end_bit;      // one past the last bit
+  uint32_t n_bnd, n_int, n_tris;
+  uint32_t n_strips;     // | kFlagRawInFifo
+  uint32_t int_base;     // output index of the first interior vertex
+  uint32_t t_off;        // first output triangle
+  uint32_t blk_off;      // LDS block, 32-bit words from the group's block base
+  uint32_t pad[3];
+};
+static_assert(sizeof(PdltEntry) == 64, "PdltEntry is one 64-byte line");
+
+// Consecutive meshlets decoded by one wave.
+struct PdltGroup {
+  uint32_t m0, m1;       // meshlet range [m0, m1)
+  uint32_t w_lo;         // first staged stream word
+  uint32_t n_words;      // staged words (even)
+  uint32_t blk_base;     // LDS word offset of the meshlet blocks (even)
+  uint32_t pad[3];
+};
+static_assert(sizeof(PdltGroup) == 32, "PdltGroup is 32 bytes");
+
+// Decode constants, widened to float64 as the reference decoder does.
+struct PdltParams {
+  double w[3];           // LIN3 weights
+  double delta;          // 2 * per_coord_err
+  double center[3];
+  double scale;
+  int32_t world;         // 1: world coordinates, 0: normalised
+  int32_t pad[3];
+};
+
+enum : uint32_t {
+  kV5Ok = 0,
+  kV5ErrSize = 1,      // n_bnd + n_int > 256
+  kV5ErrOverrun = 2,   // a field runs past the end of the stream
+  kV5ErrFifo = 3,      // reuse-FIFO index past the FIFO fill
+  kV5ErrIndex = 4,     // raw local id >= n_local
+  kV5ErrStrips = 5,    // strip lengths do not add up to n_tris
+  kV5ErrTag = 6,       // residual tag not 0, 1 or 2
+  kV5ErrRef = 7,       // boundary ref outside the boundary table
+  kV5ErrRange = 8,     // a code or coder parameter outside 32-bit decoding
+};
+
+// LDS block of one meshlet, in 32-bit words: reconstruction (3 axes x
+// n_local float64, axis major), refs (n_bnd), walk (n_int), the seen set
+// (8 words) and the walk length (1 word); even, so blocks stay 8-byte
+// aligned.
+STRIDE_DEV uint32_t pdlt_block_words(uint32_t n_bnd, uint32_t n_int) {
+  const uint32_t n_local = n_bnd + n_int;
+  return (6 * n_local + n_bnd + n_int + 9 + 1) & ~1u;
+}
+
+STRIDE_DEV uint32_t pdlt_idx_bits(uint32_t n_local) {
+  uint32_t b = 1;
+  while ((1u << b) < n_local + 1) ++b;
+  return b;
+}
+
+// Unary run at r of any length, the terminating 1 consumed: kV5ErrOverrun
+// when the stream ends first, kV5ErrRange when the run reaches `bound`.
+template <class Src>
+STRIDE_DEV uint32_t v5_scan_unary(BitReader<Src>& r, uint64_t end_all, uint64_t bound, uint64_t& q) {
+  q = 0;
+  for (;;) {
+    r.refill();
+    const int lz = r.win ? __builtin_clzll(r.win) : 64;
+    if (lz < r.valid) {
+      r.win = (r.win << lz) << 1;
+      r.valid -= lz + 1;
+      r.pos += (uint32_t)lz + 1u;
+      q += (uint64_t)lz;
+      break;
+    }
+    q += (uint64_t)r.valid;
+    r.pos += (uint32_t)r.valid;
+    r.win = 0;
+    r.valid = 0;
+    if (r.pos > end_all) return kV5ErrOverrun;
+    if (q >= bound) return kV5ErrRange;
+  }
+  if (r.pos > end_all) return kV5ErrOverrun;
+  return q >= bound ? kV5ErrRange : kV5Ok;
+}
+
+// ---------------------------------------------------------------------------
+// Pre-pass: one meshlet from start_bit, every field bounds checked.  Fills
+// counts and section offsets; returns kV5Ok or an error code.  Accepts
+// exactly the meshlets decode_paradelta_v5 decodes, minus the documented
+// limits (n_local <= 256, refs inside the boundary table, codes and coder
+// parameters within 32-bit decoding).
+// ---------------------------------------------------------------------------
+template <class Src>
+STRIDE_DEV uint32_t pdlt_scan(const Src& src, uint32_t start, uint64_t end_all,
+                              uint64_t n_boundary, PdltEntry& e) {
+  BitReader<Src> r;
+  r.init(src, start);
+  e.start_bit = start;
+  e.n_bnd = r.bits(16);
+  e.n_int = r.bits(16);
+  e.n_tris = r.bits(16);
+  const uint32_t n_strips = r.bits(16);
+  e.n_strips = n_strips;
+  e.axis_bit[0] = e.axis_bit[1] = e.axis_bit[2] = 0;
+  if (r.pos > end_all) return kV5ErrOverrun;
+  const uint32_t n_local = e.n_bnd + e.n_int;
+  if (n_local > 256) return kV5ErrSize;
+  if (e.n_bnd > 0) {
+    uint64_t prev = r.bits(32);
+    if (prev >= n_boundary) return kV5ErrRef;
+    if (e.n_bnd > 1) {
+      const uint32_t k = r.bits(8);
+      if (k > 31) return kV5ErrRange;
+      for (uint32_t i = 1; i < e.n_bnd; ++i) {
+        uint64_t q;
+        const uint32_t err = v5_scan_unary(r, end_all, 1ull << (32 - k), q);
+        if (err) return err;
+        const uint64_t u = (q << k) | r.bits((int)k);
+        prev = prev + u + 1;
+        if (prev >= n_boundary) return kV5ErrRef;
+      }
+    }
+  }
+  if (r.pos > end_all) return kV5ErrOverrun;
+  e.strips_bit = r.pos;
+  const uint32_t ib = pdlt_idx_bits(n_local);
+  MtfLite fifo;
+  fifo.clear();
+  bool raw_in_fifo = false;
+  uint64_t tris = 0;
+  for (uint32_t s = 0; s < n_strips; ++s) {
+    const uint32_t L = r.bits(16);
+    const uint32_t steps = L ? L - 1 : 0;
+    tris += steps + 1;
+    for (uint32_t t = 0; t < 3 + steps; ++t) {
+      if (t >= 3) r.bits(1);
+      uint32_t j, v;
+      if (r.bits(1) == 0) {
+        const uint32_t idx = r.bits(5);
+        if (idx >= fifo.n) return kV5ErrFifo;
+        j = fifo.n - 1 - idx;
+        v = fifo.rank(j);
+      } else {
+        v = r.bits((int)ib);
+        if (v >= n_local) return kV5ErrIndex;
+        const int pf = fifo.find(v);
+        raw_in_fifo |= pf >= 0;
+        j = pf >= 0 ? (uint32_t)pf : fifo.n;
+        if (j == fifo.n && fifo.n < 16) fifo.n++;
+      }
+      fifo.to_front(j, v);
+    }
+    if (r.pos > end_all) return kV5ErrOverrun;
+  }
+  if (tris != e.n_tris) return kV5ErrStrips;
+  if (raw_in_fifo) e.n_strips |= kFlagRawInFifo;
+  if (e.n_int > 0) {
+    for (int d = 0; d < 3; ++d) {
+      e.axis_bit[d] = r.pos;
+      const uint32_t tag = r.bits(8);
+      if (tag == 0) {
+        const uint32_t mu = r.bits(16);
+        const int64_t mn = (mu & 0x8000u) ? (int64_t)mu - 0x10000 : (int64_t)mu;
+        const uint32_t bw = r.bits(8);
+        if (bw > 32) return kV5ErrRange;
+        if (mn + (int64_t)((1ull << bw) - 1) > 0x7FFFFFFFll) {
+          for (uint32_t i = 0; i < e.n_int; ++i)
+            if (mn + (int64_t)r.bits((int)bw) > 0x7FFFFFFFll) return kV5ErrRange;
+        } else {
+          const uint64_t skip = r.pos + (uint64_t)bw * e.n_int;
+          if (skip > end_all) return kV5ErrOverrun;
+          r.init(r.src, (uint32_t)skip);
+        }
+      } else if (tag == 1 || tag == 2) {
+        const uint32_t k = r.bits(8);
+        if (k > 31) return kV5ErrRange;
+        for (uint32_t i = 0; i < e.n_int; ++i) {
+          uint64_t q;
+          const uint32_t err = v5_scan_unary(r, end_all, tag == 1 ? (1ull << (32 - k)) : 32ull, q);
+          if (err) return err;
+          if (tag == 1) {
+            r.bits((int)k);
+          } else {
+            const uint64_t shifted = ((1ull << q) - 1) + r.bits((int)q);
+            const uint64_t u = (shifted << k) | r.bits((int)k);
+            if (u >> 32) return kV5ErrRange;
+          }
+        }
+      } else {
+        return kV5ErrTag;
+      }
+      if (r.pos > end_all) return kV5ErrOverrun;
+    }
+  }
+  e.end_bit = r.pos;
+  if (r.pos > end_all) return kV5ErrOverrun;
+  return kV5Ok;
+}
+
+// ---------------------------------------------------------------------------
+// Lane code over the staged stream (big-endian words in LDS; `rel` maps a
+// stream bit to a staged bit: staged = stream - 32 * w_lo).
+// ---------------------------------------------------------------------------
+
+// n <= 32 bits at p
+STRIDE_DEV uint32_t v5_bits(const uint32_t* __restrict__ st, uint32_t p, uint32_t n) {
+  return n ? (uint32_t)(win64(st, p) >> (64 - n)) : 0u;
+}
+
+// unary prefix at p (runs of any length), p moves past the terminating 1
+STRIDE_DEV uint32_t v5_unary(const uint32_t* __restrict__ st, uint32_t& p) {
+  uint32_t q = 0;
+  uint64_t W = win64(st, p);
+  while ((W >> 32) == 0) {
+    q += 32;
+    p += 32;
+    W = win64(st, p);
+  }
+  const uint32_t z = (uint32_t)__builtin_clzll(W);
+  p += z + 1;
+  return q + z;
+}
+
+// Rice(k) / Exp-Golomb(k) value (the pre-pass bounds it below 2^32)
+STRIDE_DEV uint32_t v5_rice(const uint32_t* __restrict__ st, uint32_t& p, uint32_t k) {
+  const uint32_t q = v5_unary(st, p);
+  const uint32_t low = v5_bits(st, p, k);
+  p += k;
+  return (q << k) | low;
+}
+STRIDE_DEV uint32_t v5_eg(const uint32_t* __restrict__ st, uint32_t& p, uint32_t k) {
+  const uint32_t lb = v5_unary(st, p);
+  const uint32_t off = v5_bits(st, p, lb);
+  p += lb;
+  const uint32_t shifted = ((lb ? (1u << lb) : 1u) - 1u) + off;
+  const uint32_t low = v5_bits(st, p, k);
+  p += k;
+  return (shifted << k) | low;
+}
+
+// Token lane: boundary refs into refs[], strips into the walk (interior
+// locals in first-use order: v * 0x01010101 for a strip-root vertex,
+// v | a << 8 | b << 16 | c << 24 for a parallelogram step, a != v) and the
+// triangles into tout (global ids).  Stores the seen set (8 words) and the
+// walk length.
+STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e,
+                            uint32_t* __restrict__ refs, uint32_t* __restrict__ walk,
+                            uint32_t* __restrict__ seen_out, uint32_t* __restrict__ n_emit,
+                            uint32_t* __restrict__ tout) {
+  const uint32_t n_bnd = e.n_bnd, n_local = e.n_bnd + e.n_int;
+  uint32_t p = (uint32_t)((int64_t)e.start_bit + rel) + 64u;
+  if (n_bnd > 0) {
+    uint32_t prev = v5_bits(st, p, 32);
+    p += 32;
+    refs[0] = prev;
+    if (n_bnd > 1) {
+      const uint32_t k = v5_bits(st, p, 8);
+      p += 8;
+      for (uint32_t i = 1; i < n_bnd; ++i) {
+        prev = prev + v5_rice(st, p, k) + 1u;
+        refs[i] = prev;
+      }
+    }
+  }
+  const uint32_t n_tris = e.n_tris, n_strips = e.n_strips & kStripMask;
+  const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
+  uint32_t pc = (uint32_t)((int64_t)e.strips_bit + rel);
+  const uint32_t ib = pdlt_idx_bits(n_local);
+  MtfLite fifo;
+  fifo.clear();
+  // seen set, boundary locals preset
+  uint64_t s4[4];
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const uint32_t lo = 64u * (uint32_t)q;
+    s4[q] = n_bnd >= lo + 64 ? ~0ull : (n_bnd <= lo ? 0ull : ((1ull << (n_bnd - lo)) - 1));
+  }
+  uint32_t nw = 0, q0 = 0, q1 = 0, q2 = 0;
+  uint32_t root_j = 3, steps_left = 0;
+  const uint32_t ibase = e.int_base - n_bnd;
+  auto gid = [&](uint32_t x) { return x < n_bnd ? refs[x] : ibase + x; };
+  const uint32_t n_tok = n_tris + 2 * n_strips;
+  uint32_t* o = tout;
+  for (uint32_t it = 0; it < n_tok; ++it) {
+    uint32_t W = win32(st, pc);
+    const bool start = root_j == 3 && steps_left == 0;
+    const uint32_t L = W >> 16;
+    steps_left = start ? (L ? L - 1 : 0u) : steps_left;
+    W = start ? (W << 16) : W;
+    root_j = start ? 0u : root_j;
+    const bool is_root = root_j < 3;
+    const uint32_t ec = is_root ? 0u : (W >> 31);
+    W = is_root ? W : (W << 1);
+    const uint32_t flag = W >> 31;
+    const uint32_t val = flag ? ((W << 1) >> (32 - ib)) : ((W >> 26) & 31u);
+    pc += (start ? 16u : 0u) + (is_root ? 1u : 2u) + (flag ? ib : 5u);
+    const bool hit = flag == 0;
+    uint32_t j = hit ? ((fifo.n - 1 - val) & 15u) : fifo.n;
+    const uint32_t v = hit ? fifo.rank(j) : (val & 0xFFu);
+    if (raw_in_fifo && !hit) {
+      const int pf = fifo.find(v);
+      j = pf >= 0 ? (uint32_t)pf : j;
+    }
+    fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
+    fifo.to_front(j, v);
+    const uint32_t qw = (v >> 6) & 3u;
+    const uint64_t bit = 1ull << (v & 63u);
+    const uint64_t sw = qw == 0 ? s4[0] : qw == 1 ? s4[1] : qw == 2 ? s4[2] : s4[3];
+    const bool fresh = (sw & bit) == 0;
+    s4[0] |= qw == 0 ? bit : 0ull;
+    s4[1] |= qw == 1 ? bit : 0ull;
+    s4[2] |= qw == 2 ? bit : 0ull;
+    s4[3] |= qw == 3 ? bit : 0ull;
+    const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
+    const uint32_t entry = is_root ? v * 0x01010101u : (v | (ra << 8) | (q2 << 16) | (rc << 24));
+    if (fresh) walk[nw & 255u] = entry;
+    nw += fresh ? 1u : 0u;
+    const bool emit = !is_root || root_j == 2;
+    const uint32_t ta = is_root ? q0 : ra;
+    const uint32_t tb = is_root ? q1 : q2;
+    if (emit) {
+      o[0] = gid(ta);
+      o[1] = gid(tb);
+      o[2] = gid(v);
+      o += 3;
+    }
+    q0 = is_root ? (root_j == 0 ? v : q0) : ra;
+    q1 = is_root ? (root_j == 1 ? v : q1) : q2;
+    q2 = is_root ? (root_j == 2 ? v : q2) : v;
+    root_j += is_root ? 1u : 0u;
+    steps_left -= is_root ? 0u : 1u;
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    seen_out[2 * q] = (uint32_t)s4[q];
+    seen_out[2 * q + 1] = (uint32_t)(s4[q] >> 32);
+  }
+  *n_emit = nw;
+}
+
+// Axis lane d: boundary centroid of axis d (the boundary rows summed in
+// order, then divided by n_bnd), then the residual codes of axis d along
+// the walk.  rc = the axis-d reconstruction array of the meshlet, whose
+// boundary entries are filled.
+STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e, int d,
+                          const PdltParams& P, double* __restrict__ rc,
+                          const uint32_t* __restrict__ walk, uint32_t n_emit) {
+  if (n_emit == 0) return;   // no residual section is read (n_int may be 0)
+  const uint32_t n_bnd = e.n_bnd;
+  double s = 0.0;
+  for (uint32_t i = 0; i < n_bnd; ++i) s = s + rc[i];
+  const double fb = n_bnd ? s / (double)n_bnd : 0.0;
+  const uint32_t ab = d == 0 ? e.axis_bit[0] : (d == 1 ? e.axis_bit[1] : e.axis_bit[2]);
+  uint32_t p = (uint32_t)((int64_t)ab + rel);
+  const uint64_t H = win64(st, p);
+  const uint32_t tag = (uint32_t)(H >> 56);
+  int32_t mn = 0;
+  uint32_t bw = 0, k = 0;
+  if (tag == 0) {
+    mn = (int32_t)(int16_t)(uint16_t)(H >> 40);
+    bw = (uint32_t)(H >> 32) & 0xFFu;
+    p += 32;
+  } else {
+    k = (uint32_t)(H >> 48) & 0xFFu;
+    p += 16;
+  }
+  const double w0 = P.w[0], w1 = P.w[1], w2 = P.w[2], delta = P.delta;
+  for (uint32_t i = 0; i < n_emit; ++i) {
+    int32_t code;
+    if (tag == 0) {
+      code = (int32_t)((int64_t)mn + (int64_t)v5_bits(st, p, bw));
+      p += bw;
+    } else {
+      const uint32_t u = tag == 1 ? v5_rice(st, p, k) : v5_eg(st, p, k);
+      code = (int32_t)((u >> 1) ^ (0u - (u & 1u)));
+    }
+    const uint32_t ent = walk[i];
+    const uint32_t v = ent & 0xFFu, a = (ent >> 8) & 0xFFu;
+    double pred;
+    if (a == v) {
+      pred = fb;
+    } else {
+      const uint32_t b = (ent >> 16) & 0xFFu, c = ent >> 24;
+      const double t0 = w0 * rc[a];
+      const double t1 = w1 * rc[b];
+      const double t2 = w2 * rc[c];
+      pred = (t0 + t1) + t2;
+    }
+    rc[v] = pred + (double)code * delta;
+  }
+}
+
+// Output value of local v on axis d (interior locals never reached by a
+// strip keep position 0, as in the reference decoder).
+STRIDE_DEV float pdlt_out(double x, const PdltParams& P, int d) {
+  const double c = d == 0 ? P.center[0] : (d == 1 ? P.center[1] : P.center[2]);
+  return P.world ? (float)(x * P.scale + c) : (float)x;
+}
+
+}  // namespace stride

@@ -13,6 +13,7 @@
 #include <vector>
 
 #include "../../include/stride_hip.h"
+#include "pdlt_decode.h"
 #include "stride_decode.h"
 #include "stride_device.h"
 #include "stride_err.h"
@@ -587,6 +588,7 @@ struct DecodeSet {
 
 struct stride_decoder {
   DecodeSet set;
+  std::unique_ptr<PdltSet> v5;   // set for PDLT v5 streams
 };
 struct stride_batch {
   DecodeSet set;
@@ -640,6 +642,13 @@ int stride_decoder_create(const uint8_t* data, size_t len, stride_decoder** out)
   *out = nullptr;
   if (!data && len) return fail(STRIDE_E_ARG, "null data");
   auto h = std::make_unique<stride_decoder>();
+  if (len >= 4 && be32(data) == kPdltMagic) {
+    h->v5 = std::make_unique<PdltSet>();
+    int rc = h->v5->build(data, len);
+    if (rc) return rc;
+    *out = h.release();
+    return STRIDE_OK;
+  }
   const uint8_t* blobs[1] = {data};
   size_t lens[1] = {len};
   int rc = h->set.build(blobs, lens, 1);
@@ -700,6 +709,12 @@ int stride_decoder_create_dprb(const uint8_t* blob, size_t len, stride_decoder**
 int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
                          uint32_t* n_meshlets) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (h->v5) {
+    if (n_v) *n_v = h->v5->H.n_v;
+    if (n_t) *n_t = h->v5->H.n_t;
+    if (n_meshlets) *n_meshlets = h->v5->H.n_m;
+    return STRIDE_OK;
+  }
   if (n_v) *n_v = h->set.n_v;
   if (n_t) *n_t = h->set.n_t;
   if (n_meshlets) *n_meshlets = h->set.n_m;
@@ -708,11 +723,40 @@ int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
 
 int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (h->v5) return h->v5->decode(d_verts, d_tris, 1, (hipStream_t)stream);
   return h->set.decode(d_verts, d_tris, (hipStream_t)stream);
 }
 
+int stride_decode_normalized(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (!h->v5) return fail(STRIDE_E_ARG, "normalised output is defined for PDLT v5 streams only");
+  return h->v5->decode(d_verts, d_tris, 0, (hipStream_t)stream);
+}
+
+int stride_decoder_format(const stride_decoder* h, int32_t* version, float* lin3,
+                          uint32_t* n_boundary) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (version) *version = h->v5 ? 5 : 3;
+  if (lin3)
+    for (int i = 0; i < 3; ++i) lin3[i] = h->v5 ? h->v5->H.lin3[i] : 0.f;
+  if (n_boundary) *n_boundary = h->v5 ? h->v5->H.n_b : 0u;
+  return STRIDE_OK;
+}
+
 int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris) {
-  if (!h || (!verts && h->set.n_v) || (!tris && h->set.n_t))
+  if (!h) return fail(STRIDE_E_ARG, "null argument");
+  if (h->v5) {
+    PdltSet& p = *h->v5;
+    if ((!verts && p.H.n_v) || (!tris && p.H.n_t)) return fail(STRIDE_E_ARG, "null argument");
+    int rc = p.own_outputs();
+    if (rc) return rc;
+    rc = p.decode(p.d_v_own, p.d_t_own, 1, nullptr);
+    if (rc) return rc;
+    if (p.H.n_v) HIP_TRY(hipMemcpy(verts, p.d_v_own, 12 * p.H.n_v, hipMemcpyDeviceToHost));
+    if (p.H.n_t) HIP_TRY(hipMemcpy(tris, p.d_t_own, 12 * p.H.n_t, hipMemcpyDeviceToHost));
+    return STRIDE_OK;
+  }
+  if ((!verts && h->set.n_v) || (!tris && h->set.n_t))
     return fail(STRIDE_E_ARG, "null argument");
   DecodeSet& s = h->set;
   int rc = s.decode(s.d_v_own, s.d_t_own, nullptr);
@@ -724,6 +768,11 @@ int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris) {
 
 int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes, uint64_t* side_bytes) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (h->v5) {
+    if (stream_bytes) *stream_bytes = h->v5->H.stream_bytes;
+    if (side_bytes) *side_bytes = h->v5->side_bytes();
+    return STRIDE_OK;
+  }
   if (stream_bytes) *stream_bytes = h->set.stream_bytes;
   if (side_bytes)
     *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) + sizeof(MeshParams) +
@@ -733,6 +782,23 @@ int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes, uint64
 
 int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (h->v5) {
+    const PdltHost& p = h->v5->H;
+    if (f13) {
+      for (int i = 0; i < 3; ++i) f13[i] = p.center[i];
+      f13[3] = p.scale;
+      f13[4] = p.pce;
+      for (int i = 0; i < 3; ++i) f13[5 + i] = p.g_min[i];
+      for (int i = 0; i < 3; ++i) f13[8 + i] = p.g_range[i];
+      f13[11] = f13[12] = 0.f;
+    }
+    if (i16) {
+      for (int i = 0; i < 16; ++i) i16[i] = 0;
+      for (int i = 0; i < 3; ++i) i16[i] = p.g_bits[i];
+      i16[15] = (int32_t)p.code_width;
+    }
+    return STRIDE_OK;
+  }
   const DupHeader& d = h->set.hdr[0];
   if (f13) {
     for (int i = 0; i < 3; ++i) f13[i] = d.center[i];
@@ -753,10 +819,22 @@ int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16) {
 
 int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (h->v5) {
+    if (!t) return fail(STRIDE_E_ARG, "null argument");
+    return h->v5->set_layout(t->group_size, t->lds_bytes);
+  }
   return h->set.set_tune(t);
 }
 int stride_decoder_get_tune(const stride_decoder* h, stride_tune* t) {
   if (!h || !t) return fail(STRIDE_E_ARG, "null argument");
+  if (h->v5) {
+    std::memset(t, 0, sizeof *t);
+    t->mode = 3;
+    t->waves = 1;
+    t->group_size = h->v5->group_size;
+    t->lds_bytes = h->v5->lds_budget;
+    return STRIDE_OK;
+  }
   h->set.get_tune(t);
   return STRIDE_OK;
 }

@@ -2,23 +2,32 @@
 
 `ParaDeltaV5GpuDecoder(data)` keeps the reference's class name, module path
 and keyword, and decodes the DPR4 v3 bytes that `STRIDEEncoder.encode()`
-returns (or the DPRB side-table blob of utils/dprb.py, told apart by its
-magic), on MI355X through libstride_hip.so:
+returns (or the DPRB side-table blob of utils/dprb.py), and the PDLT v5
+streams the reference class itself decodes (encoder/paradelta_v5.py;
+written here by encoder.paradelta_v5.encode_from_prepared_v5), told apart by
+their magic, on MI355X through libstride_hip.so:
 
-* construction uploads the stream, runs the index pre-pass kernel that
-  derives the per-meshlet side table, validates every meshlet and allocates
-  the outputs;
-* `decode()` is one kernel launch on the current torch stream; it returns
-  the decoder-owned device tensors (overwritten by each call): float32
-  (n_v, 3) WORLD positions and int32 (n_t, 3) indices (the reference's v5
-  path returned normalised positions; DPR4 positions are dequantised to
-  world space inside the kernel, as decode_dup does);
+* construction uploads the stream, runs the index pre-pass (a kernel for
+  DPR4, host threads for PDLT v5) that derives the per-meshlet side table,
+  validates every meshlet and allocates the outputs;
+* `decode()` is one kernel launch on the current torch stream (two for PDLT
+  v5: boundary table, meshlets); it returns the decoder-owned device tensors
+  (overwritten by each call): float32 (n_v, 3) positions and int32 (n_t, 3)
+  indices.  DPR4 positions are WORLD positions (dequantised inside the
+  kernel, as decode_dup does);
 * `decode_to_host()` synchronises and returns numpy float32 (n_v, 3) and
   uint32 (n_t, 3).
 
+PDLT v5 streams decode bit for bit as decode_paradelta_v5
+(encoder/paradelta_v5.py:286-462, float64 reconstruction): the boundary
+table first, then each meshlet's interior vertices.  For them `decode()`
+returns normalised positions, as the reference's v5 decode() does
+(`d_verts_norm`; world = norm * scale + center); `decode_to_host()` returns
+world positions.
+
 Errors: ValueError("bad magic 0x........") / ValueError("expected v3, got
-vN") as the reference; IndexError for truncated streams (the reference's
-BitReader raises IndexError); ValueError for malformed meshlets.
+vN") (v5 for PDLT) as the reference; IndexError for truncated streams (the
+reference's BitReader raises IndexError); ValueError for malformed meshlets.
 """
 from __future__ import annotations
 
@@ -72,10 +81,15 @@ class ParaDeltaV5GpuDecoder:
         self.pred_n = i16[3:12].reshape(3, 3).copy()
         self.pred_K = i16[12:15].copy()
         self.code_width = int(i16[15])
+        ver, lin3, nb = C.c_int32(), np.zeros(3, np.float32), C.c_uint32()
+        _lib.check(L.stride_decoder_format(h, C.byref(ver), _lib.ptr(lin3), C.byref(nb)))
+        self.version = int(ver.value)
+        self.lin3_w = lin3
+        self.n_boundary = int(nb.value)
         self.d_verts = torch.empty((self.n_v_total, 3), dtype=torch.float32, device=self.device)
         self.d_tris = torch.empty((self.n_t_total, 3), dtype=torch.int32, device=self.device)
         # the reference harnesses read these names (scripts/bench_stride_decode_sweep.py:53-79)
-        self.d_verts_norm = self.d_verts
+        self.d_verts_norm = self.d_verts if self.version != 5 else torch.empty_like(self.d_verts)
         self.d_buf_bytes = self.stream_bytes
 
     def __del__(self):
@@ -99,13 +113,24 @@ class ParaDeltaV5GpuDecoder:
     # --------------------------------------------------------------- decode
     def decode(self, stream=None):
         """Launch on `stream` (default: torch's current stream); returns the
-        decoder-owned (verts f32 (n_v,3), tris int32 (n_t,3)) device tensors."""
+        decoder-owned (verts f32 (n_v,3), tris int32 (n_t,3)) device tensors:
+        world positions for DPR4, normalised ones for PDLT v5."""
+        if self.version == 5:
+            _lib.check(self._L.stride_decode_normalized(
+                self._h, _lib.ptr(self.d_verts_norm), _lib.ptr(self.d_tris), _lib.stream_ptr(stream)))
+            return self.d_verts_norm, self.d_tris
+        _lib.check(self._L.stride_decode(self._h, _lib.ptr(self.d_verts), _lib.ptr(self.d_tris),
+                                         _lib.stream_ptr(stream)))
+        return self.d_verts, self.d_tris
+
+    def decode_world(self, stream=None):
+        """As decode(), with world positions for every format."""
         _lib.check(self._L.stride_decode(self._h, _lib.ptr(self.d_verts), _lib.ptr(self.d_tris),
                                          _lib.stream_ptr(stream)))
         return self.d_verts, self.d_tris
 
     def decode_into(self, d_verts, d_tris, stream=None) -> None:
-        """Decode into caller-owned contiguous device tensors."""
+        """Decode into caller-owned contiguous device tensors (world positions)."""
         if not (d_verts.is_contiguous() and d_tris.is_contiguous()):
             raise ValueError("outputs must be contiguous")
         if d_verts.numel() < 3 * self.n_v_total or d_tris.numel() < 3 * self.n_t_total:
@@ -118,7 +143,7 @@ class ParaDeltaV5GpuDecoder:
     def decode_to_host(self):
         """(verts float32 (n_v,3) world, tris uint32 (n_t,3)), synchronous."""
         torch = _torch()
-        v, t = self.decode()
+        v, t = self.decode_world()
         torch.cuda.current_stream().synchronize()
         return (v.cpu().numpy(), t.cpu().numpy().view(np.uint32))
 

@@ -449,10 +449,12 @@ def read_header(data: bytes) -> dict:
     return h
 
 
-def decode_v5(data: bytes, return_info: bool = False):
+def decode_v5(data: bytes, return_info: bool = False, world: bool = True):
     """(V float32 (n, 3), T int64 (m, 3)) as decode_paradelta_v5
     (encoder/paradelta_v5.py:286-462): f64 positions, the boundary centroid
-    as fallback prediction, LIN3 weights as stored (f32 widened to f64)."""
+    as fallback prediction, LIN3 weights as stored (f32 widened to f64).
+    world=False: the normalised positions (before scale and center) as
+    float32."""
     h = read_header(data)
     g_bits = h["g_bits"]
     bnd_norm = dequant(h["bnd_codes"], h["g_min"], h["g_range"], g_bits) \
@@ -542,8 +544,8 @@ def decode_v5(data: bytes, return_info: bool = False):
         info["emit"].append(emit)
         for (a, b, c) in local_tris:
             tris.append((l2g[a], l2g[b], l2g[c]))
-    V = (np.array(verts, dtype=np.float64).reshape(-1, 3) * h["scale"]
-         + np.asarray(h["center"], dtype=np.float64)).astype(np.float32)
+    Vn = np.array(verts, dtype=np.float64).reshape(-1, 3)
+    V = (Vn * h["scale"] + np.asarray(h["center"], dtype=np.float64) if world else Vn).astype(np.float32)
     T = np.array(tris, dtype=np.int64).reshape(-1, 3)
     if return_info:
         info["header"] = h

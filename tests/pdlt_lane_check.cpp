@@ -1,0 +1,110 @@
+// Host build of the PDLT v5 decoder's set-up and per-lane logic
+// (meshpress_amd/csrc/pdlt_host.h, pdlt_lane.h): the header and boundary
+// table parse, the meshlet pre-pass, the wave-group layout and, per group,
+// the kernel's four phases (token lanes, boundary gather, axis lanes,
+// output) over an LDS image of exactly the launch's size, under the
+// sanitizers when built with them.
+//
+//   pdlt_lane_check <stream> <verts.f32 out> <tris.u32 out> [group_size lds_bytes [normalised]]
+//
+// Exit status: 0 ok, 2 stream rejected (message on stderr).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../meshpress_amd/csrc/pdlt_host.h"
+
+using namespace stride;
+
+int main(int argc, char** argv) {
+  if (argc < 4) return 1;
+  FILE* f = std::fopen(argv[1], "rb");
+  if (!f) return 1;
+  std::vector<uint8_t> data;
+  uint8_t buf[65536];
+  size_t r;
+  while ((r = std::fread(buf, 1, sizeof buf, f)) > 0) data.insert(data.end(), buf, buf + r);
+  std::fclose(f);
+  const int gs = argc > 5 ? std::atoi(argv[4]) : 16;
+  const int budget = argc > 5 ? std::atoi(argv[5]) : 32768;
+  const bool world = !(argc > 6 && std::atoi(argv[6]) == 1);
+  PdltHost H;
+  std::string err;
+  int rc = pdlt_parse(data.data(), data.size(), H, err);
+  if (rc) {
+    std::fprintf(stderr, "rejected (%d): %s\n", rc, err.c_str());
+    return 2;
+  }
+  std::vector<PdltGroup> groups;
+  uint32_t lds_bytes = 0;
+  rc = pdlt_layout(H.tab, gs, budget, groups, lds_bytes, err);
+  if (rc) {
+    std::fprintf(stderr, "layout (%d): %s\n", rc, err.c_str());
+    return 2;
+  }
+  PdltParams P = H.params;
+  P.world = world ? 1 : 0;
+  std::vector<float> V(3 * H.n_v);
+  std::vector<uint32_t> T(3 * H.n_t);
+  for (size_t i = 0; i < H.bnd.size(); ++i) V[i] = pdlt_out(H.bnd[i], P, (int)(i % 3));
+  for (const PdltGroup& G : groups) {
+    std::vector<uint64_t> lds64((lds_bytes + 7) / 8, 0x5A5A5A5A5A5A5A5Aull);   // stale LDS
+    uint32_t* lds = reinterpret_cast<uint32_t*>(lds64.data());
+    for (uint32_t i = 0; i < G.n_words; ++i) lds[i] = __builtin_bswap32(H.words.at(G.w_lo + i));
+    const int64_t rel = -32 * (int64_t)G.w_lo;
+    auto blk = [&](const PdltEntry& e) { return lds + G.blk_base + e.blk_off; };
+    // phase A: token lanes
+    for (uint32_t m = G.m0; m < G.m1; ++m) {
+      const PdltEntry& e = H.tab[m];
+      uint32_t* b = blk(e);
+      const uint32_t nl = e.n_bnd + e.n_int;
+      uint32_t* refs = b + 6 * nl;
+      uint32_t* walk = refs + e.n_bnd;
+      uint32_t* seen = walk + e.n_int;
+      pdlt_tokens(lds, rel, e, refs, walk, seen, seen + 8, T.data() + 3ull * e.t_off);
+    }
+    // phase B: boundary reconstructions
+    for (uint32_t m = G.m0; m < G.m1; ++m) {
+      const PdltEntry& e = H.tab[m];
+      uint32_t* b = blk(e);
+      const uint32_t nl = e.n_bnd + e.n_int;
+      double* recon = reinterpret_cast<double*>(b);
+      const uint32_t* refs = b + 6 * nl;
+      for (uint32_t i = 0; i < e.n_bnd; ++i)
+        for (int d = 0; d < 3; ++d) recon[d * nl + i] = H.bnd.at(3ull * refs[i] + d);
+    }
+    // phase C: axis lanes
+    for (uint32_t m = G.m0; m < G.m1; ++m) {
+      const PdltEntry& e = H.tab[m];
+      uint32_t* b = blk(e);
+      const uint32_t nl = e.n_bnd + e.n_int;
+      double* recon = reinterpret_cast<double*>(b);
+      const uint32_t* walk = b + 6 * nl + e.n_bnd;
+      const uint32_t n_emit = walk[e.n_int + 8];
+      for (int d = 0; d < 3; ++d) pdlt_axis(lds, rel, e, d, P, recon + d * nl, walk, n_emit);
+    }
+    // phase D: interior vertices
+    for (uint32_t m = G.m0; m < G.m1; ++m) {
+      const PdltEntry& e = H.tab[m];
+      const uint32_t* b = blk(e);
+      const uint32_t nl = e.n_bnd + e.n_int;
+      const double* rj = reinterpret_cast<const double*>(b);
+      const uint32_t* sj = b + 6 * nl + e.n_bnd + e.n_int;
+      for (uint32_t t = 0; t < 3 * e.n_int; ++t) {
+        const uint32_t i = t / 3, d = t - 3 * i, v = e.n_bnd + i;
+        const bool reached = ((sj[v >> 5] >> (v & 31)) & 1u) != 0;
+        V.at(3ull * e.int_base + t) = pdlt_out(reached ? rj[d * nl + v] : 0.0, P, (int)d);
+      }
+    }
+  }
+  FILE* fv = std::fopen(argv[2], "wb");
+  FILE* ft = std::fopen(argv[3], "wb");
+  if (!fv || !ft) return 1;
+  std::fwrite(V.data(), 4, V.size(), fv);
+  std::fwrite(T.data(), 4, T.size(), ft);
+  std::fclose(fv);
+  std::fclose(ft);
+  return 0;
+}

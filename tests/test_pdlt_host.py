@@ -1,0 +1,111 @@
+"""CPU tests of the PDLT v5 decoder's set-up and kernel logic, compiled for
+the host (tests/pdlt_lane_check.cpp, under ASan + UBSan), against the oracle
+(oracle/pdlt_oracle.py)."""
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import pdlt_oracle as PO
+from tests.conftest import small_meshes
+
+ROOT = Path(__file__).resolve().parents[1]
+GOLD = sorted((ROOT / "tests" / "golden").glob("pdlt_*.npz"))
+LAYOUTS = [(16, 32768), (1, 4096), (5, 16384), (16, 65536)]
+
+
+@pytest.fixture(scope="module")
+def pdlt_check(tmp_path_factory):
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path_factory.mktemp("pdlt") / "pdlt_lane_check"
+    subprocess.run([gxx, "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", "-ffp-contract=off", "-pthread",
+                    str(ROOT / "tests" / "pdlt_lane_check.cpp"), "-o", str(exe)], check=True)
+    return exe
+
+
+def run(exe, data, tmp_path, gs=16, lds=32768, world=True):
+    import subprocess
+    src = tmp_path / "s.pdlt"
+    src.write_bytes(data)
+    fv, ft = tmp_path / "v.f32", tmp_path / "t.u32"
+    r = subprocess.run([str(exe), str(src), str(fv), str(ft), str(gs), str(lds),
+                        "0" if world else "1"], capture_output=True, text=True)
+    if r.returncode != 0:
+        return r.returncode, r.stderr, None
+    return 0, np.fromfile(fv, np.float32).reshape(-1, 3), np.fromfile(ft, np.uint32).reshape(-1, 3)
+
+
+def same(V, T, Vo, To):
+    return (V.shape == Vo.shape and np.array_equal(V.view(np.uint32), Vo.view(np.uint32))
+            and np.array_equal(T.astype(np.int64), To))
+
+
+@pytest.mark.parametrize("path", GOLD, ids=lambda p: p.stem)
+def test_lane_logic_matches_oracle_on_goldens(pdlt_check, tmp_path, path):
+    data = np.load(path)["data"].tobytes()
+    Vo, To = PO.decode_v5(data)
+    for gs, lds in LAYOUTS:
+        rc, V, T = run(pdlt_check, data, tmp_path, gs, lds)
+        assert rc == 0, V
+        assert same(V, T, Vo, To), (gs, lds)
+    rc, V, T = run(pdlt_check, data, tmp_path, world=False)
+    Vn, _ = PO.decode_v5(data, world=False)
+    assert rc == 0 and same(V, T, Vn, To)
+
+
+def test_lane_logic_matches_oracle_on_fresh_streams(pdlt_check, tmp_path):
+    from meshpress_amd import workloads as w
+    from meshpress_amd.encoder.paradelta_v5 import encode_v5
+    cases = [(m[1], m[2], {}) for m in small_meshes()]
+    cases.append(w.heightfield(90, 0.1, 6, 7) + ({},))
+    cases.append(w.displaced_icosphere(4, 0.08, 6, 1) +
+                 (dict(precision_error=1.0 / 65534.5, precision_mode="bbox_frac"),))
+    cases.append(w.displaced_torus(40, 16, 1.0, 0.3, 0.05, 4, 5) + (dict(max_verts=64, max_tris=96),))
+    for V, T, kw in cases:
+        data = encode_v5(V, T, **kw)
+        rc, Vl, Tl = run(pdlt_check, data, tmp_path)
+        assert rc == 0
+        assert same(Vl, Tl, *PO.decode_v5(data))
+
+
+def test_prepass_rejects_or_matches_under_bit_flips(pdlt_check, tmp_path):
+    """Whenever the host-built pre-pass accepts a corrupted stream, the lane
+    decoder's output equals the oracle's (the oracle must not reject it)."""
+    base = bytearray(np.load(ROOT / "tests" / "golden" / "pdlt_hf_24_tags.npz")["data"].tobytes())
+    rng = np.random.default_rng(7)
+    accepted = 0
+    for trial in range(80):
+        d = bytearray(base)
+        pos = int(rng.integers(8 * 81, 8 * len(d)))
+        d[pos >> 3] ^= 1 << (7 - (pos & 7))
+        rc, V, T = run(pdlt_check, bytes(d), tmp_path)
+        if rc != 0:
+            assert rc == 2, V
+            continue
+        accepted += 1
+        try:
+            Vo, To = PO.decode_v5(bytes(d))
+        except (ValueError, IndexError) as e:
+            raise AssertionError(f"trial {trial}: pre-pass accepted a stream the oracle rejects: {e}")
+        assert same(V, T, Vo, To), trial
+    assert accepted > 0
+
+
+def test_truncated_and_bad_headers_rejected(pdlt_check, tmp_path):
+    data = np.load(ROOT / "tests" / "golden" / "pdlt_uv_12x24.npz")["data"].tobytes()
+    for cut in (3, 40, 81, 100, len(data) // 2, len(data) - 1):
+        rc, msg, _ = run(pdlt_check, data[:cut], tmp_path)
+        assert rc == 2, cut
+    bad = bytearray(data)
+    bad[4] = 4
+    rc, msg, _ = run(pdlt_check, bytes(bad), tmp_path)
+    assert rc == 2 and "expected v5, got v4" in msg
+    bad = bytearray(data)
+    bad[0] = 0
+    rc, msg, _ = run(pdlt_check, bytes(bad), tmp_path)
+    assert rc == 2 and "bad magic" in msg
