@@ -18,6 +18,9 @@ run also decodes
 * the learned encoder proposal against the exhaustive search on the
   headline mesh and on held-out meshes: search-time speedup, bit-rate
   regret, and decode equality of the two streams;
+* the headline mesh as a PDLT v5 stream (global boundary table, LIN3
+  predictor), decoded by the same class with the float64 chain of
+  decode_paradelta_v5 and with the encoder's float32 chain;
 
 Prints one JSON line with the headline throughput in Gtris/s, the dominant
 kernel's HBM roofline figures, a CPU baseline (the C oracle, one thread,
@@ -151,6 +154,57 @@ def single_mesh(name, V, T, bits, steps, warmup, tune=True):
         r["tune_sweep_ms"] = sweep
     log(f"{name}: {s_gpu * 1e3:.3f} ms/launch, {r['gtris_s']:.3f} Gtris/s, {r['hbm_gb_s']:.1f} GB/s")
     return dec, data, r, prep
+
+
+PDLT_LAYOUTS = [(16, 32768), (16, 49152), (16, 65536), (8, 24576), (8, 16384), (4, 16384)]
+
+
+def pdlt_v5(prep, name, steps, warmup):
+    """The headline mesh as a PDLT v5 stream (encode_from_prepared_v5 over the
+    same plans), decoded by the same class: float64 chain (bit for bit
+    decode_paradelta_v5) and the encoder's float32 chain."""
+    import torch
+    from meshpress_amd.encoder.paradelta_v5 import encode_from_prepared_v5
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    t0 = time.perf_counter()
+    data, info = encode_from_prepared_v5(prep, return_info=True)
+    enc_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    dec = ParaDeltaV5GpuDecoder(data)
+    create_s = time.perf_counter() - t0
+    log(f"{name} v5: {len(data):,} bytes ({8 * len(data) / prep['n_v']:.2f} bpv), "
+        f"{dec.n_boundary:,} boundary vertices, encode {enc_s:.1f}s, create {create_s:.2f}s")
+    sweep = []
+    for gs, lds in PDLT_LAYOUTS:
+        dec.set_tune(group_size=gs, lds_bytes=lds)
+        _, g = time_calls(dec.decode_world, 10, 3)
+        sweep.append([gs, lds, round(g * 1e3, 4)])
+    gs, lds, _ = min(sweep, key=lambda x: x[2])
+    dec.set_tune(group_size=gs, lds_bytes=lds)
+    s_wall, s_gpu = time_calls(dec.decode_world, steps, warmup)
+    v64, t64 = dec.decode_world()
+    v64 = v64.clone()
+    dec32 = ParaDeltaV5GpuDecoder(data, chain="f32")
+    dec32.set_tune(group_size=gs, lds_bytes=lds)
+    w32, g32 = time_calls(dec32.decode_world, steps, warmup)
+    v32, t32 = dec32.decode_world()
+    torch.cuda.synchronize()
+    step = float(prep["per_coord_err"]) * float(prep["scale"])
+    drift = float((v64.double() - v32.double()).abs().max()) / step
+    bytes_alg = dec.bytes_read + dec.bytes_written
+    r = {"workload": f"{name}_pdlt_v5", "n_tris": dec.n_t_total, "n_verts_out": dec.n_v_total,
+         "meshlets": dec.n_meshlets, "n_boundary": dec.n_boundary, "stream_bytes": dec.stream_bytes,
+         "side_bytes": dec.side_bytes, "bpv": 8 * len(data) / prep["n_v"],
+         "lin3": [float(x) for x in info["lin3"]], "encode_s": enc_s, "create_s": create_s,
+         "tune": {"group_size": gs, "lds_bytes": lds}, "tune_sweep_ms": sweep,
+         "ms_wall": s_wall * 1e3, "ms_kernels": s_gpu * 1e3, "gtris_s": dec.n_t_total / s_wall / 1e9,
+         "hbm_gb_s": bytes_alg / s_gpu / 1e9, "algorithmic_bytes": bytes_alg,
+         "f32_chain": {"ms_kernels": g32 * 1e3, "gtris_s": dec.n_t_total / w32 / 1e9,
+                       "tris_equal": bool(torch.equal(t64, t32)),
+                       "max_f64_f32_drift_steps": drift}}
+    log(f"{name} v5: {s_gpu * 1e3:.3f} ms, {r['gtris_s']:.3f} Gtris/s (f32 chain {g32 * 1e3:.3f} ms), "
+        f"drift {drift:.4f} steps")
+    return r
 
 
 def batched(n_meshes, steps, warmup, predictor):
@@ -329,6 +383,8 @@ def main():
             sys.exit("GPU decode differs from the oracle on the bench workload")
     del dec
     if not a.quick:
+        details[f"{name}_pdlt_v5"] = pdlt_v5(head_prep, name, a.steps, a.warmup)
+        torch.cuda.empty_cache()
         if name.startswith("heightfield_1900"):
             other = "heightfield_1900_q16" if bits == 12 else "heightfield_1900_q12"
             _, _, details[other], _ = single_mesh(other, V, T, 28 - bits, a.steps, a.warmup)

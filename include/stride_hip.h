@@ -103,6 +103,13 @@ int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris);
  * before scale and center), as the reference's decode() returns them
  * (utils/paradelta_v5_cuda.py:686-712); STRIDE_E_ARG for DPR4. */
 int stride_decode_normalized(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
+/* PDLT v5 reconstruction arithmetic: chain_bits 64 (default) = float64, bit
+ * for bit decode_paradelta_v5; 32 = the float32 chain the encoder computed
+ * its codes against (encoder/paradelta_v5_nb.py:20-212), so positions are
+ * the encoder's own reconstructions and stay within the quantisation bound
+ * (the float64 chain drifts from them, by up to 2.5 steps on 16-bit grids).
+ * DPR4 accepts only 64. */
+int stride_decoder_set_precision(stride_decoder* h, int32_t chain_bits);
 /* Stream format: *version = 3 (DPR4 v3) or 5 (PDLT v5); for PDLT v5 also the
  * LIN3 weights and the boundary-table size (zeros for DPR4). */
 int stride_decoder_format(const stride_decoder* h, int32_t* version, float* lin3,

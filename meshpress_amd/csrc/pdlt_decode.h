@@ -18,6 +18,7 @@ struct PdltSet {
   uint32_t lds_bytes = 0;               // dynamic LDS of the launch
   int group_size = 16;
   int lds_budget = 32768;
+  int chain32 = 0;                      // 1: the encoder's float32 chain
   // device
   uint32_t* d_words = nullptr;
   PdltEntry* d_tab = nullptr;

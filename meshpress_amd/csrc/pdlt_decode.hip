@@ -29,7 +29,7 @@ __global__ __launch_bounds__(256) void pdlt_boundary_kernel(const double* __rest
                                                             float* __restrict__ V) {
   for (uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x; i < n3;
        i += (uint64_t)gridDim.x * 256) {
-    V[i] = pdlt_out(bnd[i], P, (int)(i % 3));
+    V[i] = pdlt_out(pdlt_bnd(bnd[i], P), P, (int)(i % 3));
   }
 }
 
@@ -68,7 +68,7 @@ __global__ __launch_bounds__(64) void pdlt_decode_kernel(
         const uint32_t idx = base + 4 * u;
         if (idx < nb3) {
           const uint32_t i = idx / 3, d = idx - 3 * i;
-          x[u] = bnd[3ull * refs[i] + d];
+          x[u] = pdlt_bnd(bnd[3ull * refs[i] + d], P);
         }
       }
 #pragma unroll
@@ -173,6 +173,7 @@ int PdltSet::decode(float* V, uint32_t* T, int world, hipStream_t s) {
   if ((!V && H.n_v) || (!T && H.n_t)) return set_error(STRIDE_E_ARG, "null output pointer");
   PdltParams p = H.params;
   p.world = world ? 1 : 0;
+  p.chain32 = chain32;
   if (H.n_b) {
     const uint64_t n3 = 3ull * H.n_b;
     const uint32_t grid = (uint32_t)std::min<uint64_t>((n3 + 255) / 256, 8192);

@@ -70,7 +70,8 @@ struct PdltParams {
   double center[3];
   double scale;
   int32_t world;         // 1: world coordinates, 0: normalised
-  int32_t pad[3];
+  int32_t chain32;       // 1: the encoder's float32 chain, 0: float64 (decode_paradelta_v5)
+  int32_t pad[2];
 };
 
 enum : uint32_t {
@@ -385,14 +386,29 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
 // order, then divided by n_bnd), then the residual codes of axis d along
 // the walk.  rc = the axis-d reconstruction array of the meshlet, whose
 // boundary entries are filled.
+//
+// P.chain32 selects the arithmetic: 0 = float64, as decode_paradelta_v5
+// (encoder/paradelta_v5.py:422-446); 1 = the float32 chain the encoder
+// computed its codes against (encoder/paradelta_v5_nb.py:44-59,113-176:
+// float32 boundary positions, float32 centroid, float32 LIN3 terms summed
+// left to right, code * float32 delta), so every position is the encoder's
+// own reconstruction; values stay in the float64 array, exactly.
 STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e, int d,
                           const PdltParams& P, double* __restrict__ rc,
                           const uint32_t* __restrict__ walk, uint32_t n_emit) {
   if (n_emit == 0) return;   // no residual section is read (n_int may be 0)
   const uint32_t n_bnd = e.n_bnd;
-  double s = 0.0;
-  for (uint32_t i = 0; i < n_bnd; ++i) s = s + rc[i];
-  const double fb = n_bnd ? s / (double)n_bnd : 0.0;
+  const bool c32 = P.chain32 != 0;
+  double fb;
+  if (c32) {
+    float s = 0.f;
+    for (uint32_t i = 0; i < n_bnd; ++i) s = s + (float)rc[i];
+    fb = n_bnd ? (double)(s / (float)n_bnd) : 0.0;
+  } else {
+    double s = 0.0;
+    for (uint32_t i = 0; i < n_bnd; ++i) s = s + rc[i];
+    fb = n_bnd ? s / (double)n_bnd : 0.0;
+  }
   const uint32_t ab = d == 0 ? e.axis_bit[0] : (d == 1 ? e.axis_bit[1] : e.axis_bit[2]);
   uint32_t p = (uint32_t)((int64_t)ab + rel);
   const uint64_t H = win64(st, p);
@@ -408,6 +424,7 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
     p += 16;
   }
   const double w0 = P.w[0], w1 = P.w[1], w2 = P.w[2], delta = P.delta;
+  const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, deltaf = (float)delta;
   for (uint32_t i = 0; i < n_emit; ++i) {
     int32_t code;
     if (tag == 0) {
@@ -419,18 +436,32 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
     }
     const uint32_t ent = walk[i];
     const uint32_t v = ent & 0xFFu, a = (ent >> 8) & 0xFFu;
-    double pred;
-    if (a == v) {
-      pred = fb;
+    const uint32_t b = (ent >> 16) & 0xFFu, c = ent >> 24;
+    if (c32) {
+      float pred = (float)fb;
+      if (a != v) {
+        const float t0 = w0f * (float)rc[a];
+        const float t1 = w1f * (float)rc[b];
+        const float t2 = w2f * (float)rc[c];
+        pred = (t0 + t1) + t2;
+      }
+      rc[v] = (double)(pred + (float)code * deltaf);
     } else {
-      const uint32_t b = (ent >> 16) & 0xFFu, c = ent >> 24;
-      const double t0 = w0 * rc[a];
-      const double t1 = w1 * rc[b];
-      const double t2 = w2 * rc[c];
-      pred = (t0 + t1) + t2;
+      double pred = fb;
+      if (a != v) {
+        const double t0 = w0 * rc[a];
+        const double t1 = w1 * rc[b];
+        const double t2 = w2 * rc[c];
+        pred = (t0 + t1) + t2;
+      }
+      rc[v] = pred + (double)code * delta;
     }
-    rc[v] = pred + (double)code * delta;
   }
+}
+
+// Boundary-table position as the chain starts from it.
+STRIDE_DEV double pdlt_bnd(double x, const PdltParams& P) {
+  return P.chain32 ? (double)(float)x : x;
 }
 
 // Output value of local v on axis d (interior locals never reached by a

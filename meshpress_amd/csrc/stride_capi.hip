@@ -733,6 +733,17 @@ int stride_decode_normalized(stride_decoder* h, float* d_verts, uint32_t* d_tris
   return h->v5->decode(d_verts, d_tris, 0, (hipStream_t)stream);
 }
 
+int stride_decoder_set_precision(stride_decoder* h, int32_t chain_bits) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (chain_bits != 64 && chain_bits != 32) return fail(STRIDE_E_ARG, "chain_bits must be 64 or 32");
+  if (!h->v5) {
+    if (chain_bits == 64) return STRIDE_OK;   // DPR4 decodes integer codes exactly
+    return fail(STRIDE_E_ARG, "the float32 chain is defined for PDLT v5 streams only");
+  }
+  h->v5->chain32 = chain_bits == 32 ? 1 : 0;
+  return STRIDE_OK;
+}
+
 int stride_decoder_format(const stride_decoder* h, int32_t* version, float* lin3,
                           uint32_t* n_boundary) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");

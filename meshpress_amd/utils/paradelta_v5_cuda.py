@@ -23,7 +23,10 @@ PDLT v5 streams decode bit for bit as decode_paradelta_v5
 table first, then each meshlet's interior vertices.  For them `decode()`
 returns normalised positions, as the reference's v5 decode() does
 (`d_verts_norm`; world = norm * scale + center); `decode_to_host()` returns
-world positions.
+world positions.  chain="f32" reconstructs with the encoder's own float32
+chain instead (encoder/paradelta_v5_nb.py:20-212): positions are then the
+encoder's reconstructions, within the quantisation bound, where the float64
+chain of decode_paradelta_v5 drifts from them by float32 rounding.
 
 Errors: ValueError("bad magic 0x........") / ValueError("expected v3, got
 vN") (v5 for PDLT) as the reference; IndexError for truncated streams (the
@@ -46,9 +49,11 @@ def _torch():
 
 
 class ParaDeltaV5GpuDecoder:
-    def __init__(self, data: bytes, meshlets_per_block: int = 1):
+    def __init__(self, data: bytes, meshlets_per_block: int = 1, *, chain: str = "f64"):
         if meshlets_per_block < 1 or meshlets_per_block > 16:
             raise ValueError("meshlets_per_block must be in [1, 16]")
+        if chain not in ("f64", "f32"):
+            raise ValueError("chain must be 'f64' or 'f32'")
         torch = _torch()
         L = _lib.lib()
         self._L = L
@@ -86,6 +91,8 @@ class ParaDeltaV5GpuDecoder:
         self.version = int(ver.value)
         self.lin3_w = lin3
         self.n_boundary = int(nb.value)
+        _lib.check(L.stride_decoder_set_precision(h, 32 if chain == "f32" else 64))
+        self.chain = chain
         self.d_verts = torch.empty((self.n_v_total, 3), dtype=torch.float32, device=self.device)
         self.d_tris = torch.empty((self.n_t_total, 3), dtype=torch.int32, device=self.device)
         # the reference harnesses read these names (scripts/bench_stride_decode_sweep.py:53-79)

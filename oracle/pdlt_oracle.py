@@ -136,10 +136,12 @@ def strip_traversal(tris_local, strips, n_bnd):
     return order
 
 
-def interior_pass(plan, vn, bnd_recon_norm, delta, w3=None):
+def interior_pass(plan, vn, bnd_recon_norm, delta, w3=None, recon_out=None):
     """(codes int64 (n_int, 3), samples) of one meshlet with the encoder's
     float32 chain (encoder/paradelta_v5_nb.py:20-212).  w3 None is the
-    sample-collecting pass with the plain parallelogram a + b - c."""
+    sample-collecting pass with the plain parallelogram a + b - c.
+    recon_out (dict): receives global id -> float32 reconstruction of
+    every interior vertex."""
     n_bnd, n_int = int(plan["n_bnd"]), int(plan["n_int"])
     if n_int == 0:
         return np.zeros((0, 3), dtype=np.int64), []
@@ -177,6 +179,8 @@ def interior_pass(plan, vn, bnd_recon_norm, delta, w3=None):
             cd = int(np.round(f32(f32(true - pred[d]) / dlt)))
             codes[i, d] = cd
             recon[v, d] = f32(pred[d] + f32(f32(cd) * dlt))
+        if recon_out is not None:
+            recon_out[g] = recon[v].copy()
         if kind == "para" and w is None:
             a, b, c = refs
             samples.append((recon[a].astype(np.float64), recon[b].astype(np.float64),
@@ -449,18 +453,27 @@ def read_header(data: bytes) -> dict:
     return h
 
 
-def decode_v5(data: bytes, return_info: bool = False, world: bool = True):
+def decode_v5(data: bytes, return_info: bool = False, world: bool = True, chain: str = "f64"):
     """(V float32 (n, 3), T int64 (m, 3)) as decode_paradelta_v5
     (encoder/paradelta_v5.py:286-462): f64 positions, the boundary centroid
     as fallback prediction, LIN3 weights as stored (f32 widened to f64).
     world=False: the normalised positions (before scale and center) as
-    float32."""
+    float32.  chain="f32": the reconstruction chain in float32 exactly as
+    the encoder ran it (interior_pass with the stored weights), so positions
+    are the encoder's own reconstructions; the output transform stays f64."""
+    if chain not in ("f64", "f32"):
+        raise ValueError("chain must be 'f64' or 'f32'")
+    c32 = chain == "f32"
     h = read_header(data)
     g_bits = h["g_bits"]
     bnd_norm = dequant(h["bnd_codes"], h["g_min"], h["g_range"], g_bits) \
         if h["n_boundary"] else np.zeros((0, 3))
     delta = 2.0 * h["per_coord_err"]
     w0, w1, w2 = h["lin3"]
+    if c32:
+        bnd_norm = bnd_norm.astype(np.float32).astype(np.float64)
+        w32 = [f32(x) for x in h["lin3"]]
+        d32 = f32(delta)
     verts = [row.copy() for row in bnd_norm]
     tris = []
     r = BitReader(data, h["meshlets_bit"])
@@ -526,18 +539,32 @@ def decode_v5(data: bytes, return_info: bool = False, world: bool = True):
                      for lid in range(n_bnd)}
             # boundary centroid: rows summed in order, then divided by n_bnd
             fb = np.zeros(3)
-            if n_bnd > 0:
+            if n_bnd > 0 and not c32:
                 s = np.zeros(3)
                 for lid in range(n_bnd):
                     s = s + recon[lid]
                 fb = s / n_bnd
+            elif n_bnd > 0:
+                s = [f32(0.0)] * 3
+                for lid in range(n_bnd):
+                    s = [f32(s[d] + f32(recon[lid][d])) for d in range(3)]
+                fb = np.array([float(f32(s[d] / f32(n_bnd))) for d in range(3)])
             for i, (v, abc) in enumerate(emit):
-                if abc is None:
-                    pred = fb.copy()
+                if c32:
+                    if abc is None:
+                        pred = [f32(x) for x in fb]
+                    else:
+                        a, b, c = abc
+                        pred = [f32(f32(f32(w32[0] * f32(recon[a][d])) + f32(w32[1] * f32(recon[b][d])))
+                                    + f32(w32[2] * f32(recon[c][d]))) for d in range(3)]
+                    rec = np.array([float(f32(pred[d] + f32(f32(int(codes[i, d])) * d32)))
+                                    for d in range(3)])
+                elif abc is None:
+                    rec = fb.copy() + codes[i].astype(np.float64) * delta
                 else:
                     a, b, c = abc
                     pred = (w0 * recon[a] + w1 * recon[b]) + w2 * recon[c]
-                rec = pred + codes[i].astype(np.float64) * delta
+                    rec = pred + codes[i].astype(np.float64) * delta
                 recon[v] = rec
                 verts[l2g[v]] = rec
             info["codes"].append(codes)

@@ -5,7 +5,7 @@
 // output) over an LDS image of exactly the launch's size, under the
 // sanitizers when built with them.
 //
-//   pdlt_lane_check <stream> <verts.f32 out> <tris.u32 out> [group_size lds_bytes [normalised]]
+//   pdlt_lane_check <stream> <verts.f32 out> <tris.u32 out> [group_size lds_bytes [normalised [chain32]]]
 //
 // Exit status: 0 ok, 2 stream rejected (message on stderr).
 #include <cstdio>
@@ -30,6 +30,7 @@ int main(int argc, char** argv) {
   const int gs = argc > 5 ? std::atoi(argv[4]) : 16;
   const int budget = argc > 5 ? std::atoi(argv[5]) : 32768;
   const bool world = !(argc > 6 && std::atoi(argv[6]) == 1);
+  const bool chain32 = argc > 7 && std::atoi(argv[7]) == 1;
   PdltHost H;
   std::string err;
   int rc = pdlt_parse(data.data(), data.size(), H, err);
@@ -46,9 +47,10 @@ int main(int argc, char** argv) {
   }
   PdltParams P = H.params;
   P.world = world ? 1 : 0;
+  P.chain32 = chain32 ? 1 : 0;
   std::vector<float> V(3 * H.n_v);
   std::vector<uint32_t> T(3 * H.n_t);
-  for (size_t i = 0; i < H.bnd.size(); ++i) V[i] = pdlt_out(H.bnd[i], P, (int)(i % 3));
+  for (size_t i = 0; i < H.bnd.size(); ++i) V[i] = pdlt_out(pdlt_bnd(H.bnd[i], P), P, (int)(i % 3));
   for (const PdltGroup& G : groups) {
     std::vector<uint64_t> lds64((lds_bytes + 7) / 8, 0x5A5A5A5A5A5A5A5Aull);   // stale LDS
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds64.data());
@@ -73,7 +75,7 @@ int main(int argc, char** argv) {
       double* recon = reinterpret_cast<double*>(b);
       const uint32_t* refs = b + 6 * nl;
       for (uint32_t i = 0; i < e.n_bnd; ++i)
-        for (int d = 0; d < 3; ++d) recon[d * nl + i] = H.bnd.at(3ull * refs[i] + d);
+        for (int d = 0; d < 3; ++d) recon[d * nl + i] = pdlt_bnd(H.bnd.at(3ull * refs[i] + d), P);
     }
     // phase C: axis lanes
     for (uint32_t m = G.m0; m < G.m1; ++m) {

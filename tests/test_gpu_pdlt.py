@@ -18,9 +18,9 @@ LAYOUTS = [dict(group_size=16, lds_bytes=32768), dict(group_size=1, lds_bytes=40
            dict(group_size=5, lds_bytes=16384), dict(group_size=16, lds_bytes=65536)]
 
 
-def decoder(data, **tune):
+def decoder(data, chain="f64", **tune):
     from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
-    dec = ParaDeltaV5GpuDecoder(data)
+    dec = ParaDeltaV5GpuDecoder(data, chain=chain)
     if tune:
         dec.set_tune(**tune)
     return dec
@@ -42,6 +42,8 @@ def test_golden_streams_all_layouts(path):
         dec = decoder(data, **tune)
         assert dec.version == 5
         assert_same(*dec.decode_to_host(), Vo, To)
+    V32, _ = PO.decode_v5(data, chain="f32")
+    assert_same(*decoder(data, chain="f32").decode_to_host(), V32, To)
 
 
 def test_normalised_decode_matches_reference_semantics():
@@ -67,6 +69,7 @@ def test_fresh_streams():
         print(f"pdlt fresh case {i} n_t={len(T)}", flush=True)
         data = encode_v5(V, T, **kw)
         assert_same(*decoder(data).decode_to_host(), *PO.decode_v5(data))
+        assert_same(*decoder(data, chain="f32").decode_to_host(), *PO.decode_v5(data, chain="f32"))
 
 
 def test_repeated_decode_is_stable():
@@ -89,14 +92,22 @@ def test_decoder_errors():
         decoder(bytes(bad))
     with pytest.raises(IndexError):
         decoder(data[: len(data) // 2])
-    with pytest.raises(ValueError):
+    with pytest.raises(IndexError):      # an endless unary run in the boundary table
         decoder(data[:81] + bytes(len(data) - 81))
+    h = PO.read_header(data)
+    table = (h["meshlets_bit"] - 32 * h["n_meshlets"]) // 8
+    bad = bytearray(data)
+    bad[table + 3] ^= 8                   # first meshlet offset off by 8 bits
+    with pytest.raises(ValueError, match="offsets"):
+        decoder(bytes(bad))
 
 
 @pytest.mark.parametrize("n", [600])
 def test_large_mesh_error_bound_and_faces(n):
-    """A 718K-triangle mesh: every position within the quantisation bound of
-    its source vertex, the same face set, identical to a second layout."""
+    """A 718K-triangle mesh: with the encoder's float32 chain every position
+    is within the quantisation bound of its source vertex; the float64 chain
+    (decode_paradelta_v5's arithmetic) stays within float32 drift of it; the
+    face set is the source's; a second layout decodes identically."""
     from meshpress_amd import workloads as w
     from meshpress_amd.encoder.paradelta_v5 import encode_from_prepared_v5
     from meshpress_amd.encoder.prep import prepare
@@ -114,9 +125,14 @@ def test_large_mesh_error_bound_and_faces(n):
     ids = np.concatenate([np.asarray(pv["boundary_list"], np.int64)] +
                          [np.asarray(pl["local_to_global"][pl["n_bnd"]:], np.int64) for pl in plans])
     assert len(ids) == len(Vg)
-    err = np.abs(Vg.astype(np.float64) - V[ids]).max()
-    bound = p["per_coord_err"] * p["scale"] * (1 + 1e-3) + 8 * np.finfo(np.float32).eps * (np.abs(V).max() + 1)
+    V32, T32 = decoder(data, chain="f32").decode_to_host()
+    assert np.array_equal(T32, Tg)
+    step = p["per_coord_err"] * p["scale"]
+    err = np.abs(V32.astype(np.float64) - V[ids]).max()
+    bound = step + 16 * 2.0 ** -24 * (np.abs(V).max() + 1)
     assert err <= bound, (err, bound)
+    drift = np.abs(Vg.astype(np.float64) - V32).max()
+    assert drift <= 0.05 * step, (drift, step)
     got = np.sort(ids[Tg.astype(np.int64)], axis=1)
     want = np.sort(T, axis=1)
     assert np.array_equal(got[np.lexsort(got.T[::-1])], want[np.lexsort(want.T[::-1])])

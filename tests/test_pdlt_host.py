@@ -28,13 +28,13 @@ def pdlt_check(tmp_path_factory):
     return exe
 
 
-def run(exe, data, tmp_path, gs=16, lds=32768, world=True):
+def run(exe, data, tmp_path, gs=16, lds=32768, world=True, chain32=False):
     import subprocess
     src = tmp_path / "s.pdlt"
     src.write_bytes(data)
     fv, ft = tmp_path / "v.f32", tmp_path / "t.u32"
     r = subprocess.run([str(exe), str(src), str(fv), str(ft), str(gs), str(lds),
-                        "0" if world else "1"], capture_output=True, text=True)
+                        "0" if world else "1", "1" if chain32 else "0"], capture_output=True, text=True)
     if r.returncode != 0:
         return r.returncode, r.stderr, None
     return 0, np.fromfile(fv, np.float32).reshape(-1, 3), np.fromfile(ft, np.uint32).reshape(-1, 3)
@@ -56,6 +56,10 @@ def test_lane_logic_matches_oracle_on_goldens(pdlt_check, tmp_path, path):
     rc, V, T = run(pdlt_check, data, tmp_path, world=False)
     Vn, _ = PO.decode_v5(data, world=False)
     assert rc == 0 and same(V, T, Vn, To)
+    for world in (True, False):
+        rc, V, T = run(pdlt_check, data, tmp_path, 7, 20480, world=world, chain32=True)
+        V32, _ = PO.decode_v5(data, world=world, chain="f32")
+        assert rc == 0 and same(V, T, V32, To)
 
 
 def test_lane_logic_matches_oracle_on_fresh_streams(pdlt_check, tmp_path):
@@ -71,6 +75,9 @@ def test_lane_logic_matches_oracle_on_fresh_streams(pdlt_check, tmp_path):
         rc, Vl, Tl = run(pdlt_check, data, tmp_path)
         assert rc == 0
         assert same(Vl, Tl, *PO.decode_v5(data))
+        rc, Vl, Tl = run(pdlt_check, data, tmp_path, chain32=True)
+        assert rc == 0
+        assert same(Vl, Tl, *PO.decode_v5(data, chain="f32"))
 
 
 def test_prepass_rejects_or_matches_under_bit_flips(pdlt_check, tmp_path):

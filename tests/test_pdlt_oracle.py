@@ -250,3 +250,32 @@ def test_decode_errors():
         PO.decode_v5(bytes(bad))
     with pytest.raises(IndexError):
         PO.decode_v5(bytes(data[: len(data) // 2]))
+
+
+@pytest.mark.parametrize("q16", [False, True], ids=["q12", "q16"])
+def test_f32_chain_reproduces_the_encoder_reconstruction(q16):
+    """decode_v5(chain="f32") rebuilds exactly the float32 positions the
+    encoder's chain produced (interior_pass, paradelta_v5_nb.py:20-212), so
+    it stays within the quantisation bound; the float64 chain of
+    decode_paradelta_v5 drifts from them by float32 rounding."""
+    from meshpress_amd import workloads as w
+    kw = dict(precision_error=1.0 / 65534.5, precision_mode="bbox_frac") if q16 else {}
+    V, T = w.displaced_torus(60, 20, 1.0, 0.3, 0.05, 4, 11)
+    p, prep = _prep(V, T, **kw)
+    data, info = PO.encode_v5(prep, return_info=True)
+    pv = PO.prepare_v5(prep)
+    rec = {}
+    for pl in pv["plans"]:
+        PO.interior_pass(pl, pv["vn"], pv["bnd_recon_norm"], 2.0 * float(pv["per_coord_err"]),
+                         info["lin3"].astype(np.float64), rec)
+    Vn, _ = PO.decode_v5(data, world=False, chain="f32")
+    ids = _v5_order(prep)
+    nb = pv["n_boundary"]
+    want = np.array([rec[g] for g in ids[nb:]], np.float32).reshape(-1, 3)
+    assert np.array_equal(Vn[nb:].view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(Vn[:nb], pv["bnd_recon_norm"][ids[:nb]].astype(np.float32))
+    Vw, _ = PO.decode_v5(data, chain="f32")
+    step = p["per_coord_err"] * p["scale"]
+    # the encoder rounds to the grid against its float32 reconstruction:
+    # half a step (2 pce) plus float32 rounding of coordinates of size ~1
+    assert np.abs(Vw.astype(np.float64) - V[ids]).max() <= step + 4 * 2.0 ** -24 * p["scale"] * 4
