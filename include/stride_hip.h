@@ -90,6 +90,17 @@ void stride_decoder_destroy(stride_decoder* h);
 int stride_dprb_from_dpr4(const uint8_t* data, size_t len, uint8_t* out, size_t cap,
                           size_t* out_len);
 int stride_decoder_create_dprb(const uint8_t* blob, size_t len, stride_decoder** out);
+/* SBLB v1 blob (scripts/dump_stride_blob.py:1-114; the reference's native
+ * PDLT v5 decode bench input, bench_cpp/stride_decode_bench.cu:430-484): the
+ * PDLT v5 stream plus the per-meshlet tables that bench uploads (bit offsets,
+ * n_bnd, n_int, n_tris, n_strips, tris_off, int_cursor) and the float32
+ * boundary table.  stride_sblb_from_pdlt writes one from a PDLT v5 stream
+ * (out == NULL: only *out_len); stride_decoder_create_sblb decodes one and
+ * rejects it (STRIDE_E_CORRUPT) unless every table equals what the pre-pass
+ * derives from its stream. */
+int stride_sblb_from_pdlt(const uint8_t* data, size_t len, uint8_t* out, size_t cap,
+                          size_t* out_len);
+int stride_decoder_create_sblb(const uint8_t* blob, size_t len, stride_decoder** out);
 /* n_v = sum of per-meshlet n_local (duplicated boundary vertices included;
  * PDLT v5: n_boundary + sum of n_int), n_t = triangles, n_meshlets. */
 int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,

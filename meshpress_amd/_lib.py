@@ -54,6 +54,8 @@ SIGNATURES = {
     "stride_decode_normalized": (C.c_int, [p, p, p, p]),
     "stride_decoder_format": (C.c_int, [p, P(i32), p, P(u32)]),
     "stride_decoder_set_precision": (C.c_int, [p, i32]),
+    "stride_sblb_from_pdlt": (C.c_int, [p, sz, p, sz, P(sz)]),
+    "stride_decoder_create_sblb": (C.c_int, [p, sz, P(p)]),
     "stride_decoder_destroy": (None, [p]),
     "stride_dprb_from_dpr4": (C.c_int, [p, sz, p, sz, P(sz)]),
     "stride_decoder_create_dprb": (C.c_int, [p, sz, P(p)]),

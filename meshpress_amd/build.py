@@ -29,7 +29,7 @@ ARCH = "gfx950"
 
 HIP_SOURCES = ["stride_decode.hip", "stride_search.hip", "stride_capi.hip", "stride_mlp.hip",
                "pdlt_decode.hip"]
-CXX_SOURCES = ["stride_host.cpp", "stride_dprb.cpp"]
+CXX_SOURCES = ["stride_host.cpp", "stride_dprb.cpp", "pdlt_sblb.cpp"]
 
 HIP_FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}", "-ffp-contract=off",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

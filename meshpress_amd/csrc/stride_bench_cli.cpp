@@ -1,10 +1,13 @@
 // Native decode bench over the C ABI (the counterpart of the reference's
-// bench_cpp/stride_dup_decode_bench.cu command line): loads a DPR4 v3 stream
-// or a DPRB v4 blob, decodes it repeatedly into device buffers and prints
-// one JSON line with the event-timed decode rate.
+// bench_cpp/stride_dup_decode_bench.cu and stride_decode_bench.cu command
+// lines): loads a DPR4 v3 stream, a DPRB v4 blob, a PDLT v5 stream or an
+// SBLB v1 blob, decodes it repeatedly into device buffers and prints one
+// JSON line with the event-timed decode rate.
 //
 //   stride_bench FILE [--iters N] [--warmup W] [--mode M] [--group G]
-//                [--lds BYTES] [--grid R] [--queue 0|1]
+//                [--lds BYTES] [--grid R] [--queue 0|1] [--chain 64|32]
+//
+// PDLT v5 uses --group, --lds and --chain (float64 or float32 chain).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -37,7 +40,7 @@ int main(int argc, char** argv) {
                          "[--lds BYTES] [--grid R] [--queue 0|1]\n", argv[0]);
     return 2;
   }
-  int iters = 100, warmup = 10;
+  int iters = 100, warmup = 10, chain = 64;
   stride_tune want{};
   bool set_mode = false, set_group = false, set_lds = false, set_grid = false, set_queue = false;
   for (int i = 2; i + 1 < argc; i += 2) {
@@ -50,6 +53,7 @@ int main(int argc, char** argv) {
     else if (k == "--lds") { want.lds_bytes = v; set_lds = true; }
     else if (k == "--grid") { want.grid = v; set_grid = true; }
     else if (k == "--queue") { want.use_queue = v; set_queue = true; }
+    else if (k == "--chain") chain = v;
     else { std::fprintf(stderr, "stride_bench: unknown option %s\n", k.c_str()); return 2; }
   }
   if (iters < 1 || warmup < 0) { std::fprintf(stderr, "stride_bench: bad iteration counts\n"); return 2; }
@@ -61,12 +65,18 @@ int main(int argc, char** argv) {
   while ((r = std::fread(buf, 1, sizeof buf, f)) > 0) data.insert(data.end(), buf, buf + r);
   std::fclose(f);
   const bool dprb = data.size() >= 4 && std::memcmp(data.data(), "DPRB", 4) == 0;
+  const bool sblb = data.size() >= 4 && std::memcmp(data.data(), "SBLB", 4) == 0;
   int rc = stride_init(0);
   if (rc) return die("stride_init", rc);
   stride_decoder* h = nullptr;
   rc = dprb ? stride_decoder_create_dprb(data.data(), data.size(), &h)
-            : stride_decoder_create(data.data(), data.size(), &h);
+            : sblb ? stride_decoder_create_sblb(data.data(), data.size(), &h)
+                   : stride_decoder_create(data.data(), data.size(), &h);
   if (rc) return die("decoder create", rc);
+  int32_t version = 3;
+  stride_decoder_format(h, &version, nullptr, nullptr);
+  if ((rc = stride_decoder_set_precision(h, chain))) return die("set_precision", rc);
+  const char* fmt = dprb ? "dprb" : sblb ? "sblb" : version == 5 ? "pdlt_v5" : "dpr4";
   stride_tune t{};
   stride_decoder_get_tune(h, &t);
   if (set_mode) t.mode = want.mode;
@@ -102,10 +112,10 @@ int main(int argc, char** argv) {
   stride_decoder_get_tune(h, &t);
   std::printf("{\"file\": \"%s\", \"format\": \"%s\", \"n_tris\": %llu, \"n_verts\": %llu, "
               "\"meshlets\": %u, \"iters\": %d, \"ms\": %.6f, \"gtris_s\": %.4f, \"hbm_gb_s\": %.2f, "
-              "\"tune\": {\"mode\": %d, \"group_size\": %d, \"lds_bytes\": %d, \"grid\": %d, "
-              "\"use_queue\": %d}}\n",
-              argv[1], dprb ? "dprb" : "dpr4", (unsigned long long)n_t, (unsigned long long)n_v,
-              n_m, iters, ms, (double)n_t / (ms * 1e-3) / 1e9, bytes / (ms * 1e-3) / 1e9, t.mode,
+              "\"chain\": %d, \"tune\": {\"mode\": %d, \"group_size\": %d, \"lds_bytes\": %d, "
+              "\"grid\": %d, \"use_queue\": %d}}\n",
+              argv[1], fmt, (unsigned long long)n_t, (unsigned long long)n_v,
+              n_m, iters, ms, (double)n_t / (ms * 1e-3) / 1e9, bytes / (ms * 1e-3) / 1e9, chain, t.mode,
               t.group_size, t.lds_bytes, t.grid, t.use_queue);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);

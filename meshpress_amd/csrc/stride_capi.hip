@@ -706,6 +706,32 @@ int stride_decoder_create_dprb(const uint8_t* blob, size_t len, stride_decoder**
   return STRIDE_OK;
 }
 
+int stride_sblb_from_pdlt(const uint8_t* data, size_t len, uint8_t* out, size_t cap,
+                          size_t* out_len) {
+  if (!out_len || (!data && len)) return fail(STRIDE_E_ARG, "null argument");
+  std::vector<uint8_t> blob;
+  std::string err;
+  if (int rc = sblb_from_pdlt(data, len, blob, err)) return fail(rc, err);
+  *out_len = blob.size();
+  if (out) {
+    if (cap < blob.size()) return fail(STRIDE_E_ARG, "output buffer too small");
+    std::memcpy(out, blob.data(), blob.size());
+  }
+  return STRIDE_OK;
+}
+
+int stride_decoder_create_sblb(const uint8_t* blob, size_t len, stride_decoder** out) {
+  if (!out) return fail(STRIDE_E_ARG, "null out");
+  *out = nullptr;
+  if (!blob && len) return fail(STRIDE_E_ARG, "null blob");
+  std::string err;
+  if (int rc = sblb_check(blob, len, err)) return fail(rc, err);
+  const uint8_t* s = nullptr;
+  size_t sl = 0;
+  sblb_split(blob, len, s, sl, err);
+  return stride_decoder_create(s, sl, out);
+}
+
 int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
                          uint32_t* n_meshlets) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");

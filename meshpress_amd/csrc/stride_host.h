@@ -154,6 +154,15 @@ void v5_samples(const Prep& P, const int64_t* codes, const double* vn, const V5H
 std::vector<uint8_t> write_v5(const Prep& P, const int64_t* codes, const double* vn,
                               const V5Header& H, const float* lin3, int n_threads);
 
+// SBLB v1 blob (PDLT v5 stream + the reference bench's tables) <-> PDLT v5
+// (pdlt_sblb.cpp).  Return 0 or a negative status with a message in err.
+int sblb_from_pdlt(const uint8_t* data, size_t len, std::vector<uint8_t>& out, std::string& err);
+// The embedded stream of a blob whose size matches its header.
+int sblb_split(const uint8_t* blob, size_t len, const uint8_t*& stream, size_t& stream_len,
+               std::string& err);
+// sblb_split plus: every table equals what the pre-pass derives from the stream.
+int sblb_check(const uint8_t* blob, size_t len, std::string& err);
+
 // tagk: per meshlet 12 bytes: (tag,k) x 3 delta then (tag,k) x 3 para.
 std::vector<uint8_t> write_dup(const Prep& P, const Residuals& R, const Header& H,
                                const uint8_t* tagk);

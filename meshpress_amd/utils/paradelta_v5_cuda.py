@@ -4,8 +4,9 @@
 and keyword, and decodes the DPR4 v3 bytes that `STRIDEEncoder.encode()`
 returns (or the DPRB side-table blob of utils/dprb.py), and the PDLT v5
 streams the reference class itself decodes (encoder/paradelta_v5.py;
-written here by encoder.paradelta_v5.encode_from_prepared_v5), told apart by
-their magic, on MI355X through libstride_hip.so:
+written here by encoder.paradelta_v5.encode_from_prepared_v5; or their SBLB
+blob of utils/sblb.py), told apart by their magic, on MI355X through
+libstride_hip.so:
 
 * construction uploads the stream, runs the index pre-pass (a kernel for
   DPR4, host threads for PDLT v5) that derives the per-meshlet side table,
@@ -63,7 +64,8 @@ class ParaDeltaV5GpuDecoder:
         h = C.c_void_p()
         buf = np.frombuffer(self.data, dtype=np.uint8)
         _lib.check(L.stride_init(self.device.index))
-        create = L.stride_decoder_create_dprb if self.data[:4] == b"DPRB" else L.stride_decoder_create
+        create = {b"DPRB": L.stride_decoder_create_dprb,
+                  b"SBLB": L.stride_decoder_create_sblb}.get(self.data[:4], L.stride_decoder_create)
         _lib.check(create(_lib.ptr(buf) if len(buf) else None, len(buf), C.byref(h)))
         self._h = h
         nv, nt, nm = C.c_uint64(), C.c_uint64(), C.c_uint32()

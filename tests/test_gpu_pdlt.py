@@ -138,3 +138,39 @@ def test_large_mesh_error_bound_and_faces(n):
     assert np.array_equal(got[np.lexsort(got.T[::-1])], want[np.lexsort(want.T[::-1])])
     V2, T2 = decoder(data, group_size=3, lds_bytes=12288).decode_to_host()
     assert np.array_equal(V2.view(np.uint32), Vg.view(np.uint32)) and np.array_equal(T2, Tg)
+
+
+def test_sblb_blobs_decode_like_pdlt():
+    from meshpress_amd.utils.sblb import to_sblb
+    for path in GOLD:
+        data = np.load(path)["data"].tobytes()
+        Vo, To = PO.decode_v5(data)
+        print(f"sblb {path.stem}", flush=True)
+        assert_same(*decoder(to_sblb(data)).decode_to_host(), Vo, To)
+    blob = bytearray(to_sblb(np.load(GOLD[0])["data"].tobytes()))
+    blob[-1] ^= 0x40                               # last boundary coordinate
+    with pytest.raises(ValueError, match="SBLB"):
+        decoder(bytes(blob))
+
+
+def test_native_bench_cli_pdlt(tmp_path):
+    """The native bench decodes a PDLT v5 stream and its SBLB blob with both
+    chains and reports the event-timed rate."""
+    import json
+    import subprocess
+    from meshpress_amd import build
+    from meshpress_amd.utils.sblb import to_sblb
+    if not build.BENCH.exists():
+        pytest.skip("native bench not built")
+    data = np.load([p for p in GOLD if p.stem == "pdlt_uv_c1"][0])["data"].tobytes()
+    (tmp_path / "c1.pdlt").write_bytes(data)
+    (tmp_path / "c1.sblb").write_bytes(to_sblb(data))
+    for name, fmt in (("c1.pdlt", "pdlt_v5"), ("c1.sblb", "sblb")):
+        for chain in ("64", "32"):
+            r = subprocess.run([str(build.BENCH), str(tmp_path / name), "--iters", "5", "--warmup", "1",
+                                "--group", "8", "--lds", "24576", "--chain", chain],
+                               capture_output=True, text=True, timeout=120)
+            assert r.returncode == 0, r.stderr
+            out = json.loads(r.stdout)
+            assert out["format"] == fmt and out["n_tris"] == 65536 and out["ms"] > 0
+            assert out["chain"] == int(chain) and out["tune"]["group_size"] == 8

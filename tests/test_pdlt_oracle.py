@@ -279,3 +279,74 @@ def test_f32_chain_reproduces_the_encoder_reconstruction(q16):
     # the encoder rounds to the grid against its float32 reconstruction:
     # half a step (2 pce) plus float32 rounding of coordinates of size ~1
     assert np.abs(Vw.astype(np.float64) - V[ids]).max() <= step + 4 * 2.0 ** -24 * p["scale"] * 4
+
+
+def _meshlet_counts(data):
+    h = PO.read_header(data)
+    rows = []
+    for off in h["offsets"]:
+        r = PO.BitReader(data, off)
+        rows.append([r.get(16) for _ in range(4)])
+    return h, np.array(rows, dtype=np.int64).reshape(-1, 4)
+
+
+@pytest.mark.parametrize("path", PDLT_GOLDEN, ids=lambda p: p.stem)
+def test_sblb_writer_layout(path):
+    """The SBLB v1 blob (scripts/dump_stride_blob.py:1-114 layout) holds the
+    stream and the tables the reference bench uploads, each recomputed here
+    from the oracle's parse."""
+    from meshpress_amd.utils.sblb import to_sblb
+    data = np.load(path)["data"].tobytes()
+    blob = to_sblb(data)
+    h, cnt = _meshlet_counts(data)
+    n_m, n_b = h["n_meshlets"], h["n_boundary"]
+    V, T = PO.decode_v5(data)
+    hdr = struct.unpack("<8I4f4I", blob[:64])
+    assert hdr[0] == 0x424C4253 and hdr[1] == 1
+    assert hdr[2] == h["code_width"] and hdr[3] == n_m and hdr[4] == n_b
+    assert hdr[5] == len(V) == n_b + cnt[:, 1].sum() and hdr[6] == len(T) and hdr[7] == len(data)
+    assert np.array_equal(np.float32(hdr[8:11]), np.float32(h["lin3"]))
+    assert np.float32(hdr[11]) == np.float32(2.0 * h["per_coord_err"])
+    code_bytes = 4 if h["code_width"] else 2
+    nl, nb, ni, nt = ((cnt[:, 0] + cnt[:, 1]).max(), cnt[:, 0].max(), cnt[:, 1].max(), cnt[:, 2].max()) \
+        if n_m else (0, 0, 0, 0)
+    warp = int(nl) * 16 + int(nb) * 4 + 32 + int(ni) * 3 * code_bytes + int(ni) * 8 + int(nt) * 3
+    assert hdr[12] == (warp + 15) & ~15 and hdr[13] == 1
+    p = 64
+    assert blob[p:p + len(data)] == data
+    p += len(data)
+    off = np.frombuffer(blob, np.uint64, n_m, p)
+    assert np.array_equal(off, np.array(h["offsets"], np.uint64))
+    p += 8 * n_m
+    for j in range(4):
+        assert np.array_equal(np.frombuffer(blob, np.int32, n_m, p), cnt[:, j])
+        p += 4 * n_m
+    tris_off = np.frombuffer(blob, np.int32, n_m + 1, p)
+    assert np.array_equal(tris_off, np.concatenate([[0], np.cumsum(cnt[:, 2])]))
+    p += 4 * (n_m + 1)
+    cur = np.frombuffer(blob, np.int32, n_m, p)
+    assert np.array_equal(cur, np.concatenate([[0], np.cumsum(cnt[:, 1])[:-1]]) if n_m else cur)
+    p += 4 * n_m
+    bnd = np.frombuffer(blob, np.float32, 3 * n_b, p).reshape(-1, 3)
+    want = PO.dequant(h["bnd_codes"], h["g_min"], h["g_range"], h["g_bits"]).astype(np.float32)
+    assert np.array_equal(bnd.view(np.uint32), want.view(np.uint32))
+    assert p + 12 * n_b == len(blob)
+
+
+def test_sblb_rejects_inconsistent_tables():
+    """Checked on the host before any device work: a table that disagrees
+    with the embedded stream is STRIDE_E_CORRUPT."""
+    import ctypes as C
+    from meshpress_amd import _lib
+    from meshpress_amd.utils.sblb import to_sblb
+    data = np.load(PDLT_GOLDEN[0])["data"].tobytes()
+    blob = bytearray(to_sblb(data))
+    L = _lib.lib()
+    for pos in (8, 20, 44, 64 + len(data) + 3, len(blob) - 2):
+        bad = np.frombuffer(bytes(blob[:pos]) + bytes([blob[pos] ^ 1]) + bytes(blob[pos + 1:]), np.uint8)
+        h = C.c_void_p()
+        rc = L.stride_decoder_create_sblb(_lib.ptr(bad), len(bad), C.byref(h))
+        assert rc == _lib.E_CORRUPT, (pos, rc, _lib.last_error())
+    short = np.frombuffer(bytes(blob[:-1]), np.uint8)
+    h = C.c_void_p()
+    assert L.stride_decoder_create_sblb(_lib.ptr(short), len(short), C.byref(h)) == -12
