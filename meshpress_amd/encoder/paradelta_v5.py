@@ -44,3 +44,32 @@ def encode_from_prepared_v5(prep: dict, *, verbose: bool = False, return_info: b
 def encode_v5(verts, tris, **kw) -> bytes:
     """prepare() + encode_from_prepared_v5 for a vertex / triangle array pair."""
     return encode_from_prepared_v5(prepare(verts, tris, **kw))
+
+
+def _main(argv=None) -> int:
+    """Writes a synthetic workload as a PDLT v5 stream (or its SBLB blob):
+
+        python -m meshpress_amd.encoder.paradelta_v5 --workload heightfield_1900 --bits 12 -o FILE
+    """
+    import argparse
+    from .. import workloads
+    ap = argparse.ArgumentParser(description=_main.__doc__)
+    ap.add_argument("--workload", default="heightfield_1900",
+                    choices=("heightfield_1900", "icosphere_l8", "uv_sphere_c1"))
+    ap.add_argument("--bits", type=int, default=12, help="bbox grid bits on the longest axis")
+    ap.add_argument("--sblb", action="store_true", help="write the SBLB v1 blob instead")
+    ap.add_argument("-o", "--out", required=True)
+    a = ap.parse_args(argv)
+    V, T = getattr(workloads, a.workload)()
+    data = encode_v5(V, T, precision_error=1.0 / ((1 << a.bits) - 1.5), precision_mode="bbox_frac")
+    if a.sblb:
+        from ..utils.sblb import to_sblb
+        data = to_sblb(data)
+    with open(a.out, "wb") as f:
+        f.write(data)
+    print(f"{a.out}: {len(data):,} bytes, {len(T):,} triangles")
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(_main())
