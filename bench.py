@@ -47,6 +47,8 @@ TUNE_CANDIDATES = ([dict(mode=2, group_size=gs, lds_bytes=lds, grid=0, use_queue
                     for gs in (8, 16) for lds in (16384, 24576, 32768, 40960, 49152)]
                    + [dict(mode=2, group_size=16, lds_bytes=24576, grid=g, use_queue=1)
                       for g in (1536, 3072)]
+                   + [dict(mode=4, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
+                      for gs, lds in ((16, 16384), (16, 32768), (8, 16384))]
                    + [dict(mode=1, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
                       for gs in (32, 64) for lds in (16384, 40960)])
 
@@ -156,7 +158,10 @@ def single_mesh(name, V, T, bits, steps, warmup, tune=True):
     return dec, data, r, prep
 
 
-PDLT_LAYOUTS = [(16, 32768), (16, 49152), (16, 65536), (8, 24576), (8, 16384), (4, 16384)]
+# PDLT v5 launch shapes: (mode, group_size, lds_bytes, token-pass LDS words)
+PDLT_LAYOUTS = [(4, 16, 24576, 4096), (4, 8, 24576, 4096), (4, 16, 16384, 4096),
+                (4, 16, 49152, 4096), (4, 16, 24576, 8192), (3, 16, 49152, 4096),
+                (3, 16, 24576, 4096)]
 
 
 def pdlt_v5(prep, name, steps, warmup):
@@ -175,17 +180,18 @@ def pdlt_v5(prep, name, steps, warmup):
     log(f"{name} v5: {len(data):,} bytes ({8 * len(data) / prep['n_v']:.2f} bpv), "
         f"{dec.n_boundary:,} boundary vertices, encode {enc_s:.1f}s, create {create_s:.2f}s")
     sweep = []
-    for gs, lds in PDLT_LAYOUTS:
-        dec.set_tune(group_size=gs, lds_bytes=lds)
+    for mode, gs, lds, sw in PDLT_LAYOUTS:
+        dec.set_tune(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
         _, g = time_calls(dec.decode_world, 10, 3)
-        sweep.append([gs, lds, round(g * 1e3, 4)])
-    gs, lds, _ = min(sweep, key=lambda x: x[2])
-    dec.set_tune(group_size=gs, lds_bytes=lds)
+        sweep.append([mode, gs, lds, sw, round(g * 1e3, 4)])
+    mode, gs, lds, sw, _ = min(sweep, key=lambda x: x[-1])
+    best = dict(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
+    dec.set_tune(**best)
     s_wall, s_gpu = time_calls(dec.decode_world, steps, warmup)
     v64, t64 = dec.decode_world()
     v64 = v64.clone()
     dec32 = ParaDeltaV5GpuDecoder(data, chain="f32")
-    dec32.set_tune(group_size=gs, lds_bytes=lds)
+    dec32.set_tune(**best)
     w32, g32 = time_calls(dec32.decode_world, steps, warmup)
     v32, t32 = dec32.decode_world()
     torch.cuda.synchronize()
@@ -196,7 +202,7 @@ def pdlt_v5(prep, name, steps, warmup):
          "meshlets": dec.n_meshlets, "n_boundary": dec.n_boundary, "stream_bytes": dec.stream_bytes,
          "side_bytes": dec.side_bytes, "bpv": 8 * len(data) / prep["n_v"],
          "lin3": [float(x) for x in info["lin3"]], "encode_s": enc_s, "create_s": create_s,
-         "tune": {"group_size": gs, "lds_bytes": lds}, "tune_sweep_ms": sweep,
+         "tune": best, "tune_sweep_ms": sweep,
          "ms_wall": s_wall * 1e3, "ms_kernels": s_gpu * 1e3, "gtris_s": dec.n_t_total / s_wall / 1e9,
          "hbm_gb_s": bytes_alg / s_gpu / 1e9, "algorithmic_bytes": bytes_alg,
          "f32_chain": {"ms_kernels": g32 * 1e3, "gtris_s": dec.n_t_total / w32 / 1e9,

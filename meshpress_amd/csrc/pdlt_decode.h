@@ -14,15 +14,22 @@ namespace stride {
 
 struct PdltSet {
   PdltHost H;
-  std::vector<PdltGroup> groups;
-  uint32_t lds_bytes = 0;               // dynamic LDS of the launch
+  std::vector<PdltGroup> groups;        // chain (or single-pass) groups
+  std::vector<PdltGroup> tok_groups;    // token groups of the two-pass decode
+  uint32_t lds_bytes = 0;               // dynamic LDS of the group launch
+  uint32_t tok_lds_bytes = 0;           // dynamic LDS of the token launch
   int group_size = 16;
-  int lds_budget = 32768;
+  int lds_budget = 24576;
+  int tok_lds_budget = 16384;
   int chain32 = 0;                      // 1: the encoder's float32 chain
+  int two_pass = 1;                     // 1: token pass + chain pass, 0: one kernel
   // device
   uint32_t* d_words = nullptr;
   PdltEntry* d_tab = nullptr;
   PdltGroup* d_groups = nullptr;
+  PdltGroup* d_tok_groups = nullptr;
+  uint32_t* d_walk = nullptr;           // two-pass walks (n_v - n_b words)
+  uint32_t* d_seen = nullptr;           // two-pass seen sets + walk lengths (kSeenWords per meshlet)
   double* d_bnd = nullptr;              // boundary table, normalised f64 (n_b x 3)
   float* d_v_own = nullptr;
   uint32_t* d_t_own = nullptr;
@@ -31,7 +38,7 @@ struct PdltSet {
   // Parses, validates and uploads.  STRIDE_OK or a STRIDE_E_* status with
   // the message set.
   int build(const uint8_t* data, size_t len);
-  int set_layout(int group_size, int lds_budget);
+  int set_layout(int group_size, int lds_budget, int tok_lds_budget);
   // world = 1: world coordinates; 0: normalised.
   int decode(float* d_verts, uint32_t* d_tris, int world, hipStream_t stream);
   int own_outputs();

@@ -33,31 +33,20 @@ __global__ __launch_bounds__(256) void pdlt_boundary_kernel(const double* __rest
   }
 }
 
-__global__ __launch_bounds__(64) void pdlt_decode_kernel(
-    const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
-    const PdltGroup* __restrict__ groups, const double* __restrict__ bnd, PdltParams P,
-    float* __restrict__ V, uint32_t* __restrict__ T) {
-  extern __shared__ uint64_t lds64[];
-  uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
-  const int lane = threadIdx.x;
-  const int slot = lane & 15, role = lane >> 4;
-  const PdltGroup G = groups[blockIdx.x];
-  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
-  const uint32_t m = G.m0 + (uint32_t)slot;
-  const bool active = m < G.m1;
-  PdltEntry e = {};
-  if (active) e = tab[m];
+// Phases B-D of a group (after the walk, seen set, walk length and refs of
+// every meshlet are in its LDS block): boundary gather, axis chains, output.
+__device__ __forceinline__ void pdlt_group_tail(const uint32_t* __restrict__ lds_c, uint32_t* lds,
+                                                const PdltGroup& G, const PdltEntry& e, bool active,
+                                                int role, int lane, int64_t rel,
+                                                const PdltEntry* __restrict__ tab,
+                                                const double* __restrict__ bnd, const PdltParams& P,
+                                                float* __restrict__ V) {
   const uint32_t nl = e.n_bnd + e.n_int;
   uint32_t* blk = lds + G.blk_base + e.blk_off;
   double* recon = reinterpret_cast<double*>(blk);
-  uint32_t* refs = blk + 6 * nl;
-  uint32_t* walk = refs + e.n_bnd;
-  uint32_t* seen = walk + e.n_int;
-  uint32_t* n_emit = seen + 8;
-  const int64_t rel = -32 * (int64_t)G.w_lo;
-  __syncthreads();
-  if (active && role == 0) pdlt_tokens(lds, rel, e, refs, walk, seen, n_emit, T + 3ull * e.t_off);
-  __syncthreads();
+  const uint32_t* refs = blk + 6 * nl;
+  const uint32_t* walk = refs + e.n_bnd;
+  const uint32_t* n_emit = walk + e.n_int + 8;
   // boundary reconstructions: the four lanes of a slot gather its refs
   if (active) {
     const uint32_t nb3 = 3 * e.n_bnd;
@@ -83,7 +72,7 @@ __global__ __launch_bounds__(64) void pdlt_decode_kernel(
   }
   __syncthreads();
   if (active && role > 0)
-    pdlt_axis(lds, rel, e, role - 1, P, recon + (uint32_t)(role - 1) * nl, walk, *n_emit);
+    pdlt_axis(lds_c, rel, e, role - 1, P, recon + (uint32_t)(role - 1) * nl, walk, *n_emit);
   __syncthreads();
   // interior vertices, one meshlet at a time, 64 coordinates per step
   for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
@@ -100,6 +89,90 @@ __global__ __launch_bounds__(64) void pdlt_decode_kernel(
       vo[t] = pdlt_out(reached ? rj[d * nlj + v] : 0.0, P, (int)d);
     }
   }
+}
+
+// Single-pass decode: the token lanes run inside the group.
+__global__ __launch_bounds__(64) void pdlt_decode_kernel(
+    const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
+    const PdltGroup* __restrict__ groups, const double* __restrict__ bnd, PdltParams P,
+    float* __restrict__ V, uint32_t* __restrict__ T) {
+  extern __shared__ uint64_t lds64[];
+  uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
+  const int lane = threadIdx.x;
+  const int slot = lane & 15, role = lane >> 4;
+  const PdltGroup G = groups[blockIdx.x];
+  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
+  const uint32_t m = G.m0 + (uint32_t)slot;
+  const bool active = m < G.m1;
+  PdltEntry e = {};
+  if (active) e = tab[m];
+  const uint32_t nl = e.n_bnd + e.n_int;
+  uint32_t* blk = lds + G.blk_base + e.blk_off;
+  uint32_t* refs = blk + 6 * nl;
+  uint32_t* walk = refs + e.n_bnd;
+  uint32_t* seen = walk + e.n_int;
+  const int64_t rel = -32 * (int64_t)G.w_lo;
+  __syncthreads();
+  if (active && role == 0) pdlt_tokens(lds, rel, e, refs, walk, seen, seen + 8, T + 3ull * e.t_off);
+  __syncthreads();
+  pdlt_group_tail(lds, lds, G, e, active, role, lane, rel, tab, bnd, P, V);
+}
+
+// Two-pass decode, pass 1: one token lane per meshlet, up to 64 meshlets
+// per wave (refs in LDS for the global triangle ids); the walk, seen set
+// and walk length go to HBM.
+__global__ __launch_bounds__(64) void pdlt_tokens_kernel(
+    const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
+    const PdltGroup* __restrict__ groups, uint32_t n_b, uint32_t* __restrict__ walk_g,
+    uint32_t* __restrict__ seen_g, uint32_t* __restrict__ T) {
+  extern __shared__ uint64_t lds64[];
+  uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
+  const int lane = threadIdx.x;
+  const PdltGroup G = groups[blockIdx.x];
+  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
+  __syncthreads();
+  const uint32_t m = G.m0 + (uint32_t)lane;
+  if (m < G.m1) {
+    const PdltEntry e = tab[m];
+    uint32_t* seen = seen_g + (uint64_t)kSeenWords * m;
+    pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, lds + G.blk_base + e.tok_ref_off,
+                walk_g + (e.int_base - n_b), seen, seen + 8, T + 3ull * e.t_off);
+  }
+}
+
+// Two-pass decode, pass 2: the group's blocks take the walk, seen set and
+// walk length from HBM and the refs decoded again, then phases B-D.
+__global__ __launch_bounds__(64) void pdlt_chain_kernel(
+    const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
+    const PdltGroup* __restrict__ groups, const double* __restrict__ bnd, PdltParams P,
+    uint32_t n_b, const uint32_t* __restrict__ walk_g, const uint32_t* __restrict__ seen_g,
+    float* __restrict__ V) {
+  extern __shared__ uint64_t lds64[];
+  uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
+  const int lane = threadIdx.x;
+  const int slot = lane & 15, role = lane >> 4;
+  const PdltGroup G = groups[blockIdx.x];
+  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
+  const uint32_t m = G.m0 + (uint32_t)slot;
+  const bool active = m < G.m1;
+  PdltEntry e = {};
+  if (active) e = tab[m];
+  const uint32_t nl = e.n_bnd + e.n_int;
+  uint32_t* blk = lds + G.blk_base + e.blk_off;
+  uint32_t* refs = blk + 6 * nl;
+  uint32_t* walk = refs + e.n_bnd;
+  uint32_t* seen = walk + e.n_int;
+  const int64_t rel = -32 * (int64_t)G.w_lo;
+  if (active) {
+    const uint32_t* ws = walk_g + (e.int_base - n_b);
+    for (uint32_t i = (uint32_t)role; i < e.n_int; i += 4) walk[i] = ws[i];
+    const uint32_t* ss = seen_g + (uint64_t)kSeenWords * m;
+    for (uint32_t i = (uint32_t)role; i < kSeenWords; i += 4) seen[i] = ss[i];
+  }
+  __syncthreads();
+  if (active && role == 0) pdlt_refs(lds, rel, e, refs);
+  __syncthreads();
+  pdlt_group_tail(lds, lds, G, e, active, role, lane, rel, tab, bnd, P, V);
 }
 
 // ---------------------------------------------------------------------------
@@ -119,17 +192,26 @@ PdltSet::~PdltSet() {
   (void)hipFree(d_bnd);
   (void)hipFree(d_v_own);
   (void)hipFree(d_t_own);
+  (void)hipFree(d_tok_groups);
+  (void)hipFree(d_walk);
+  (void)hipFree(d_seen);
 }
 
 size_t PdltSet::side_bytes() const {
   return H.tab.size() * sizeof(PdltEntry) + groups.size() * sizeof(PdltGroup) +
-         H.bnd.size() * sizeof(double);
+         H.bnd.size() * sizeof(double) +
+         (two_pass ? tok_groups.size() * sizeof(PdltGroup) : 0);
 }
 
 int PdltSet::build(const uint8_t* d, size_t len) {
   std::string err;
   int rc = pdlt_parse(d, len, H, err);
   if (rc) return set_error(rc, err);
+  // two-pass scratch: the walks (one word per interior vertex) and the seen
+  // sets with walk lengths
+  const uint64_t n_int = H.n_v - H.n_b;
+  PDLT_HIP(hipMalloc(&d_walk, 4 * std::max<uint64_t>(1, n_int)));
+  PDLT_HIP(hipMalloc(&d_seen, 4ull * kSeenWords * std::max<uint32_t>(1, H.n_m)));
   PDLT_HIP(hipMalloc(&d_words, H.words.size() * 4));
   PDLT_HIP(hipMemcpy(d_words, H.words.data(), H.words.size() * 4, hipMemcpyHostToDevice));
   if (!H.bnd.empty()) {
@@ -137,28 +219,39 @@ int PdltSet::build(const uint8_t* d, size_t len) {
     PDLT_HIP(hipMemcpy(d_bnd, H.bnd.data(), H.bnd.size() * sizeof(double), hipMemcpyHostToDevice));
   }
   if (!H.tab.empty()) PDLT_HIP(hipMalloc(&d_tab, H.tab.size() * sizeof(PdltEntry)));
-  return set_layout(group_size, lds_budget);
+  return set_layout(group_size, lds_budget, tok_lds_budget);
 }
 
-int PdltSet::set_layout(int gs, int budget) {
+int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   std::string err;
-  std::vector<PdltGroup> gr;
-  uint32_t lds = 0;
+  std::vector<PdltGroup> gr, tg;
+  uint32_t lds = 0, tlds = 0;
   std::vector<PdltEntry> tab = H.tab;
   int rc = pdlt_layout(tab, gs, budget, gr, lds, err);
+  if (!rc) rc = pdlt_layout_tokens(tab, tok_budget, tg, tlds, err);
   if (rc) return set_error(rc, err);
   H.tab.swap(tab);
   groups.swap(gr);
+  tok_groups.swap(tg);
   lds_bytes = lds;
+  tok_lds_bytes = tlds;
   group_size = gs;
   lds_budget = budget;
+  tok_lds_budget = tok_budget;
   (void)hipFree(d_groups);
   d_groups = nullptr;
+  (void)hipFree(d_tok_groups);
+  d_tok_groups = nullptr;
   if (!H.tab.empty())
     PDLT_HIP(hipMemcpy(d_tab, H.tab.data(), H.tab.size() * sizeof(PdltEntry), hipMemcpyHostToDevice));
   if (!groups.empty()) {
     PDLT_HIP(hipMalloc(&d_groups, groups.size() * sizeof(PdltGroup)));
     PDLT_HIP(hipMemcpy(d_groups, groups.data(), groups.size() * sizeof(PdltGroup), hipMemcpyHostToDevice));
+  }
+  if (!tok_groups.empty()) {
+    PDLT_HIP(hipMalloc(&d_tok_groups, tok_groups.size() * sizeof(PdltGroup)));
+    PDLT_HIP(hipMemcpy(d_tok_groups, tok_groups.data(), tok_groups.size() * sizeof(PdltGroup),
+                       hipMemcpyHostToDevice));
   }
   return STRIDE_OK;
 }
@@ -180,11 +273,18 @@ int PdltSet::decode(float* V, uint32_t* T, int world, hipStream_t s) {
     hipLaunchKernelGGL(pdlt_boundary_kernel, dim3(grid), dim3(256), 0, s, d_bnd, n3, p, V);
     PDLT_HIP(hipGetLastError());
   }
-  if (!groups.empty()) {
+  if (groups.empty()) return STRIDE_OK;
+  if (two_pass) {
+    hipLaunchKernelGGL(pdlt_tokens_kernel, dim3((uint32_t)tok_groups.size()), dim3(64), tok_lds_bytes,
+                       s, d_words, d_tab, d_tok_groups, H.n_b, d_walk, d_seen, T);
+    PDLT_HIP(hipGetLastError());
+    hipLaunchKernelGGL(pdlt_chain_kernel, dim3((uint32_t)groups.size()), dim3(64), lds_bytes, s,
+                       d_words, d_tab, d_groups, d_bnd, p, H.n_b, d_walk, d_seen, V);
+  } else {
     hipLaunchKernelGGL(pdlt_decode_kernel, dim3((uint32_t)groups.size()), dim3(64), lds_bytes, s,
                        d_words, d_tab, d_groups, d_bnd, p, V, T);
-    PDLT_HIP(hipGetLastError());
   }
+  PDLT_HIP(hipGetLastError());
   return STRIDE_OK;
 }
 

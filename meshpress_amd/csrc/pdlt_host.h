@@ -251,4 +251,39 @@ inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vec
   return STRIDE_OK;
 }
 
+// Token groups of the two-pass decode: up to 64 consecutive meshlets per
+// wave (one token lane each) whose staged words and refs fit `budget` bytes.
+// Sets each entry's tok_ref_off; lds = the largest group's LDS bytes.
+inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, std::vector<PdltGroup>& groups,
+                              uint32_t& lds, std::string& err) {
+  if (budget < 4096 || budget > 65536) { err = "lds_bytes must be in [4096, 65536]"; return STRIDE_E_ARG; }
+  groups.clear();
+  uint32_t max_lds = 0;
+  const uint32_t n_m = (uint32_t)tab.size();
+  uint32_t m = 0;
+  while (m < n_m) {
+    PdltGroup g{};
+    g.m0 = m;
+    g.w_lo = tab[m].start_bit >> 5;
+    uint32_t refs = 0, words = 0, j = m;
+    while (j < n_m && j - m < 64) {
+      const uint32_t w = ((((tab[j].end_bit - 1) >> 5) + 2 - g.w_lo) + 1) & ~1u;
+      if (j > m && 4ull * (w + refs + tab[j].n_bnd) > (uint64_t)budget) break;
+      tab[j].tok_ref_off = refs;
+      refs += tab[j].n_bnd;
+      words = w;
+      ++j;
+    }
+    g.m1 = j;
+    g.n_words = words;
+    g.blk_base = words;
+    max_lds = std::max(max_lds, 4 * (words + refs));
+    groups.push_back(g);
+    m = j;
+  }
+  if (max_lds > 65536) { err = "a meshlet needs more than 64 KiB of LDS"; return STRIDE_E_UNSUPPORTED; }
+  lds = max_lds;
+  return STRIDE_OK;
+}
+
 }  // namespace stride

@@ -49,7 +49,8 @@ struct PdltEntry {
   uint32_t int_base;     // output index of the first interior vertex
   uint32_t t_off;        // first output triangle
   uint32_t blk_off;      // LDS block, 32-bit words from the group's block base
-  uint32_t pad[3];
+  uint32_t tok_ref_off;  // two-pass layout: refs in the token group's LDS, words
+  uint32_t pad[2];
 };
 static_assert(sizeof(PdltEntry) == 64, "PdltEntry is one 64-byte line");
 
@@ -62,6 +63,11 @@ struct PdltGroup {
   uint32_t pad[3];
 };
 static_assert(sizeof(PdltGroup) == 32, "PdltGroup is 32 bytes");
+
+// Two-pass layout: the walk of meshlet m (n_int words) goes to the walk
+// buffer at int_base - n_boundary; its seen set (8 words) and walk length
+// (1 word) to the seen buffer at kSeenWords * m.
+constexpr uint32_t kSeenWords = 9;
 
 // Decode constants, widened to float64 as the reference decoder does.
 struct PdltParams {
@@ -282,6 +288,25 @@ STRIDE_DEV uint32_t v5_eg(const uint32_t* __restrict__ st, uint32_t& p, uint32_t
   return (shifted << k) | low;
 }
 
+// Boundary refs of a meshlet into refs[] (Rice-coded differences minus one).
+STRIDE_DEV void pdlt_refs(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e,
+                          uint32_t* __restrict__ refs) {
+  const uint32_t n_bnd = e.n_bnd;
+  uint32_t p = (uint32_t)((int64_t)e.start_bit + rel) + 64u;
+  if (n_bnd == 0) return;
+  uint32_t prev = v5_bits(st, p, 32);
+  p += 32;
+  refs[0] = prev;
+  if (n_bnd > 1) {
+    const uint32_t k = v5_bits(st, p, 8);
+    p += 8;
+    for (uint32_t i = 1; i < n_bnd; ++i) {
+      prev = prev + v5_rice(st, p, k) + 1u;
+      refs[i] = prev;
+    }
+  }
+}
+
 // Token lane: boundary refs into refs[], strips into the walk (interior
 // locals in first-use order: v * 0x01010101 for a strip-root vertex,
 // v | a << 8 | b << 16 | c << 24 for a parallelogram step, a != v) and the
@@ -292,20 +317,7 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
                             uint32_t* __restrict__ seen_out, uint32_t* __restrict__ n_emit,
                             uint32_t* __restrict__ tout) {
   const uint32_t n_bnd = e.n_bnd, n_local = e.n_bnd + e.n_int;
-  uint32_t p = (uint32_t)((int64_t)e.start_bit + rel) + 64u;
-  if (n_bnd > 0) {
-    uint32_t prev = v5_bits(st, p, 32);
-    p += 32;
-    refs[0] = prev;
-    if (n_bnd > 1) {
-      const uint32_t k = v5_bits(st, p, 8);
-      p += 8;
-      for (uint32_t i = 1; i < n_bnd; ++i) {
-        prev = prev + v5_rice(st, p, k) + 1u;
-        refs[i] = prev;
-      }
-    }
-  }
+  pdlt_refs(st, rel, e, refs);
   const uint32_t n_tris = e.n_tris, n_strips = e.n_strips & kStripMask;
   const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
   uint32_t pc = (uint32_t)((int64_t)e.strips_bit + rel);

@@ -5,9 +5,11 @@
 // JSON line with the event-timed decode rate.
 //
 //   stride_bench FILE [--iters N] [--warmup W] [--mode M] [--group G]
-//                [--lds BYTES] [--grid R] [--queue 0|1] [--chain 64|32]
+//                [--lds BYTES] [--grid R] [--queue 0|1] [--stage WORDS] [--chain 64|32]
 //
-// PDLT v5 uses --group, --lds and --chain (float64 or float32 chain).
+// PDLT v5 uses --mode (3 one kernel, 4 token pass + chain pass), --group,
+// --lds, --stage (the token pass's LDS words) and --chain (float64 or
+// float32 chain).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -42,7 +44,8 @@ int main(int argc, char** argv) {
   }
   int iters = 100, warmup = 10, chain = 64;
   stride_tune want{};
-  bool set_mode = false, set_group = false, set_lds = false, set_grid = false, set_queue = false;
+  bool set_mode = false, set_group = false, set_lds = false, set_grid = false, set_queue = false,
+       set_stage = false;
   for (int i = 2; i + 1 < argc; i += 2) {
     const std::string k = argv[i];
     const int v = std::atoi(argv[i + 1]);
@@ -54,6 +57,7 @@ int main(int argc, char** argv) {
     else if (k == "--grid") { want.grid = v; set_grid = true; }
     else if (k == "--queue") { want.use_queue = v; set_queue = true; }
     else if (k == "--chain") chain = v;
+    else if (k == "--stage") { want.stage_words = v; set_stage = true; }
     else { std::fprintf(stderr, "stride_bench: unknown option %s\n", k.c_str()); return 2; }
   }
   if (iters < 1 || warmup < 0) { std::fprintf(stderr, "stride_bench: bad iteration counts\n"); return 2; }
@@ -84,6 +88,7 @@ int main(int argc, char** argv) {
   if (set_lds) t.lds_bytes = want.lds_bytes;
   if (set_grid) t.grid = want.grid;
   if (set_queue) t.use_queue = want.use_queue;
+  if (set_stage) t.stage_words = want.stage_words;
   if ((rc = stride_decoder_set_tune(h, &t))) return die("set_tune", rc);
   uint64_t n_v = 0, n_t = 0, sb = 0, side = 0;
   uint32_t n_m = 0;
@@ -113,10 +118,10 @@ int main(int argc, char** argv) {
   std::printf("{\"file\": \"%s\", \"format\": \"%s\", \"n_tris\": %llu, \"n_verts\": %llu, "
               "\"meshlets\": %u, \"iters\": %d, \"ms\": %.6f, \"gtris_s\": %.4f, \"hbm_gb_s\": %.2f, "
               "\"chain\": %d, \"tune\": {\"mode\": %d, \"group_size\": %d, \"lds_bytes\": %d, "
-              "\"grid\": %d, \"use_queue\": %d}}\n",
+              "\"grid\": %d, \"use_queue\": %d, \"stage_words\": %d}}\n",
               argv[1], fmt, (unsigned long long)n_t, (unsigned long long)n_v,
               n_m, iters, ms, (double)n_t / (ms * 1e-3) / 1e9, bytes / (ms * 1e-3) / 1e9, chain, t.mode,
-              t.group_size, t.lds_bytes, t.grid, t.use_queue);
+              t.group_size, t.lds_bytes, t.grid, t.use_queue, t.stage_words);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);

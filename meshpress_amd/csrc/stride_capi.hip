@@ -146,6 +146,12 @@ struct DecodeSet {
   uint32_t n_groups = 0;
   std::vector<uint32_t> g_base;           // first group of each mesh, n_meshes + 1
   uint32_t* d_legacy = nullptr;           // meshlets decoded by the wave-per-meshlet kernel
+  // mode 4: token groups (up to 64 meshlets, unions of consecutive split
+  // groups) and the walks, one word per local vertex
+  GroupEntry* d_tgroups = nullptr;
+  uint32_t n_tgroups = 0, tlds_bytes = 0;
+  std::vector<uint32_t> tg_base;          // first token group of each mesh, n_meshes + 1
+  uint32_t* d_walk = nullptr;
   uint32_t n_legacy = 0;
   std::vector<uint32_t> l_base;           // first legacy entry of each mesh
   // captured graphs keyed by (verts, tris, tune)
@@ -163,6 +169,8 @@ struct DecodeSet {
     (void)hipFree(d_t_own);
     (void)hipFree(d_groups);
     (void)hipFree(d_legacy);
+    (void)hipFree(d_tgroups);
+    (void)hipFree(d_walk);
   }
 
   void drop_graphs() {
@@ -314,9 +322,9 @@ struct DecodeSet {
     for (int pass = 0; pass < 2; ++pass) {
       groups.clear();
       g_base.assign(n_meshes + 1, 0);
-      if (cfg.mode == 1 || cfg.mode == 2) {
+      if (cfg.mode == 1 || cfg.mode == 2 || cfg.mode == 4) {
         const uint64_t budget = (uint64_t)cfg.lds_bytes;
-        const bool split = cfg.mode == 2;
+        const bool split = cfg.mode != 1;
         const bool nar = narrow();
         const uint32_t gmax = split ? std::min(cfg.group_size, 16) : cfg.group_size;
         for (int i = 0; i < n_meshes; ++i) {
@@ -374,12 +382,16 @@ struct DecodeSet {
         HIP_TRY(hipMemcpy(d_groups, groups.data(), sizeof(GroupEntry) * n_groups,
                           hipMemcpyHostToDevice));
       }
+      if (cfg.mode == 4) {
+        int rc = token_layout(groups);
+        if (rc) return rc;
+      }
       if (pass == 1 || n_groups == 0) break;
       // validation decode of the groups
       uint32_t* d_flags = nullptr;
       HIP_TRY(hipMalloc(&d_flags, 4 * n_groups));
       hipError_t e = hipMemset(d_flags, 0, 4 * n_groups);
-      GroupLaunch g = group_args(d_v_own, d_t_own, 0, n_groups, d_flags);
+      GroupLaunch g = group_args(d_v_own, d_t_own, 0, n_groups, d_flags, 0, n_tgroups);
       g.queue = nullptr;   // validation: one workgroup per group
       g.grid = 0;
       if (e == hipSuccess) e = launch_group(g, nullptr);
@@ -430,7 +442,52 @@ struct DecodeSet {
     return STRIDE_OK;
   }
 
-  GroupLaunch group_args(float* V, uint32_t* T, uint32_t g0, uint32_t g1, uint32_t* flags) const {
+  // mode 4 token groups: unions of consecutive split groups of one mesh, up
+  // to 64 meshlets and kTokenLds bytes of staged words
+  static constexpr uint64_t kTokenLds = 32768;
+  int token_layout(const std::vector<GroupEntry>& groups) {
+    const int n_meshes = (int)mp.size();
+    std::vector<GroupEntry> tg;
+    tg_base.assign(n_meshes + 1, 0);
+    uint32_t tl = 0;
+    for (int i = 0; i < n_meshes; ++i) {
+      tg_base[i] = (uint32_t)tg.size();
+      GroupEntry cur{};
+      bool open = false;
+      for (uint32_t gi = g_base[i]; gi < g_base[i + 1]; ++gi) {
+        const GroupEntry& G = groups[gi];
+        const uint64_t words = (uint64_t)G.w_lo + G.n_words - cur.w_lo;
+        if (open && G.m0 == cur.m1 && G.m1 - cur.m0 <= 64 && 4 * words <= kTokenLds) {
+          cur.m1 = G.m1;
+          cur.n_words = (uint32_t)words;
+          continue;
+        }
+        if (open) tg.push_back(cur);
+        cur = GroupEntry{};
+        cur.m0 = G.m0;
+        cur.m1 = G.m1;
+        cur.w_lo = G.w_lo;
+        cur.n_words = G.n_words;
+        open = true;
+      }
+      if (open) tg.push_back(cur);
+    }
+    tg_base[n_meshes] = (uint32_t)tg.size();
+    for (const GroupEntry& G : tg) tl = std::max(tl, 4 * G.n_words);
+    (void)hipFree(d_tgroups);
+    d_tgroups = nullptr;
+    n_tgroups = (uint32_t)tg.size();
+    tlds_bytes = tl;
+    if (n_tgroups) {
+      HIP_TRY(hipMalloc(&d_tgroups, sizeof(GroupEntry) * n_tgroups));
+      HIP_TRY(hipMemcpy(d_tgroups, tg.data(), sizeof(GroupEntry) * n_tgroups, hipMemcpyHostToDevice));
+    }
+    if (!d_walk) HIP_TRY(hipMalloc(&d_walk, 4 * std::max<uint64_t>(n_v, 1)));
+    return STRIDE_OK;
+  }
+
+  GroupLaunch group_args(float* V, uint32_t* T, uint32_t g0, uint32_t g1, uint32_t* flags,
+                         uint32_t tg0 = 0, uint32_t tg1 = 0) const {
     GroupLaunch g;
     g.buf = d_buf;
     g.tab = d_tab;
@@ -442,9 +499,15 @@ struct DecodeSet {
     g.overflow = flags;
     g.lds_bytes = (uint32_t)cfg.lds_bytes;
     g.grid = (uint32_t)cfg.grid;
-    g.split = cfg.mode == 2;
+    g.split = cfg.mode == 2 || cfg.mode == 4;
     g.narrow = narrow();
     g.queue = (cfg.mode == 2 && cfg.use_queue && cfg.grid > 0) ? d_queue : nullptr;
+    const bool two = cfg.mode == 4;
+    g.tgroups = two ? d_tgroups + tg0 : nullptr;
+    g.n_tgroups = two ? tg1 - tg0 : 0;
+    g.tlds_bytes = tlds_bytes;
+    g.walk = two ? d_walk : nullptr;
+    if (two) g.grid = 0;
     return g;
   }
 
@@ -477,10 +540,12 @@ struct DecodeSet {
 
   // Enqueue the decode of meshes [mesh0, mesh1).
   int enqueue(float* V, uint32_t* T, hipStream_t s, int mesh0, int mesh1) {
-    if (cfg.mode == 1 || cfg.mode == 2) {
+    if (cfg.mode == 1 || cfg.mode == 2 || cfg.mode == 4) {
       const uint32_t g0 = g_base[mesh0], g1 = g_base[mesh1];
       if (g1 > g0) {
-        GroupLaunch g = group_args(V, T, g0, g1, nullptr);
+        const bool two = cfg.mode == 4;
+        GroupLaunch g = group_args(V, T, g0, g1, nullptr, two ? tg_base[mesh0] : 0,
+                                   two ? tg_base[mesh1] : 0);
         if (g.queue) {   // a fill kernel, not a memset, so hipGraph capture stays clean
           hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_queue, 0u);
           HIP_TRY(hipGetLastError());
@@ -553,13 +618,13 @@ struct DecodeSet {
     if (!t) return fail(STRIDE_E_ARG, "null tune");
     bool ok_w = t->waves == 1 || t->waves == 2 || t->waves == 4;
     bool ok_s = t->stage_words == 256 || t->stage_words == 512 || t->stage_words == 1024;
-    bool ok_m = t->mode == 0 || t->mode == 1 || t->mode == 2;
-    bool ok_g = t->group_size >= 1 && t->group_size <= (t->mode == 2 ? 16 : 64);
+    bool ok_m = t->mode == 0 || t->mode == 1 || t->mode == 2 || t->mode == 4;
+    bool ok_g = t->group_size >= 1 && t->group_size <= (t->mode == 2 || t->mode == 4 ? 16 : 64);
     bool ok_l = t->lds_bytes >= 8192 && t->lds_bytes <= 65536 && t->lds_bytes % 8 == 0;
     if (!ok_w || !ok_s || !ok_m || !ok_g || !ok_l || t->grid < 0 || (t->use_queue && t->grid == 0))
       return fail(STRIDE_E_ARG,
-                  "tune: mode in {0,1,2}, waves in {1,2,4}, stage_words in {256,512,1024}, "
-                  "group_size in [1,64] ([1,16] for mode 2), lds_bytes in [8192,65536] "
+                  "tune: mode in {0,1,2,4}, waves in {1,2,4}, stage_words in {256,512,1024}, "
+                  "group_size in [1,64] ([1,16] for modes 2 and 4), lds_bytes in [8192,65536] "
                   "(multiple of 8), grid >= 0, "
                   "use_queue needs a persistent grid");
     const bool relayout = t->mode != cfg.mode || t->group_size != cfg.group_size ||
@@ -858,7 +923,13 @@ int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
   if (h->v5) {
     if (!t) return fail(STRIDE_E_ARG, "null argument");
-    return h->v5->set_layout(t->group_size, t->lds_bytes);
+    if (t->mode != 0 && t->mode != 3 && t->mode != 4)
+      return fail(STRIDE_E_ARG, "PDLT v5 modes: 3 (one kernel) or 4 (token pass + chain pass)");
+    const int tok = t->stage_words ? 4 * t->stage_words : h->v5->tok_lds_budget;
+    const int rc = h->v5->set_layout(t->group_size, t->lds_bytes, tok);
+    if (rc) return rc;
+    if (t->mode) h->v5->two_pass = t->mode == 4;
+    return STRIDE_OK;
   }
   return h->set.set_tune(t);
 }
@@ -866,10 +937,11 @@ int stride_decoder_get_tune(const stride_decoder* h, stride_tune* t) {
   if (!h || !t) return fail(STRIDE_E_ARG, "null argument");
   if (h->v5) {
     std::memset(t, 0, sizeof *t);
-    t->mode = 3;
+    t->mode = h->v5->two_pass ? 4 : 3;
     t->waves = 1;
     t->group_size = h->v5->group_size;
     t->lds_bytes = h->v5->lds_budget;
+    t->stage_words = h->v5->tok_lds_budget / 4;
     return STRIDE_OK;
   }
   h->set.get_tune(t);

@@ -11,7 +11,9 @@ namespace stride {
 
 struct DecodeConfig {
   int mode = 2;           // 0 = one wave per meshlet, 1 = groups (one lane per meshlet),
-                          // 2 = split groups (token lane + three axis lanes per meshlet)
+                          // 2 = split groups (token lane + three axis lanes per meshlet),
+                          // 4 = two passes: token lanes (64 meshlets per wave, walks to
+                          //     HBM), then the axis lanes of the split groups
   int waves = 1;          // mode 0: waves per workgroup (1, 2, 4); one meshlet per wave
   int stage_words = 512;  // mode 0: LDS words staged per meshlet (256, 512, 1024)
   int grid = 0;           // 0 = one workgroup per unit of work, else persistent grid
@@ -48,6 +50,11 @@ struct GroupLaunch {
   int split;              // mode 2 kernel
   int narrow;             // mode 2: uint16 codes (every mesh has g_bits <= 16)
   uint32_t* queue;        // mode 2: zeroed work-queue counter (persistent grid), or null
+  // mode 4 (two passes): token groups of up to 64 meshlets and the walks
+  const GroupEntry* tgroups;
+  uint32_t n_tgroups;
+  uint32_t tlds_bytes;
+  uint32_t* walk;         // one word per local vertex (mesh v_base + v_off)
 };
 
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream);

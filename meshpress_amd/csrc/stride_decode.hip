@@ -409,7 +409,8 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
       const MeshParams& P = meshes[e.mesh];
       const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
       const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
-      split_tokens(lds32, rel, e, blk, codes0, T + 3 * (P.t_base + e.t_off));
+      ScratchSeen<CodeT> seen{codes0};
+      split_tokens(lds32, rel, e, blk, seen, T + 3 * (P.t_base + e.t_off));
     }
     __syncthreads();
     if (active && role > 0) {
@@ -446,6 +447,95 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Two-pass split decode.  Pass 1: one token lane per meshlet, up to 64
+// meshlets per wave, seen set in registers; the walks go to HBM and the
+// triangles to the output.  Pass 2: the split groups' axis lanes on LDS
+// blocks that take the walk back from HBM, then the dequantisation.  The
+// token work no longer holds the groups' LDS blocks, so more meshlets are
+// resident per CU during the chains.
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void stride_tokens_kernel(
+    const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
+    const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
+    uint32_t* __restrict__ walk_g, uint32_t* __restrict__ T) {
+  extern __shared__ uint32_t lds32[];
+  const int lane = threadIdx.x;
+  const GroupEntry G = groups[blockIdx.x];
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
+  for (uint32_t i = lane; i < G.n_words; i += 64) lds32[i] = __builtin_bswap32(words[G.w_lo + i]);
+  __syncthreads();
+  const uint32_t m = G.m0 + (uint32_t)lane;
+  if (m < G.m1) {
+    const MeshletEntry e = tab[m];
+    const MeshParams& P = meshes[e.mesh];
+    const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
+    const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
+    RegSeen seen;
+    split_tokens(lds32, rel, e, walk_g + P.v_base + e.v_off, seen, T + 3 * (P.t_base + e.t_off));
+  }
+}
+
+template <bool NARROW>
+__global__ __launch_bounds__(64) void stride_axes_kernel(
+    const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
+    const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
+    const uint32_t* __restrict__ walk_g, float* __restrict__ V, uint32_t* __restrict__ overflow) {
+  using CodeT = typename std::conditional<NARROW, uint16_t, int32_t>::type;
+  extern __shared__ uint32_t lds32[];
+  const int lane = threadIdx.x;
+  const int slot = lane & 15, role = lane >> 4;
+  const uint32_t g = blockIdx.x;
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
+  const GroupEntry G = groups[g];
+  for (uint32_t i = lane; i < G.n_words; i += 64) lds32[i] = __builtin_bswap32(words[G.w_lo + i]);
+  const uint32_t m = G.m0 + (uint32_t)slot;
+  const bool active = m < G.m1;
+  MeshletEntry e;
+  if (active) e = tab[m];
+  const uint32_t nl = active ? e.n_local : 0u;
+  uint32_t total = 0;
+  const uint32_t pre0 = wave_excl_scan(role == 0 ? split_block_words(nl, NARROW) : 0u, total);
+  const uint32_t pre = __shfl(pre0, slot, 64);
+  uint32_t* blk = lds32 + G.codes_off + pre;
+  CodeT* codes0 = reinterpret_cast<CodeT*>(blk + nl);
+  const uint32_t cstride = split_code_stride(nl, NARROW);
+  if (active) {
+    const MeshParams& P = meshes[e.mesh];
+    const uint32_t* ws = walk_g + P.v_base + e.v_off;
+    for (uint32_t i = (uint32_t)role; i < nl; i += 4) blk[i] = ws[i];
+  }
+  __syncthreads();
+  if (active && role > 0) {
+    const MeshParams& P = meshes[e.mesh];
+    const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
+    const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
+    CodeT* codes = codes0 + cstride * (uint32_t)(role - 1);
+    if (!split_axis(lds32, G.n_words, rel, e, P, role - 1, blk, codes) && overflow)
+      overflow[g] = 1u;
+  }
+  __syncthreads();
+  for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
+    const MeshletEntry ej = tab[G.m0 + j];
+    const MeshParams& Pj = meshes[ej.mesh];
+    const uint32_t pj = __shfl(pre0, (int)j, 64);
+    const uint32_t n = ej.n_local, cs = split_code_stride(n, NARROW);
+    const CodeT* cb = reinterpret_cast<const CodeT*>(lds32 + G.codes_off + pj + n);
+    float* vo = V + 3 * (Pj.v_base + ej.v_off);
+    for (uint32_t v = lane; v < n; v += 64) {
+      const float x = dequant((int32_t)cb[v], Pj.g_min[0], Pj.g_range[0], Pj.g_bits[0], Pj.scale,
+                              Pj.center[0]);
+      const float y = dequant((int32_t)cb[cs + v], Pj.g_min[1], Pj.g_range[1], Pj.g_bits[1],
+                              Pj.scale, Pj.center[1]);
+      const float z = dequant((int32_t)cb[2 * cs + v], Pj.g_min[2], Pj.g_range[2], Pj.g_bits[2],
+                              Pj.scale, Pj.center[2]);
+      vo[3 * v] = x;
+      vo[3 * v + 1] = y;
+      vo[3 * v + 2] = z;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Host launchers
 // ---------------------------------------------------------------------------
 hipError_t launch_scan(const uint8_t* buf, const MeshletEntry* seeds, const MeshParams* meshes,
@@ -468,6 +558,21 @@ static hipError_t launch_decode_t(const DecodeLaunch& d, hipStream_t stream) {
 
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream) {
   if (d.n_groups == 0) return hipSuccess;
+  if (d.walk) {   // mode 4: token pass, then the axis pass
+    if (d.n_tgroups) {
+      hipLaunchKernelGGL(stride_tokens_kernel, dim3(d.n_tgroups), dim3(64), d.tlds_bytes, stream,
+                         d.buf, d.tab, d.meshes, d.tgroups, d.walk, d.T);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    }
+    if (d.narrow)
+      hipLaunchKernelGGL(stride_axes_kernel<true>, dim3(d.n_groups), dim3(64), d.lds_bytes, stream,
+                         d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
+    else
+      hipLaunchKernelGGL(stride_axes_kernel<false>, dim3(d.n_groups), dim3(64), d.lds_bytes, stream,
+                         d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
+    return hipGetLastError();
+  }
   const uint32_t grid = d.grid ? std::min(d.grid, d.n_groups) : d.n_groups;
   if (d.split && d.narrow)
     hipLaunchKernelGGL(stride_decode_split_kernel<true>, dim3(grid), dim3(64), d.lds_bytes, stream,

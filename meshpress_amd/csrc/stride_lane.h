@@ -267,11 +267,38 @@ STRIDE_DEV uint32_t win32(const uint32_t* __restrict__ st, uint32_t p) {
   return (st[w] << sh) | ((st[w + 1] >> 1) >> (31u - sh));
 }
 
-// Token lane.  `seen` is n_local zeroed entries of scratch (the x-axis code
-// array, which the axis lanes overwrite afterwards).
-template <class SeenT>
+// Seen sets of the token lane: fresh(v) is true on the first sighting of
+// local id v and marks it.
+// In LDS: n_local zeroed entries of scratch (the x-axis code array, which the
+// axis lanes overwrite afterwards).
+template <class T>
+struct ScratchSeen {
+  T* __restrict__ p;
+  STRIDE_DEV bool fresh(uint32_t v) {
+    const bool f = p[v] == 0;
+    p[v] = 1;
+    return f;
+  }
+};
+// In registers: 256 bits, selected without indexing.
+struct RegSeen {
+  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  STRIDE_DEV bool fresh(uint32_t v) {
+    const uint32_t q = (v >> 6) & 3u;
+    const uint64_t bit = 1ull << (v & 63u);
+    const uint64_t w = q == 0 ? s0 : q == 1 ? s1 : q == 2 ? s2 : s3;
+    s0 |= q == 0 ? bit : 0ull;
+    s1 |= q == 1 ? bit : 0ull;
+    s2 |= q == 2 ? bit : 0ull;
+    s3 |= q == 3 ? bit : 0ull;
+    return (w & bit) == 0;
+  }
+};
+
+// Token lane: strip tokens -> triangles (tout) and the walk.
+template <class Seen>
 STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
-                             uint32_t* __restrict__ walk, SeenT* __restrict__ seen,
+                             uint32_t* __restrict__ walk, Seen& seen,
                              uint32_t* __restrict__ tout) {
   const uint32_t n_local = e.n_local, n_tris = e.n_tris;
   const uint32_t n_strips = e.n_strips & kStripMask;
@@ -313,8 +340,7 @@ STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, cons
     fifo.to_front(j, v);
     // first sighting -> walk entry; parallelogram refs (a, b, c):
     // ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
-    const bool fresh = seen[v] == 0;
-    seen[v] = 1;
+    const bool fresh = seen.fresh(v);
     const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
     const uint32_t entry = is_root ? (kRootMark | v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
     if (fresh) walk[nw & 255u] = entry;

@@ -7,7 +7,9 @@
 //
 // With "split" the split decoder runs instead: every meshlet's token lane
 // (split_tokens), then its three axis lanes (split_axis), as the kernel's
-// phases do.
+// phases do.  "twopass" / "twopass16" run the two-pass decoder: all token
+// lanes first with register seen sets, then the axis lanes on blocks that
+// hold only the copied walk.
 // Exit status: 0 ok, 2 stream rejected by the pre-pass, 3 code overflow
 // (the kernels route such groups to the wave-per-meshlet decoder).
 #include <cstdio>
@@ -85,20 +87,37 @@ int main(int argc, char** argv) {
     nv += s.n_local;
     nt += s.n_tris;
   }
-  const bool split = argc > 4 && std::strncmp(argv[4], "split", 5) == 0;
-  const bool narrow = argc > 4 && std::strcmp(argv[4], "split16") == 0;
+  const bool two = argc > 4 && std::strncmp(argv[4], "twopass", 7) == 0;
+  const bool split = two || (argc > 4 && std::strncmp(argv[4], "split", 5) == 0);
+  const bool narrow = argc > 4 && (std::strcmp(argv[4], "split16") == 0 ||
+                                   std::strcmp(argv[4], "twopass16") == 0);
   std::vector<uint64_t> codes(nv, kUnseen);
   std::vector<uint32_t> tris(3 * nt, 0xFFFFFFFFu);
   std::vector<int32_t> codes3(3 * nv, 0);
   int rc = 0;
+  // two-pass: every token lane first (register seen set, walks to one
+  // exactly sized array), then the axis lanes over blocks holding stale data
+  // except for the copied walk
+  std::vector<uint32_t> walk_g(two ? nv : 0);
+  if (two)
+    for (uint32_t m = 0; m < n_m; ++m) {
+      const MeshletEntry& e = tab[m];
+      RegSeen seen;
+      split_tokens(st.data(), e.start_bit, e, walk_g.data() + e.v_off, seen,
+                   tris.data() + 3ull * e.t_off);
+    }
   for (uint32_t m = 0; m < n_m; ++m) {
     const MeshletEntry& e = tab[m];
     // one group = the whole stream staged: rel is the absolute start bit
     if (split && narrow) {
-      std::vector<uint32_t> blk(split_block_words(e.n_local, true), 0u);
+      std::vector<uint32_t> blk(split_block_words(e.n_local, true), two ? 0xA5A5A5A5u : 0u);
       uint16_t* c16 = reinterpret_cast<uint16_t*>(blk.data() + e.n_local);
       const uint32_t cs = split_code_stride(e.n_local, true);
-      split_tokens(st.data(), e.start_bit, e, blk.data(), c16, tris.data() + 3ull * e.t_off);
+      ScratchSeen<uint16_t> seen{c16};
+      if (two)
+        for (uint32_t v = 0; v < e.n_local; ++v) blk[v] = walk_g.at((size_t)e.v_off + v);
+      else
+        split_tokens(st.data(), e.start_bit, e, blk.data(), seen, tris.data() + 3ull * e.t_off);
       for (int d = 0; d < 3; ++d) {
         if (!split_axis(st.data(), (uint32_t)n_words, e.start_bit, e, P, d, blk.data(), c16 + cs * d))
           rc = 3;
@@ -106,9 +125,13 @@ int main(int argc, char** argv) {
           codes3[3 * ((size_t)e.v_off + v) + d] = c16[cs * d + v];
       }
     } else if (split) {
-      std::vector<uint32_t> blk(split_block_words(e.n_local, false), 0u);
+      std::vector<uint32_t> blk(split_block_words(e.n_local, false), two ? 0xA5A5A5A5u : 0u);
       int32_t* c32 = reinterpret_cast<int32_t*>(blk.data() + e.n_local);
-      split_tokens(st.data(), e.start_bit, e, blk.data(), c32, tris.data() + 3ull * e.t_off);
+      ScratchSeen<int32_t> seen{c32};
+      if (two)
+        for (uint32_t v = 0; v < e.n_local; ++v) blk[v] = walk_g.at((size_t)e.v_off + v);
+      else
+        split_tokens(st.data(), e.start_bit, e, blk.data(), seen, tris.data() + 3ull * e.t_off);
       for (int d = 0; d < 3; ++d) {
         int32_t* cd = c32 + (size_t)e.n_local * d;
         if (!split_axis(st.data(), (uint32_t)n_words, e.start_bit, e, P, d, blk.data(), cd))

@@ -5,7 +5,12 @@
 // output) over an LDS image of exactly the launch's size, under the
 // sanitizers when built with them.
 //
-//   pdlt_lane_check <stream> <verts.f32 out> <tris.u32 out> [group_size lds_bytes [normalised [chain32]]]
+//   pdlt_lane_check <stream> <verts.f32 out> <tris.u32 out>
+//                   [group_size lds_bytes [normalised [chain32 [two_pass]]]]
+//
+// two_pass = 1 runs the two-pass decode: the token pass over token groups
+// of up to 64 meshlets (walks and seen sets to global arrays), then the
+// chain pass that copies them back into the group blocks.
 //
 // Exit status: 0 ok, 2 stream rejected (message on stderr).
 #include <cstdio>
@@ -31,6 +36,7 @@ int main(int argc, char** argv) {
   const int budget = argc > 5 ? std::atoi(argv[5]) : 32768;
   const bool world = !(argc > 6 && std::atoi(argv[6]) == 1);
   const bool chain32 = argc > 7 && std::atoi(argv[7]) == 1;
+  const bool two_pass = argc > 8 && std::atoi(argv[8]) == 1;
   PdltHost H;
   std::string err;
   int rc = pdlt_parse(data.data(), data.size(), H, err);
@@ -51,13 +57,35 @@ int main(int argc, char** argv) {
   std::vector<float> V(3 * H.n_v);
   std::vector<uint32_t> T(3 * H.n_t);
   for (size_t i = 0; i < H.bnd.size(); ++i) V[i] = pdlt_out(pdlt_bnd(H.bnd[i], P), P, (int)(i % 3));
+  // two-pass: the token pass first, into exactly sized global arrays
+  std::vector<uint32_t> walk_g(H.n_v - H.n_b), seen_g((size_t)kSeenWords * H.n_m);
+  if (two_pass) {
+    std::vector<PdltGroup> tg;
+    uint32_t tlds = 0;
+    rc = pdlt_layout_tokens(H.tab, budget, tg, tlds, err);
+    if (rc) {
+      std::fprintf(stderr, "token layout (%d): %s\n", rc, err.c_str());
+      return 2;
+    }
+    for (const PdltGroup& G : tg) {
+      std::vector<uint64_t> lds64((tlds + 7) / 8, 0x5A5A5A5A5A5A5A5Aull);
+      uint32_t* lds = reinterpret_cast<uint32_t*>(lds64.data());
+      for (uint32_t i = 0; i < G.n_words; ++i) lds[i] = __builtin_bswap32(H.words.at(G.w_lo + i));
+      for (uint32_t m = G.m0; m < G.m1; ++m) {
+        const PdltEntry& e = H.tab[m];
+        uint32_t* seen = seen_g.data() + (size_t)kSeenWords * m;
+        pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, lds + G.blk_base + e.tok_ref_off,
+                    walk_g.data() + (e.int_base - H.n_b), seen, seen + 8, T.data() + 3ull * e.t_off);
+      }
+    }
+  }
   for (const PdltGroup& G : groups) {
     std::vector<uint64_t> lds64((lds_bytes + 7) / 8, 0x5A5A5A5A5A5A5A5Aull);   // stale LDS
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds64.data());
     for (uint32_t i = 0; i < G.n_words; ++i) lds[i] = __builtin_bswap32(H.words.at(G.w_lo + i));
     const int64_t rel = -32 * (int64_t)G.w_lo;
     auto blk = [&](const PdltEntry& e) { return lds + G.blk_base + e.blk_off; };
-    // phase A: token lanes
+    // phase A: token lanes, or the copy-back of the token pass and the refs
     for (uint32_t m = G.m0; m < G.m1; ++m) {
       const PdltEntry& e = H.tab[m];
       uint32_t* b = blk(e);
@@ -65,7 +93,13 @@ int main(int argc, char** argv) {
       uint32_t* refs = b + 6 * nl;
       uint32_t* walk = refs + e.n_bnd;
       uint32_t* seen = walk + e.n_int;
-      pdlt_tokens(lds, rel, e, refs, walk, seen, seen + 8, T.data() + 3ull * e.t_off);
+      if (!two_pass) {
+        pdlt_tokens(lds, rel, e, refs, walk, seen, seen + 8, T.data() + 3ull * e.t_off);
+        continue;
+      }
+      for (uint32_t i = 0; i < e.n_int; ++i) walk[i] = walk_g.at(e.int_base - H.n_b + i);
+      for (uint32_t i = 0; i < kSeenWords; ++i) seen[i] = seen_g.at((size_t)kSeenWords * m + i);
+      pdlt_refs(lds, rel, e, refs);
     }
     // phase B: boundary reconstructions
     for (uint32_t m = G.m0; m < G.m1; ++m) {

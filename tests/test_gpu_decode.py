@@ -83,6 +83,12 @@ TUNES = [dict(mode=0, waves=w, stage_words=s) for w in (1, 2, 4) for s in (256, 
          dict(mode=2, group_size=4, lds_bytes=16384, grid=3, use_queue=1),
          dict(mode=2, group_size=5, lds_bytes=16384),
          dict(mode=2, group_size=1, lds_bytes=8192),
+         # two passes: token lanes (64 meshlets per wave), then the axis lanes
+         dict(mode=4, group_size=16, lds_bytes=24576),
+         dict(mode=4, group_size=16, lds_bytes=49152),
+         dict(mode=4, group_size=5, lds_bytes=16384),
+         dict(mode=4, group_size=1, lds_bytes=8192),
+         dict(mode=4, group_size=16, lds_bytes=8192),
          # too small for most meshlets: exercises the wave-per-meshlet fallback list
          dict(mode=1, group_size=32, lds_bytes=8192, stage_words=256)]
 
@@ -178,7 +184,8 @@ def test_batch_decode_matches_single(coracle):
     bd = BatchDecoder(blobs)
     for tune in (dict(), dict(mode=0, waves=4, stage_words=512, grid=128, use_queue=1),
                  dict(mode=1, group_size=16, lds_bytes=16384, grid=0, use_queue=0),
-                 dict(mode=2, group_size=16, lds_bytes=24576, grid=96, use_queue=1)):
+                 dict(mode=2, group_size=16, lds_bytes=24576, grid=96, use_queue=1),
+                 dict(mode=4, group_size=16, lds_bytes=24576, grid=0, use_queue=0)):
         if tune:
             bd.set_tune(**tune)
         for mode in ("launch", "per_mesh", "graph", "graph", "graph"):
@@ -207,6 +214,8 @@ def test_icosphere_l8_full_size(coracle, bits):
     Vg, Tg = gpu_decode(data)
     Vo, To, codes = coracle.decode(data, n_threads=8, with_codes=True)
     assert_same(Vg, Tg, Vo, To)
+    V4, T4 = gpu_decode(data, mode=4, group_size=16, lds_bytes=24576)   # two passes
+    assert_same(V4, T4, Vo, To)
     plans = prep["native"].export_plans()
     l2g = np.concatenate([p["local_to_global"] for p in plans])
     assert np.array_equal(codes, prep["global_codes"][l2g])

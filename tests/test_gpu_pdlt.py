@@ -14,8 +14,13 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
 
 GOLD = sorted((Path(__file__).resolve().parent / "golden").glob("pdlt_*.npz"))
-LAYOUTS = [dict(group_size=16, lds_bytes=32768), dict(group_size=1, lds_bytes=4096),
-           dict(group_size=5, lds_bytes=16384), dict(group_size=16, lds_bytes=65536)]
+# mode 4: token pass + chain pass (default); mode 3: one kernel.  stage_words
+# is the token pass's LDS budget in words.
+LAYOUTS = [dict(mode=4, group_size=16, lds_bytes=32768, stage_words=8192),
+           dict(mode=3, group_size=16, lds_bytes=32768),
+           dict(mode=4, group_size=1, lds_bytes=4096, stage_words=1024),
+           dict(mode=3, group_size=5, lds_bytes=16384),
+           dict(mode=4, group_size=16, lds_bytes=65536, stage_words=16384)]
 
 
 def decoder(data, chain="f64", **tune):
@@ -136,8 +141,10 @@ def test_large_mesh_error_bound_and_faces(n):
     got = np.sort(ids[Tg.astype(np.int64)], axis=1)
     want = np.sort(T, axis=1)
     assert np.array_equal(got[np.lexsort(got.T[::-1])], want[np.lexsort(want.T[::-1])])
-    V2, T2 = decoder(data, group_size=3, lds_bytes=12288).decode_to_host()
-    assert np.array_equal(V2.view(np.uint32), Vg.view(np.uint32)) and np.array_equal(T2, Tg)
+    for tune in (dict(mode=4, group_size=3, lds_bytes=12288, stage_words=2048),
+                 dict(mode=3, group_size=3, lds_bytes=12288)):
+        V2, T2 = decoder(data, **tune).decode_to_host()
+        assert np.array_equal(V2.view(np.uint32), Vg.view(np.uint32)) and np.array_equal(T2, Tg)
 
 
 def test_sblb_blobs_decode_like_pdlt():

@@ -213,7 +213,8 @@ def lane_check(tmp_path_factory):
 
 def run_lane_check(exe, data, tmp_path, split=False):
     """split: False (one lane per meshlet), True or "split" (int32 codes),
-    "split16" (uint16 codes)."""
+    "split16" (uint16 codes), "twopass" / "twopass16" (token pass, then the
+    axis lanes)."""
     import subprocess
     src = tmp_path / "s.dpr4"
     src.write_bytes(data)
@@ -224,7 +225,8 @@ def run_lane_check(exe, data, tmp_path, split=False):
         np.fromfile(ft, np.uint32).reshape(-1, 3) if r.returncode in (0, 3) else None
 
 
-@pytest.mark.parametrize("split", [False, "split", "split16"], ids=["lane", "split", "split16"])
+@pytest.mark.parametrize("split", [False, "split", "split16", "twopass", "twopass16"],
+                         ids=["lane", "split", "split16", "twopass", "twopass16"])
 def test_lane_decoder_matches_oracle(lane_check, coracle, tmp_path, split):
     """The gfx950 kernels' pre-pass and per-lane decoders (one lane per
     meshlet, and the split token/axis lanes), compiled for the host,

@@ -28,13 +28,14 @@ def pdlt_check(tmp_path_factory):
     return exe
 
 
-def run(exe, data, tmp_path, gs=16, lds=32768, world=True, chain32=False):
+def run(exe, data, tmp_path, gs=16, lds=32768, world=True, chain32=False, two_pass=False):
     import subprocess
     src = tmp_path / "s.pdlt"
     src.write_bytes(data)
     fv, ft = tmp_path / "v.f32", tmp_path / "t.u32"
     r = subprocess.run([str(exe), str(src), str(fv), str(ft), str(gs), str(lds),
-                        "0" if world else "1", "1" if chain32 else "0"], capture_output=True, text=True)
+                        "0" if world else "1", "1" if chain32 else "0", "1" if two_pass else "0"],
+                       capture_output=True, text=True)
     if r.returncode != 0:
         return r.returncode, r.stderr, None
     return 0, np.fromfile(fv, np.float32).reshape(-1, 3), np.fromfile(ft, np.uint32).reshape(-1, 3)
@@ -50,9 +51,10 @@ def test_lane_logic_matches_oracle_on_goldens(pdlt_check, tmp_path, path):
     data = np.load(path)["data"].tobytes()
     Vo, To = PO.decode_v5(data)
     for gs, lds in LAYOUTS:
-        rc, V, T = run(pdlt_check, data, tmp_path, gs, lds)
-        assert rc == 0, V
-        assert same(V, T, Vo, To), (gs, lds)
+        for two_pass in (False, True):
+            rc, V, T = run(pdlt_check, data, tmp_path, gs, lds, two_pass=two_pass)
+            assert rc == 0, V
+            assert same(V, T, Vo, To), (gs, lds, two_pass)
     rc, V, T = run(pdlt_check, data, tmp_path, world=False)
     Vn, _ = PO.decode_v5(data, world=False)
     assert rc == 0 and same(V, T, Vn, To)
@@ -78,6 +80,9 @@ def test_lane_logic_matches_oracle_on_fresh_streams(pdlt_check, tmp_path):
         rc, Vl, Tl = run(pdlt_check, data, tmp_path, chain32=True)
         assert rc == 0
         assert same(Vl, Tl, *PO.decode_v5(data, chain="f32"))
+        rc, Vl, Tl = run(pdlt_check, data, tmp_path, 7, 12288, two_pass=True)
+        assert rc == 0
+        assert same(Vl, Tl, *PO.decode_v5(data))
 
 
 def test_prepass_rejects_or_matches_under_bit_flips(pdlt_check, tmp_path):
