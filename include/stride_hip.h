@@ -133,7 +133,14 @@ int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes,
 int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
 
 /* Launch configuration (autotuned).
- *   mode 2 (default; group_size 16, lds_bytes 24576): split groups of up to
+ *   mode 4 (default; group_size 16, lds_bytes 16384): two passes.  Token
+ *     lanes first, one per meshlet, tok_group (<= 64, default 32) consecutive
+ *     meshlets per wave, their bits staged in LDS with an 8-word seen set per
+ *     lane, writing the walks to a device buffer; then the axis lanes of the
+ *     split groups (group_size, lds_bytes) stream their substreams from
+ *     device memory and keep only the codes in LDS.  The launch takes the
+ *     LDS its largest group needs, at most lds_bytes.
+ *   mode 2 (group_size 16, lds_bytes 24576 typical): split groups of up to
  *     16 consecutive meshlets per wave, four lanes per meshlet (one decodes
  *     the strip tokens into a walk list in LDS, then one per axis decodes
  *     that axis's residual substreams and prediction chain); lds_bytes of
@@ -147,7 +154,11 @@ int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
  *   mode 0: one wave per meshlet: waves per workgroup {1,2,4}, LDS stage
  *     words per meshlet {256,512,1024}, use_queue = persistent grid pulling
  *     meshlets from a device atomic counter.
- *   grid: 0 = one workgroup per unit of work, else a persistent grid. */
+ *   grid: 0 = one workgroup per unit of work, else a persistent grid.
+ *   PDLT v5 streams: mode 4 (default: token pass, then chain groups of
+ *   group_size meshlets in lds_bytes; stage_words = the token pass's LDS
+ *   words, tok_group its meshlets per wave) or mode 3 (one kernel).
+ *   get_tune reports in `reserved` the meshlets decoded by mode 0. */
 typedef struct stride_tune {
   int32_t waves;
   int32_t stage_words;
@@ -157,6 +168,8 @@ typedef struct stride_tune {
   int32_t group_size;
   int32_t lds_bytes;
   int32_t reserved;
+  int32_t tok_group;   /* mode 4: meshlets per token-pass wave, 1..64 (0: the
+                          default, 32 for DPR4 streams, 64 for PDLT v5) */
 } stride_tune;
 
 int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t);

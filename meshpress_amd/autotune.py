@@ -6,8 +6,8 @@ launch stream and keeps the fastest.  The search space:
 
   mode 2 (split groups, 4 lanes per meshlet): group_size x lds_bytes, and
                                      persistent grids with a device work queue
-  mode 4 (two passes: token lanes over 64 meshlets per wave, then the split
-         groups' axis lanes):       group_size x lds_bytes
+  mode 4 (two passes: token lanes over tok_group meshlets per wave, then the split
+         groups' axis lanes):       group_size x lds_bytes x tok_group
   mode 1 (lane-per-meshlet groups):  group_size x lds_bytes x grid
   mode 0 (wave per meshlet):         waves x stage_words (x persistent queue)
 
@@ -36,9 +36,9 @@ def candidates(include_wave_mode: bool = True, grids=(0,)):
     # persistent grids pulling groups from the device work queue
     for g in QUEUE_GRIDS:
         out.append(dict(mode=2, group_size=16, lds_bytes=24576, grid=g, use_queue=1))
-    # two passes: token lanes over 64 meshlets per wave, then the axis lanes
-    for gs, lds in itertools.product((8, 16), (16384, 24576, 32768)):
-        out.append(dict(mode=4, group_size=gs, lds_bytes=lds, grid=0, use_queue=0))
+    # two passes: token lanes over tok_group meshlets per wave, then the axis lanes
+    for gs, lds, tg in itertools.product((8, 16), (16384, 24576), (32, 64)):
+        out.append(dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, grid=0, use_queue=0))
     for gs, lds, grid in itertools.product(GROUP_SIZES, LDS_BUDGETS, grids):
         out.append(dict(mode=1, group_size=gs, lds_bytes=lds, grid=grid, use_queue=0))
     if include_wave_mode:

@@ -21,6 +21,7 @@ struct PdltSet {
   int group_size = 16;
   int lds_budget = 24576;
   int tok_lds_budget = 16384;
+  int tok_group = 64;                   // meshlets per token-pass wave
   int chain32 = 0;                      // 1: the encoder's float32 chain
   int two_pass = 1;                     // 1: token pass + chain pass, 0: one kernel
   // device

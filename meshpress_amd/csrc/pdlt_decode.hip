@@ -119,8 +119,8 @@ __global__ __launch_bounds__(64) void pdlt_decode_kernel(
 }
 
 // Two-pass decode, pass 1: one token lane per meshlet, up to 64 meshlets
-// per wave (refs in LDS for the global triangle ids); the walk, seen set
-// and walk length go to HBM.
+// per wave (refs in LDS for the global triangle ids, seen set in LDS); the
+// walk, seen set and walk length go to HBM.
 __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
     const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
     const PdltGroup* __restrict__ groups, uint32_t n_b, uint32_t* __restrict__ walk_g,
@@ -134,9 +134,15 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
   const uint32_t m = G.m0 + (uint32_t)lane;
   if (m < G.m1) {
     const PdltEntry e = tab[m];
-    uint32_t* seen = seen_g + (uint64_t)kSeenWords * m;
-    pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, lds + G.blk_base + e.tok_ref_off,
-                walk_g + (e.int_base - n_b), seen, seen + 8, T + 3ull * e.t_off);
+    uint32_t* refs = lds + G.blk_base + e.tok_ref_off;
+    uint32_t* seen = refs + e.n_bnd;
+    uint32_t n_emit = 0;
+    pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g + (e.int_base - n_b), seen, &n_emit,
+                T + 3ull * e.t_off);
+    uint32_t* so = seen_g + (uint64_t)kSeenWords * m;
+#pragma unroll
+    for (int q = 0; q < 8; ++q) so[q] = seen[q];
+    so[8] = n_emit;
   }
 }
 
@@ -228,7 +234,7 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   uint32_t lds = 0, tlds = 0;
   std::vector<PdltEntry> tab = H.tab;
   int rc = pdlt_layout(tab, gs, budget, gr, lds, err);
-  if (!rc) rc = pdlt_layout_tokens(tab, tok_budget, tg, tlds, err);
+  if (!rc) rc = pdlt_layout_tokens(tab, tok_budget, tok_group, tg, tlds, err);
   if (rc) return set_error(rc, err);
   H.tab.swap(tab);
   groups.swap(gr);

@@ -251,12 +251,14 @@ inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vec
   return STRIDE_OK;
 }
 
-// Token groups of the two-pass decode: up to 64 consecutive meshlets per
-// wave (one token lane each) whose staged words and refs fit `budget` bytes.
-// Sets each entry's tok_ref_off; lds = the largest group's LDS bytes.
-inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, std::vector<PdltGroup>& groups,
-                              uint32_t& lds, std::string& err) {
+// Token groups of the two-pass decode: up to gmax (<= 64) consecutive
+// meshlets per wave (one token lane each) whose staged words, refs and seen
+// sets fit `budget` bytes.  Sets each entry's tok_ref_off; lds = the largest group's
+// LDS bytes.
+inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, int gmax,
+                              std::vector<PdltGroup>& groups, uint32_t& lds, std::string& err) {
   if (budget < 4096 || budget > 65536) { err = "lds_bytes must be in [4096, 65536]"; return STRIDE_E_ARG; }
+  if (gmax < 1 || gmax > 64) { err = "tok_group must be in [1, 64]"; return STRIDE_E_ARG; }
   groups.clear();
   uint32_t max_lds = 0;
   const uint32_t n_m = (uint32_t)tab.size();
@@ -266,11 +268,12 @@ inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, std::vect
     g.m0 = m;
     g.w_lo = tab[m].start_bit >> 5;
     uint32_t refs = 0, words = 0, j = m;
-    while (j < n_m && j - m < 64) {
+    while (j < n_m && j - m < (uint32_t)gmax) {
       const uint32_t w = ((((tab[j].end_bit - 1) >> 5) + 2 - g.w_lo) + 1) & ~1u;
-      if (j > m && 4ull * (w + refs + tab[j].n_bnd) > (uint64_t)budget) break;
+      // per meshlet: refs (n_bnd words), then the seen set (8 words)
+      if (j > m && 4ull * (w + refs + tab[j].n_bnd + 8) > (uint64_t)budget) break;
       tab[j].tok_ref_off = refs;
-      refs += tab[j].n_bnd;
+      refs += tab[j].n_bnd + 8;
       words = w;
       ++j;
     }

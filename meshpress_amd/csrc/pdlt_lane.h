@@ -310,8 +310,8 @@ STRIDE_DEV void pdlt_refs(const uint32_t* __restrict__ st, int64_t rel, const Pd
 // Token lane: boundary refs into refs[], strips into the walk (interior
 // locals in first-use order: v * 0x01010101 for a strip-root vertex,
 // v | a << 8 | b << 16 | c << 24 for a parallelogram step, a != v) and the
-// triangles into tout (global ids).  Stores the seen set (8 words) and the
-// walk length.
+// triangles into tout (global ids).  seen_out (8 words, LDS in the kernels)
+// is the live seen set; the walk length goes to *n_emit.
 STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e,
                             uint32_t* __restrict__ refs, uint32_t* __restrict__ walk,
                             uint32_t* __restrict__ seen_out, uint32_t* __restrict__ n_emit,
@@ -325,12 +325,8 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
   MtfLite fifo;
   fifo.clear();
   // seen set, boundary locals preset
-  uint64_t s4[4];
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    const uint32_t lo = 64u * (uint32_t)q;
-    s4[q] = n_bnd >= lo + 64 ? ~0ull : (n_bnd <= lo ? 0ull : ((1ull << (n_bnd - lo)) - 1));
-  }
+  BitSeen seen{seen_out};
+  seen.preset(n_bnd);
   uint32_t nw = 0, q0 = 0, q1 = 0, q2 = 0;
   uint32_t root_j = 3, steps_left = 0;
   const uint32_t ibase = e.int_base - n_bnd;
@@ -359,14 +355,7 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
     }
     fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
     fifo.to_front(j, v);
-    const uint32_t qw = (v >> 6) & 3u;
-    const uint64_t bit = 1ull << (v & 63u);
-    const uint64_t sw = qw == 0 ? s4[0] : qw == 1 ? s4[1] : qw == 2 ? s4[2] : s4[3];
-    const bool fresh = (sw & bit) == 0;
-    s4[0] |= qw == 0 ? bit : 0ull;
-    s4[1] |= qw == 1 ? bit : 0ull;
-    s4[2] |= qw == 2 ? bit : 0ull;
-    s4[3] |= qw == 3 ? bit : 0ull;
+    const bool fresh = seen.fresh(v);
     const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
     const uint32_t entry = is_root ? v * 0x01010101u : (v | (ra << 8) | (q2 << 16) | (rc << 24));
     if (fresh) walk[nw & 255u] = entry;
@@ -385,11 +374,6 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
     q2 = is_root ? (root_j == 2 ? v : q2) : v;
     root_j += is_root ? 1u : 0u;
     steps_left -= is_root ? 0u : 1u;
-  }
-#pragma unroll
-  for (int q = 0; q < 4; ++q) {
-    seen_out[2 * q] = (uint32_t)s4[q];
-    seen_out[2 * q + 1] = (uint32_t)(s4[q] >> 32);
   }
   *n_emit = nw;
 }

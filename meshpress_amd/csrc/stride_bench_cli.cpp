@@ -5,11 +5,12 @@
 // JSON line with the event-timed decode rate.
 //
 //   stride_bench FILE [--iters N] [--warmup W] [--mode M] [--group G]
-//                [--lds BYTES] [--grid R] [--queue 0|1] [--stage WORDS] [--chain 64|32]
+//                [--lds BYTES] [--grid R] [--queue 0|1] [--stage WORDS] [--tok N]
+//                [--chain 64|32]
 //
-// PDLT v5 uses --mode (3 one kernel, 4 token pass + chain pass), --group,
-// --lds, --stage (the token pass's LDS words) and --chain (float64 or
-// float32 chain).
+// --tok sets the meshlets per token-pass wave of mode 4.  PDLT v5 uses
+// --mode (3 one kernel, 4 token pass + chain pass), --group, --lds, --stage
+// (the token pass's LDS words), --tok and --chain (float64 or float32 chain).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
@@ -45,7 +46,7 @@ int main(int argc, char** argv) {
   int iters = 100, warmup = 10, chain = 64;
   stride_tune want{};
   bool set_mode = false, set_group = false, set_lds = false, set_grid = false, set_queue = false,
-       set_stage = false;
+       set_stage = false, set_tok = false;
   for (int i = 2; i + 1 < argc; i += 2) {
     const std::string k = argv[i];
     const int v = std::atoi(argv[i + 1]);
@@ -58,6 +59,7 @@ int main(int argc, char** argv) {
     else if (k == "--queue") { want.use_queue = v; set_queue = true; }
     else if (k == "--chain") chain = v;
     else if (k == "--stage") { want.stage_words = v; set_stage = true; }
+    else if (k == "--tok") { want.tok_group = v; set_tok = true; }
     else { std::fprintf(stderr, "stride_bench: unknown option %s\n", k.c_str()); return 2; }
   }
   if (iters < 1 || warmup < 0) { std::fprintf(stderr, "stride_bench: bad iteration counts\n"); return 2; }
@@ -89,6 +91,7 @@ int main(int argc, char** argv) {
   if (set_grid) t.grid = want.grid;
   if (set_queue) t.use_queue = want.use_queue;
   if (set_stage) t.stage_words = want.stage_words;
+  if (set_tok) t.tok_group = want.tok_group;
   if ((rc = stride_decoder_set_tune(h, &t))) return die("set_tune", rc);
   uint64_t n_v = 0, n_t = 0, sb = 0, side = 0;
   uint32_t n_m = 0;
@@ -118,10 +121,10 @@ int main(int argc, char** argv) {
   std::printf("{\"file\": \"%s\", \"format\": \"%s\", \"n_tris\": %llu, \"n_verts\": %llu, "
               "\"meshlets\": %u, \"iters\": %d, \"ms\": %.6f, \"gtris_s\": %.4f, \"hbm_gb_s\": %.2f, "
               "\"chain\": %d, \"tune\": {\"mode\": %d, \"group_size\": %d, \"lds_bytes\": %d, "
-              "\"grid\": %d, \"use_queue\": %d, \"stage_words\": %d}}\n",
+              "\"grid\": %d, \"use_queue\": %d, \"stage_words\": %d, \"tok_group\": %d}}\n",
               argv[1], fmt, (unsigned long long)n_t, (unsigned long long)n_v,
               n_m, iters, ms, (double)n_t / (ms * 1e-3) / 1e9, bytes / (ms * 1e-3) / 1e9, chain, t.mode,
-              t.group_size, t.lds_bytes, t.grid, t.use_queue, t.stage_words);
+              t.group_size, t.lds_bytes, t.grid, t.use_queue, t.stage_words, t.tok_group);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);

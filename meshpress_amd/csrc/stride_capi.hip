@@ -150,6 +150,7 @@ struct DecodeSet {
   // groups) and the walks, one word per local vertex
   GroupEntry* d_tgroups = nullptr;
   uint32_t n_tgroups = 0, tlds_bytes = 0;
+  uint32_t group_lds = 0;                 // LDS bytes of the group launches (largest group)
   std::vector<uint32_t> tg_base;          // first token group of each mesh, n_meshes + 1
   uint32_t* d_walk = nullptr;
   uint32_t n_legacy = 0;
@@ -325,8 +326,17 @@ struct DecodeSet {
       if (cfg.mode == 1 || cfg.mode == 2 || cfg.mode == 4) {
         const uint64_t budget = (uint64_t)cfg.lds_bytes;
         const bool split = cfg.mode != 1;
+        const bool two = cfg.mode == 4;
         const bool nar = narrow();
         const uint32_t gmax = split ? std::min(cfg.group_size, 16) : cfg.group_size;
+        // bytes of LDS: staged words + per-meshlet code storage ("codes"
+        // counts vertices in mode 1, block words in mode 2, code words only
+        // in mode 4, whose axis pass stages no bits)
+        auto need = [split, two](uint64_t words, uint64_t cu) {
+          return two ? 4 * cu
+                     : (split ? 4 * ((words + 3) & ~3ull) + 4 * cu : 8 * ((words + 1) / 2) + 8 * cu);
+        };
+        group_lds = 0;
         for (int i = 0; i < n_meshes; ++i) {
           g_base[i] = (uint32_t)groups.size();
           const uint64_t bb = mp[i].blob_byte * 8ull;
@@ -339,6 +349,7 @@ struct DecodeSet {
               // mode 2: walk/codes blocks after the words, in 4-byte units
               cur.codes_off = split ? ((cur.n_words + 3) & ~3u) : (cur.n_words + 1) / 2;
               groups.push_back(cur);
+              group_lds = std::max<uint32_t>(group_lds, (uint32_t)need(cur.n_words, codes));
             }
             open = false;
           };
@@ -347,12 +358,8 @@ struct DecodeSet {
             const MeshletEntry& e = tab[j];
             const uint64_t lo = (bb + e.start_bit) >> 5;
             const uint64_t hi = ((bb + e.end_bit + 31) >> 5) + 2;
-            // bytes of LDS: staged words + per-meshlet code storage ("codes"
-            // counts vertices in mode 1, block words in mode 2)
-            const uint64_t unit = split ? split_block_words(e.n_local, nar) : e.n_local;
-            auto need = [split](uint64_t words, uint64_t cu) {
-              return split ? 4 * ((words + 3) & ~3ull) + 4 * cu : 8 * ((words + 1) / 2) + 8 * cu;
-            };
+            const uint64_t unit = two ? split_code_words(e.n_local, nar)
+                                      : (split ? split_block_words(e.n_local, nar) : e.n_local);
             if (open && (j - cur.m0 < gmax) && need(hi - cur.w_lo, codes + unit) <= budget) {
               cur.m1 = j + 1;
               cur.n_words = (uint32_t)(hi - cur.w_lo);
@@ -443,7 +450,7 @@ struct DecodeSet {
   }
 
   // mode 4 token groups: unions of consecutive split groups of one mesh, up
-  // to 64 meshlets and kTokenLds bytes of staged words
+  // to tok_group meshlets and kTokenLds bytes of staged words
   static constexpr uint64_t kTokenLds = 32768;
   int token_layout(const std::vector<GroupEntry>& groups) {
     const int n_meshes = (int)mp.size();
@@ -457,7 +464,8 @@ struct DecodeSet {
       for (uint32_t gi = g_base[i]; gi < g_base[i + 1]; ++gi) {
         const GroupEntry& G = groups[gi];
         const uint64_t words = (uint64_t)G.w_lo + G.n_words - cur.w_lo;
-        if (open && G.m0 == cur.m1 && G.m1 - cur.m0 <= 64 && 4 * words <= kTokenLds) {
+        if (open && G.m0 == cur.m1 && G.m1 - cur.m0 <= (uint32_t)cfg.tok_group &&
+            4 * words <= kTokenLds) {
           cur.m1 = G.m1;
           cur.n_words = (uint32_t)words;
           continue;
@@ -473,7 +481,8 @@ struct DecodeSet {
       if (open) tg.push_back(cur);
     }
     tg_base[n_meshes] = (uint32_t)tg.size();
-    for (const GroupEntry& G : tg) tl = std::max(tl, 4 * G.n_words);
+    // staged words + an 8-word seen set per token lane
+    for (const GroupEntry& G : tg) tl = std::max(tl, 4 * (G.n_words + 8 * (G.m1 - G.m0)));
     (void)hipFree(d_tgroups);
     d_tgroups = nullptr;
     n_tgroups = (uint32_t)tg.size();
@@ -482,7 +491,9 @@ struct DecodeSet {
       HIP_TRY(hipMalloc(&d_tgroups, sizeof(GroupEntry) * n_tgroups));
       HIP_TRY(hipMemcpy(d_tgroups, tg.data(), sizeof(GroupEntry) * n_tgroups, hipMemcpyHostToDevice));
     }
-    if (!d_walk) HIP_TRY(hipMalloc(&d_walk, 4 * std::max<uint64_t>(n_v, 1)));
+    // walks at 16-byte aligned slots (walk_slot): n_v + 4 words per meshlet,
+    // + 8 for the 4-entry reads past the last one
+    if (!d_walk) HIP_TRY(hipMalloc(&d_walk, 4 * (n_v + 4ull * n_m + 8)));
     return STRIDE_OK;
   }
 
@@ -497,7 +508,7 @@ struct DecodeSet {
     g.V = V;
     g.T = T;
     g.overflow = flags;
-    g.lds_bytes = (uint32_t)cfg.lds_bytes;
+    g.lds_bytes = group_lds;   // the largest group's need (<= the lds_bytes budget)
     g.grid = (uint32_t)cfg.grid;
     g.split = cfg.mode == 2 || cfg.mode == 4;
     g.narrow = narrow();
@@ -591,7 +602,8 @@ struct DecodeSet {
   int decode_graph(float* V, uint32_t* T, hipStream_t s) {
     if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
     auto key = std::make_tuple((void*)V, (void*)T, cfg.mode, cfg.waves, cfg.stage_words,
-                               cfg.grid, cfg.use_queue, cfg.group_size, cfg.lds_bytes);
+                               cfg.grid, cfg.use_queue, cfg.group_size,
+                               cfg.lds_bytes * 128 + cfg.tok_group);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
       if (!cap_stream) HIP_TRY(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
@@ -627,8 +639,11 @@ struct DecodeSet {
                   "group_size in [1,64] ([1,16] for modes 2 and 4), lds_bytes in [8192,65536] "
                   "(multiple of 8), grid >= 0, "
                   "use_queue needs a persistent grid");
+    const int tok_group = t->tok_group ? t->tok_group : DecodeConfig{}.tok_group;
+    if (tok_group < 1 || tok_group > 64) return fail(STRIDE_E_ARG, "tok_group must be in [1, 64]");
     const bool relayout = t->mode != cfg.mode || t->group_size != cfg.group_size ||
-                          t->lds_bytes != cfg.lds_bytes;
+                          t->lds_bytes != cfg.lds_bytes || tok_group != cfg.tok_group;
+    cfg.tok_group = tok_group;
     cfg.waves = t->waves;
     cfg.stage_words = t->stage_words;
     cfg.grid = t->grid;
@@ -648,6 +663,7 @@ struct DecodeSet {
     t->group_size = cfg.group_size;
     t->lds_bytes = cfg.lds_bytes;
     t->reserved = (int32_t)n_legacy;
+    t->tok_group = cfg.tok_group;
   }
 };
 
@@ -926,6 +942,9 @@ int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t) {
     if (t->mode != 0 && t->mode != 3 && t->mode != 4)
       return fail(STRIDE_E_ARG, "PDLT v5 modes: 3 (one kernel) or 4 (token pass + chain pass)");
     const int tok = t->stage_words ? 4 * t->stage_words : h->v5->tok_lds_budget;
+    const int tg = t->tok_group ? t->tok_group : 64;
+    if (tg < 1 || tg > 64) return fail(STRIDE_E_ARG, "tok_group must be in [1, 64]");
+    h->v5->tok_group = tg;
     const int rc = h->v5->set_layout(t->group_size, t->lds_bytes, tok);
     if (rc) return rc;
     if (t->mode) h->v5->two_pass = t->mode == 4;
@@ -942,6 +961,7 @@ int stride_decoder_get_tune(const stride_decoder* h, stride_tune* t) {
     t->group_size = h->v5->group_size;
     t->lds_bytes = h->v5->lds_budget;
     t->stage_words = h->v5->tok_lds_budget / 4;
+    t->tok_group = h->v5->tok_group;
     return STRIDE_OK;
   }
   h->set.get_tune(t);

@@ -10,7 +10,7 @@
 namespace stride {
 
 struct DecodeConfig {
-  int mode = 2;           // 0 = one wave per meshlet, 1 = groups (one lane per meshlet),
+  int mode = 4;           // 0 = one wave per meshlet, 1 = groups (one lane per meshlet),
                           // 2 = split groups (token lane + three axis lanes per meshlet),
                           // 4 = two passes: token lanes (64 meshlets per wave, walks to
                           //     HBM), then the axis lanes of the split groups
@@ -19,7 +19,8 @@ struct DecodeConfig {
   int grid = 0;           // 0 = one workgroup per unit of work, else persistent grid
   int use_queue = 0;      // mode 0: persistent grid pulls meshlets from an atomic counter
   int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; mode 2: 1..16
-  int lds_bytes = 24576;  // modes 1, 2: LDS per group (stage words + codes / walk blocks)
+  int lds_bytes = 16384;  // modes 1, 2, 4: LDS per group (stage words + codes / walk blocks)
+  int tok_group = 32;     // mode 4: meshlets per token-pass wave, 1..64
 };
 
 struct DecodeLaunch {

@@ -448,11 +448,13 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
 
 // ---------------------------------------------------------------------------
 // Two-pass split decode.  Pass 1: one token lane per meshlet, up to 64
-// meshlets per wave, seen set in registers; the walks go to HBM and the
-// triangles to the output.  Pass 2: the split groups' axis lanes on LDS
-// blocks that take the walk back from HBM, then the dequantisation.  The
-// token work no longer holds the groups' LDS blocks, so more meshlets are
-// resident per CU during the chains.
+// meshlets per wave, seen set in LDS after the staged words (an HBM-streamed
+// cursor measured slower: its refills sit on the serial token chain); the
+// walks go to HBM and the triangles to the output.  Pass 2: the split groups' axis lanes stream
+// their substreams and the walk from HBM (split_axis_g) and keep only the
+// code arrays in LDS, then the dequantisation.  With neither the token work
+// nor staged bits in their LDS, many more meshlets are resident per CU
+// during the chains.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void stride_tokens_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -470,8 +472,9 @@ __global__ __launch_bounds__(64) void stride_tokens_kernel(
     const MeshParams& P = meshes[e.mesh];
     const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
     const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
-    RegSeen seen;
-    split_tokens(lds32, rel, e, walk_g + P.v_base + e.v_off, seen, T + 3 * (P.t_base + e.t_off));
+    BitSeen seen{lds32 + G.n_words + lane, G.m1 - G.m0};
+    seen.preset(0);
+    split_tokens(lds32, rel, e, walk_g + walk_slot(P, e, m), seen, T + 3 * (P.t_base + e.t_off));
   }
 }
 
@@ -487,31 +490,31 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
   const uint32_t g = blockIdx.x;
   const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
   const GroupEntry G = groups[g];
-  for (uint32_t i = lane; i < G.n_words; i += 64) lds32[i] = __builtin_bswap32(words[G.w_lo + i]);
   const uint32_t m = G.m0 + (uint32_t)slot;
   const bool active = m < G.m1;
   MeshletEntry e;
   if (active) e = tab[m];
   const uint32_t nl = active ? e.n_local : 0u;
   uint32_t total = 0;
-  const uint32_t pre0 = wave_excl_scan(role == 0 ? split_block_words(nl, NARROW) : 0u, total);
+  // LDS holds only the code arrays: the bits stream from HBM, the walk too
+  const uint32_t pre0 = wave_excl_scan(role == 0 ? split_code_words(nl, NARROW) : 0u, total);
   const uint32_t pre = __shfl(pre0, slot, 64);
-  uint32_t* blk = lds32 + G.codes_off + pre;
-  CodeT* codes0 = reinterpret_cast<CodeT*>(blk + nl);
+  CodeT* codes0 = reinterpret_cast<CodeT*>(lds32 + pre);
   const uint32_t cstride = split_code_stride(nl, NARROW);
-  if (active) {
-    const MeshParams& P = meshes[e.mesh];
-    const uint32_t* ws = walk_g + P.v_base + e.v_off;
-    for (uint32_t i = (uint32_t)role; i < nl; i += 4) blk[i] = ws[i];
-  }
-  __syncthreads();
   if (active && role > 0) {
     const MeshParams& P = meshes[e.mesh];
-    const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
-    const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
     CodeT* codes = codes0 + cstride * (uint32_t)(role - 1);
-    if (!split_axis(lds32, G.n_words, rel, e, P, role - 1, blk, codes) && overflow)
-      overflow[g] = 1u;
+    const int d = role - 1;
+    const int32_t n0 = pick3(d, P.pred_n[0], P.pred_n[3], P.pred_n[6]);
+    const int32_t n1 = pick3(d, P.pred_n[1], P.pred_n[4], P.pred_n[7]);
+    const int32_t n2 = pick3(d, P.pred_n[2], P.pred_n[5], P.pred_n[8]);
+    const int32_t K = pick3(d, P.pred_K[0], P.pred_K[1], P.pred_K[2]);
+    const Walk4* wv = reinterpret_cast<const Walk4*>(walk_g + walk_slot(P, e, m));
+    // one loop per wave: 32-bit IRLP when it is exact for every lane
+    const bool ok = __all(NARROW && irlp32_exact(n0, n1, n2, K))
+                        ? split_axis_g<true>(words, P.blob_byte * 8ull, e, P, d, wv, codes)
+                        : split_axis_g<false>(words, P.blob_byte * 8ull, e, P, d, wv, codes);
+    if (!ok && overflow) overflow[g] = 1u;
   }
   __syncthreads();
   for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
@@ -519,7 +522,7 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
     const MeshParams& Pj = meshes[ej.mesh];
     const uint32_t pj = __shfl(pre0, (int)j, 64);
     const uint32_t n = ej.n_local, cs = split_code_stride(n, NARROW);
-    const CodeT* cb = reinterpret_cast<const CodeT*>(lds32 + G.codes_off + pj + n);
+    const CodeT* cb = reinterpret_cast<const CodeT*>(lds32 + pj);
     float* vo = V + 3 * (Pj.v_base + ej.v_off);
     for (uint32_t v = lane; v < n; v += 64) {
       const float x = dequant((int32_t)cb[v], Pj.g_min[0], Pj.g_range[0], Pj.g_bits[0], Pj.scale,

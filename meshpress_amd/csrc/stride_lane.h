@@ -280,30 +280,84 @@ struct ScratchSeen {
     return f;
   }
 };
-// In registers: 256 bits, selected without indexing.
+// In registers: 256 bits as eight words, read and updated with selects and
+// masks only (no indexing, no branches).
 struct RegSeen {
-  uint64_t s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+  uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  // ids [0, n) start as seen
+  STRIDE_DEV void preset(uint32_t n) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t lo = 32u * (uint32_t)q;
+      w[q] = n >= lo + 32 ? ~0u : (n <= lo ? 0u : ((1u << (n - lo)) - 1u));
+    }
+  }
   STRIDE_DEV bool fresh(uint32_t v) {
-    const uint32_t q = (v >> 6) & 3u;
-    const uint64_t bit = 1ull << (v & 63u);
-    const uint64_t w = q == 0 ? s0 : q == 1 ? s1 : q == 2 ? s2 : s3;
-    s0 |= q == 0 ? bit : 0ull;
-    s1 |= q == 1 ? bit : 0ull;
-    s2 |= q == 2 ? bit : 0ull;
-    s3 |= q == 3 ? bit : 0ull;
-    return (w & bit) == 0;
+    const uint32_t q = (v >> 5) & 7u;
+    const uint32_t bit = 1u << (v & 31u);
+    uint32_t cur = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const uint32_t m = (q == (uint32_t)i) ? bit : 0u;
+      cur |= w[i] & m;
+      w[i] |= m;
+    }
+    return cur == 0;
   }
 };
 
-// Token lane: strip tokens -> triangles (tout) and the walk.
+// In memory owned by one lane (LDS in the kernels): 8 words, updated with an
+// atomic or whose returned old word tells whether v was seen -- one LDS
+// instruction per token instead of a select over registers.
+struct BitSeen {
+  uint32_t* __restrict__ w;
+  // words between consecutive 32-id blocks: a wave's lanes interleave their
+  // sets (lane i at w + i, stride = lanes) so one access touches distinct banks
+  uint32_t stride = 1;
+  // ids [0, n) start as seen
+  STRIDE_DEV void preset(uint32_t n) {
+#pragma unroll
+    for (int q = 0; q < 8; ++q) {
+      const uint32_t lo = 32u * (uint32_t)q;
+      w[q * stride] = n >= lo + 32 ? ~0u : (n <= lo ? 0u : ((1u << (n - lo)) - 1u));
+    }
+  }
+  STRIDE_DEV bool fresh(uint32_t v) {
+    const uint32_t bit = 1u << (v & 31u);
+    const uint32_t old = __atomic_fetch_or(w + ((v >> 5) & 7u) * stride, bit, __ATOMIC_RELAXED);
+    return (old & bit) == 0;
+  }
+};
+
+// Bit cursors of the token lane: peek32() = the next 32 bits, advance(n)
+// consumes n <= 27 of them.
+struct LdsCursor {            // staged words in LDS
+  const uint32_t* __restrict__ st;
+  uint32_t pc;
+  STRIDE_DEV uint32_t peek32() const { return win32(st, pc); }
+  STRIDE_DEV void advance(uint32_t n) { pc += n; }
+};
+
+// Token lane: strip tokens -> triangles (tout) and the walk.  `cur` starts
+// at the first strip length.
+template <class Cursor, class Seen>
+STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ walk,
+                               Seen& seen, uint32_t* __restrict__ tout);
+
 template <class Seen>
 STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
                              uint32_t* __restrict__ walk, Seen& seen,
                              uint32_t* __restrict__ tout) {
+  LdsCursor cur{st, rel + 48};
+  split_tokens_c(cur, e, walk, seen, tout);
+}
+
+template <class Cursor, class Seen>
+STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ walk,
+                               Seen& seen, uint32_t* __restrict__ tout) {
   const uint32_t n_local = e.n_local, n_tris = e.n_tris;
   const uint32_t n_strips = e.n_strips & kStripMask;
   const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
-  uint32_t pc = rel + 48;
   uint32_t ib = 1;
   while ((1u << ib) < n_local + 1) ++ib;
   MtfLite fifo;
@@ -313,9 +367,14 @@ STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, cons
   const uint32_t v_off = e.v_off;
   const uint32_t n_tok = n_tris + 2 * n_strips;
   uint32_t* o = tout;
+  // strip length (16 bits) + edge bit + flag + id (<= 9 bits): one 32-bit
+  // window per token.  The next token's window is read as soon as this
+  // token's length is known, so its latency hides under the FIFO work; the
+  // read after the last token stays inside the meshlet's anchor and
+  // substream bits or the buffer space after them.
+  uint32_t Wn = cur.peek32();
   for (uint32_t it = 0; it < n_tok; ++it) {
-    // strip length (16 bits) + edge bit + flag + id (<= 9 bits): one 32-bit window
-    uint32_t W = win32(st, pc);
+    uint32_t W = Wn;
     const bool start = root_j == 3 && steps_left == 0;
     steps_left = start ? (W >> 16) - 1 : steps_left;
     W = start ? (W << 16) : W;
@@ -325,7 +384,8 @@ STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, cons
     W = is_root ? W : (W << 1);
     const uint32_t flag = W >> 31;
     const uint32_t val = flag ? ((W << 1) >> (32 - ib)) : ((W >> 26) & 31u);
-    pc += (start ? 16u : 0u) + (is_root ? 1u : 2u) + (flag ? ib : 5u);
+    cur.advance((start ? 16u : 0u) + (is_root ? 1u : 2u) + (flag ? ib : 5u));
+    Wn = cur.peek32();
     // reuse FIFO: a hit names FIFO index val (oldest first), a raw id is
     // appended (the oldest entry leaving a full FIFO); either way the value
     // becomes the most recent entry
@@ -420,6 +480,216 @@ STRIDE_DEV bool split_axis(const uint32_t* __restrict__ st, uint32_t n_words, ui
     ok &= code_fits<CodeT>(c);
     codes[w & 0xFFu] = (CodeT)c;
     prev = c;
+  }
+  return ok;
+}
+
+// ---------------------------------------------------------------------------
+// Streaming reads from the stream in global memory (the two-pass axis
+// lanes): no LDS staging.  A lane reads its substream sequentially through a
+// 64-bit window holding at least 33 valid bits, with the next word already
+// requested, so the load latency overlaps the decoding of the bits before
+// it.  The stream buffer is zero padded past its end, so the look-ahead load
+// stays inside the allocation.
+// ---------------------------------------------------------------------------
+STRIDE_DEV uint32_t load_be(const uint32_t* __restrict__ g, uint64_t i) {
+  return __builtin_bswap32(g[i]);
+}
+
+// n <= 32 bits at absolute bit position p of the little-endian words g.
+STRIDE_DEV uint32_t bits_at(const uint32_t* __restrict__ g, uint64_t p, uint32_t n) {
+  const uint64_t w = p >> 5;
+  const uint64_t x = (((uint64_t)load_be(g, w) << 32) | load_be(g, w + 1)) << (p & 31);
+  return n ? (uint32_t)(x >> (64 - n)) : 0u;
+}
+
+struct StreamBits {
+  const uint32_t* __restrict__ g;
+  uint64_t win;      // valid bits, top aligned; zeros below
+  uint64_t next;     // index of the next word to request
+  uint32_t valid;    // 33..64 between reads
+  uint32_t ahead;    // word next - 1, byte-swapped, not yet in win
+
+  STRIDE_DEV void init(const uint32_t* __restrict__ words, uint64_t bit) {
+    g = words;
+    const uint64_t w0 = bit >> 5;
+    const uint32_t sh = (uint32_t)(bit & 31);
+    win = (((uint64_t)load_be(g, w0) << 32) | load_be(g, w0 + 1)) << sh;
+    valid = 64 - sh;
+    ahead = load_be(g, w0 + 2);
+    next = w0 + 3;
+    fill();
+  }
+  STRIDE_DEV void fill() {
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+      if (valid <= 32) {
+        win |= (uint64_t)ahead << (32 - valid);
+        valid += 32;
+        ahead = load_be(g, next++);
+      }
+  }
+  // drops n <= 33 bits
+  STRIDE_DEV void drop(uint32_t n) {
+    win = n < 64 ? (win << n) : 0ull;
+    valid -= n;
+    fill();
+  }
+  STRIDE_DEV uint32_t bits(uint32_t n) {   // n <= 32
+    const uint32_t v = n ? (uint32_t)(win >> (64 - n)) : 0u;
+    drop(n);
+    return v;
+  }
+  // token-lane cursor interface
+  STRIDE_DEV uint32_t peek32() const { return (uint32_t)(win >> 32); }
+  STRIDE_DEV void advance(uint32_t n) { drop(n); }
+};
+
+// codeword() on a stream.
+STRIDE_DEV int64_t codeword_s(StreamBits& s, bool eg, int k) {
+  const uint64_t W = s.win;
+  const uint32_t q = W ? (uint32_t)__builtin_clzll(W) : 64u;
+  const uint32_t nb = (eg ? q : 0u) + (uint32_t)k;
+  const uint32_t len = q + 1 + nb;
+  uint64_t u;
+  if (len <= 33) {
+    const uint64_t body = nb ? ((W << (q + 1)) >> (64 - nb)) : 0ull;
+    const uint64_t off = body >> k;
+    const uint64_t low = body & ((1ull << k) - 1);
+    const uint64_t shifted = eg ? (((1ull << q) - 1) + off) : (uint64_t)q;
+    u = (shifted << k) | low;
+    s.drop(len);
+  } else {
+    // long codeword (outliers): the unary part bit by bit, then the rest
+    uint32_t z = 0;
+    while (s.bits(1) == 0) ++z;
+    uint64_t shifted = z;
+    if (eg) shifted = z ? (((1ull << z) - 1) + s.bits(z)) : 0ull;
+    const uint64_t low = k ? s.bits((uint32_t)k) : 0ull;
+    u = (shifted << k) | low;
+  }
+  return unzigzag(u);
+}
+
+// The short form of codeword_s on a window W (>= 33 valid bits): the
+// codeword's length with u = its zigzagged value, or 0 for a codeword longer
+// than 33 bits (decoded by codeword_s instead).
+STRIDE_DEV uint32_t codeword_win(uint64_t W, bool eg, int k, uint64_t& u) {
+  const uint32_t q = W ? (uint32_t)__builtin_clzll(W) : 64u;
+  const uint32_t nb = (eg ? q : 0u) + (uint32_t)k;
+  const uint32_t len = q + 1 + nb;
+  const uint64_t body = nb ? ((W << (q + 1)) >> (64 - nb)) : 0ull;
+  const uint64_t off = body >> k;
+  const uint64_t low = body & ((1ull << k) - 1);
+  const uint64_t shifted = eg ? (((1ull << q) - 1) + off) : (uint64_t)q;
+  u = (shifted << k) | low;
+  return len <= 33 ? len : 0u;
+}
+
+// IRLP in 32-bit arithmetic (24-bit multiplies): equal to irlp64 when the
+// codes are 16-bit unsigned and irlp32_exact(n0, n1, n2, K) holds.
+// The int16 numerators and 16-bit codes make every product a 24-bit multiply.
+STRIDE_DEV int32_t irlp24(int32_t a, int32_t b, int32_t c, int32_t n0, int32_t n1, int32_t n2,
+                          int32_t K, int32_t rnd) {
+  return ((int32_t)(int16_t)n0 * a + (int32_t)(int16_t)n1 * b + (int32_t)(int16_t)n2 * c + rnd) >>
+         K;
+}
+// int16 numerators, K <= 30 and sum |n_i| * 65535 + 2^(K-1) < 2^31: no
+// product or partial sum of irlp24 leaves int32.
+STRIDE_DEV bool irlp32_exact(int32_t n0, int32_t n1, int32_t n2, int32_t K) {
+  auto mag = [](int32_t x) { return (int64_t)(x < 0 ? -(int64_t)x : (int64_t)x); };
+  const bool small = n0 == (int16_t)n0 && n1 == (int16_t)n1 && n2 == (int16_t)n2;
+  const int64_t top = (mag(n0) + mag(n1) + mag(n2)) * 65535 + (K > 0 ? (1ll << (K - 1)) : 0);
+  return small && K >= 0 && K <= 30 && top <= 0x7FFFFFFFll;
+}
+
+// Walk slot of global meshlet m in the two-pass walk buffer: its first
+// vertex rounded up to 4 words, + 4 words per preceding meshlet, so every
+// slot is 16-byte aligned and a 4-entry read past a slot's end stays inside
+// the next slot or the 8 words after the last.
+STRIDE_DEV uint64_t walk_slot(const MeshParams& P, const MeshletEntry& e, uint32_t m) {
+  return ((P.v_base + e.v_off + 3) & ~3ull) + 4ull * m;
+}
+struct alignas(16) Walk4 {
+  uint32_t x, y, z, w;
+};
+
+// Code storage words of one meshlet in the two-pass axis pass (the three
+// code arrays only; the walk stays in HBM).
+STRIDE_DEV uint32_t split_code_words(uint32_t n_local, bool narrow) {
+  return narrow ? 3 * ((n_local + 1) / 2) : 3 * n_local;
+}
+
+// split_axis over the stream in global memory: `words` are the stream words,
+// `base` the bit offset of the mesh's blob in them, `walk` the meshlet's walk
+// in HBM (read four entries ahead).
+template <bool I32, class CodeT>
+STRIDE_DEV bool split_axis_g(const uint32_t* __restrict__ words, uint64_t base,
+                             const MeshletEntry& e, const MeshParams& P, int d,
+                             const Walk4* __restrict__ walk, CodeT* __restrict__ codes) {
+  const uint32_t n_local = e.n_local;
+  const uint32_t gb0 = (uint32_t)P.g_bits[0], gb1 = (uint32_t)P.g_bits[1],
+                 gb2 = (uint32_t)P.g_bits[2];
+  const uint32_t gb = pick3(d, gb0, gb1, gb2);
+  const uint64_t r0 = base + e.resid_bit;
+  const int64_t anchor = (int64_t)bits_at(words, r0 + (d > 0 ? gb0 : 0u) + (d > 1 ? gb1 : 0u), gb);
+  // (tag, k) of delta-d at tag byte pair d, of para-d at pair 3 + d
+  const uint64_t tb = r0 + gb0 + gb1 + gb2;
+  const uint32_t TD = bits_at(words, tb + 16u * (uint32_t)d, 16);
+  const uint32_t TP = bits_at(words, tb + 16u * (uint32_t)(3 + d), 16);
+  const bool deg = (TD >> 8) != 1u, peg = (TP >> 8) != 1u;
+  const int dk = (int)(TD & 31u), pk = (int)(TP & 31u);
+  StreamBits sd, sp;
+  sd.init(words, base + pick3(d, e.sub_bit[0], e.sub_bit[1], e.sub_bit[2]));
+  sp.init(words, base + pick3(d, e.sub_bit[3], e.sub_bit[4], e.sub_bit[5]));
+  const int32_t n0 = (int32_t)pick3(d, (uint32_t)P.pred_n[0], (uint32_t)P.pred_n[3], (uint32_t)P.pred_n[6]);
+  const int32_t n1 = (int32_t)pick3(d, (uint32_t)P.pred_n[1], (uint32_t)P.pred_n[4], (uint32_t)P.pred_n[7]);
+  const int32_t n2 = (int32_t)pick3(d, (uint32_t)P.pred_n[2], (uint32_t)P.pred_n[5], (uint32_t)P.pred_n[8]);
+  const int32_t K = (int32_t)pick3(d, (uint32_t)P.pred_K[0], (uint32_t)P.pred_K[1], (uint32_t)P.pred_K[2]);
+  const int32_t rnd = (I32 && K > 0) ? (1 << (K - 1)) : 0;
+  bool ok = true;
+  int64_t prev = anchor;
+  // the walk slot is 16-byte aligned, with readable words past its end
+  Walk4 wq = walk[0];
+  codes[wq.x & 0xFFu] = (CodeT)anchor;
+  ok &= code_fits<CodeT>(anchor);
+  for (uint32_t i0 = 0; i0 < n_local; i0 += 4) {
+    const Walk4 nq = walk[(i0 >> 2) + 1];
+    const uint32_t wb[4] = {wq.x, wq.y, wq.z, wq.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t i = i0 + (uint32_t)j;
+      if (i >= 1 && i < n_local) {
+        const uint32_t w = wb[j];
+        const bool root = (w & 0xFFFFFF00u) == kRootMark;
+        // the code comes from the delta stream (strip roots) or the
+        // parallelogram stream; only the chosen one advances
+        uint64_t u;
+        const uint32_t len = codeword_win(root ? sd.win : sp.win, root ? deg : peg, root ? dk : pk, u);
+        int64_t r;
+        if (len) {
+          sd.drop(root ? len : 0u);
+          sp.drop(root ? 0u : len);
+          r = unzigzag(u);
+        } else {
+          r = root ? codeword_s(sd, deg, dk) : codeword_s(sp, peg, pk);
+        }
+        const uint32_t ia = root ? 0u : (w >> 8) & 0xFFu, ib = root ? 0u : (w >> 16) & 0xFFu,
+                       ic = root ? 0u : w >> 24;
+        int64_t pred;
+        if constexpr (I32 && sizeof(CodeT) == 2) {
+          pred = (int64_t)irlp24((int32_t)codes[ia], (int32_t)codes[ib], (int32_t)codes[ic], n0, n1,
+                                 n2, K, rnd);
+        } else {
+          pred = irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1, n2, K);
+        }
+        const int64_t c = (root ? prev : pred) + r;
+        ok &= code_fits<CodeT>(c);
+        codes[w & 0xFFu] = (CodeT)c;
+        prev = c;
+      }
+    }
+    wq = nq;
   }
   return ok;
 }

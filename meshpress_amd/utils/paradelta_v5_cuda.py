@@ -114,8 +114,9 @@ class ParaDeltaV5GpuDecoder:
         return t.as_dict()
 
     def set_tune(self, **kw) -> None:
-        """Launch shape: mode (1 = lane-per-meshlet groups, 0 = wave per
-        meshlet), group_size, lds_bytes, waves, stage_words, grid, use_queue."""
+        """Launch shape: mode (4 = two passes, 2 = split groups, 1 =
+        lane-per-meshlet groups, 0 = wave per meshlet; 3 or 4 for PDLT v5),
+        group_size, lds_bytes, tok_group, waves, stage_words, grid, use_queue."""
         t = _lib.make_tune(self.get_tune(), kw)
         _lib.check(self._L.stride_decoder_set_tune(self._h, C.byref(t)))
 

@@ -12,6 +12,7 @@
 // hold only the copied walk.
 // Exit status: 0 ok, 2 stream rejected by the pre-pass, 3 code overflow
 // (the kernels route such groups to the wave-per-meshlet decoder).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -98,24 +99,53 @@ int main(int argc, char** argv) {
   // two-pass: every token lane first (register seen set, walks to one
   // exactly sized array), then the axis lanes over blocks holding stale data
   // except for the copied walk
-  std::vector<uint32_t> walk_g(two ? nv : 0);
+  // the walk buffer of the device: 16-byte aligned slots (walk_slot)
+  std::vector<Walk4> walk4(two ? (nv + 4ull * n_m + 8) / 4 + 1 : 0);
+  uint32_t* walk_g = reinterpret_cast<uint32_t*>(walk4.data());
   if (two)
     for (uint32_t m = 0; m < n_m; ++m) {
       const MeshletEntry& e = tab[m];
-      RegSeen seen;
-      split_tokens(st.data(), e.start_bit, e, walk_g.data() + e.v_off, seen,
-                   tris.data() + 3ull * e.t_off);
+      uint32_t sw[8];
+      BitSeen seen{sw};
+      seen.preset(0);
+      StreamBits cur;
+      cur.init(le.data(), e.start_bit + 48);
+      split_tokens_c(cur, e, walk_g + walk_slot(P, e, m), seen, tris.data() + 3ull * e.t_off);
     }
   for (uint32_t m = 0; m < n_m; ++m) {
     const MeshletEntry& e = tab[m];
     // one group = the whole stream staged: rel is the absolute start bit
-    if (split && narrow) {
+    if (two) {
+      // the two-pass axis lanes: bits and walk from "HBM", codes-only block
+      std::vector<uint32_t> blk(split_code_words(e.n_local, narrow), 0xA5A5A5A5u);
+      const uint32_t cs = split_code_stride(e.n_local, narrow);
+      const Walk4* wv = reinterpret_cast<const Walk4*>(walk_g + walk_slot(P, e, m));
+      for (int d = 0; d < 3; ++d) {
+        bool okd;
+        if (narrow) {
+          uint16_t* c = reinterpret_cast<uint16_t*>(blk.data()) + cs * d;
+          okd = split_axis_g<false>(le.data(), 0, e, P, d, wv, c);
+          if (irlp32_exact(P.pred_n[3 * d], P.pred_n[3 * d + 1], P.pred_n[3 * d + 2], P.pred_K[d])) {
+            // the 32-bit IRLP loop gives the same codes
+            std::vector<uint16_t> c2(e.n_local, 0x5A5Au);
+            const bool ok2 = split_axis_g<true>(le.data(), 0, e, P, d, wv, c2.data());
+            if (ok2 != okd || (okd && !std::equal(c2.begin(), c2.end(), c))) rc = 4;
+          }
+          for (uint32_t v = 0; v < e.n_local; ++v) codes3[3 * ((size_t)e.v_off + v) + d] = c[v];
+        } else {
+          int32_t* c = reinterpret_cast<int32_t*>(blk.data()) + cs * d;
+          okd = split_axis_g<false>(le.data(), 0, e, P, d, wv, c);
+          for (uint32_t v = 0; v < e.n_local; ++v) codes3[3 * ((size_t)e.v_off + v) + d] = c[v];
+        }
+        if (!okd) rc = 3;
+      }
+    } else if (split && narrow) {
       std::vector<uint32_t> blk(split_block_words(e.n_local, true), two ? 0xA5A5A5A5u : 0u);
       uint16_t* c16 = reinterpret_cast<uint16_t*>(blk.data() + e.n_local);
       const uint32_t cs = split_code_stride(e.n_local, true);
       ScratchSeen<uint16_t> seen{c16};
       if (two)
-        for (uint32_t v = 0; v < e.n_local; ++v) blk[v] = walk_g.at((size_t)e.v_off + v);
+        for (uint32_t v = 0; v < e.n_local; ++v) blk[v] = walk_g[walk_slot(P, e, m) + v];
       else
         split_tokens(st.data(), e.start_bit, e, blk.data(), seen, tris.data() + 3ull * e.t_off);
       for (int d = 0; d < 3; ++d) {
@@ -129,7 +159,7 @@ int main(int argc, char** argv) {
       int32_t* c32 = reinterpret_cast<int32_t*>(blk.data() + e.n_local);
       ScratchSeen<int32_t> seen{c32};
       if (two)
-        for (uint32_t v = 0; v < e.n_local; ++v) blk[v] = walk_g.at((size_t)e.v_off + v);
+        for (uint32_t v = 0; v < e.n_local; ++v) blk[v] = walk_g[walk_slot(P, e, m) + v];
       else
         split_tokens(st.data(), e.start_bit, e, blk.data(), seen, tris.data() + 3ull * e.t_off);
       for (int d = 0; d < 3; ++d) {

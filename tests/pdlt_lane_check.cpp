@@ -62,7 +62,7 @@ int main(int argc, char** argv) {
   if (two_pass) {
     std::vector<PdltGroup> tg;
     uint32_t tlds = 0;
-    rc = pdlt_layout_tokens(H.tab, budget, tg, tlds, err);
+    rc = pdlt_layout_tokens(H.tab, budget, 64, tg, tlds, err);
     if (rc) {
       std::fprintf(stderr, "token layout (%d): %s\n", rc, err.c_str());
       return 2;
@@ -73,9 +73,14 @@ int main(int argc, char** argv) {
       for (uint32_t i = 0; i < G.n_words; ++i) lds[i] = __builtin_bswap32(H.words.at(G.w_lo + i));
       for (uint32_t m = G.m0; m < G.m1; ++m) {
         const PdltEntry& e = H.tab[m];
-        uint32_t* seen = seen_g.data() + (size_t)kSeenWords * m;
-        pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, lds + G.blk_base + e.tok_ref_off,
-                    walk_g.data() + (e.int_base - H.n_b), seen, seen + 8, T.data() + 3ull * e.t_off);
+        uint32_t* refs = lds + G.blk_base + e.tok_ref_off;
+        uint32_t* seen = refs + e.n_bnd;
+        uint32_t n_emit = 0;
+        pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g.data() + (e.int_base - H.n_b), seen,
+                    &n_emit, T.data() + 3ull * e.t_off);
+        uint32_t* so = seen_g.data() + (size_t)kSeenWords * m;
+        for (int q = 0; q < 8; ++q) so[q] = seen[q];
+        so[8] = n_emit;
       }
     }
   }
