@@ -330,8 +330,8 @@ struct DecodeSet {
         const bool nar = narrow();
         const uint32_t gmax = split ? std::min(cfg.group_size, 16) : cfg.group_size;
         // bytes of LDS: staged words + per-meshlet code storage ("codes"
-        // counts vertices in mode 1, block words in mode 2, code words only
-        // in mode 4, whose axis pass stages no bits)
+        // counts vertices in mode 1, block words in mode 2, code words and
+        // staged axis words in mode 4, whose axis pass stages nothing else)
         auto need = [split, two](uint64_t words, uint64_t cu) {
           return two ? 4 * cu
                      : (split ? 4 * ((words + 3) & ~3ull) + 4 * cu : 8 * ((words + 1) / 2) + 8 * cu);
@@ -358,7 +358,7 @@ struct DecodeSet {
             const MeshletEntry& e = tab[j];
             const uint64_t lo = (bb + e.start_bit) >> 5;
             const uint64_t hi = ((bb + e.end_bit + 31) >> 5) + 2;
-            const uint64_t unit = two ? split_code_words(e.n_local, nar)
+            const uint64_t unit = two ? split_code_words(e.n_local, nar) + axis_stage_words(bb, e)
                                       : (split ? split_block_words(e.n_local, nar) : e.n_local);
             if (open && (j - cur.m0 < gmax) && need(hi - cur.w_lo, codes + unit) <= budget) {
               cur.m1 = j + 1;

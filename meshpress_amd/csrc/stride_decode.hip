@@ -450,11 +450,14 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
 // Two-pass split decode.  Pass 1: one token lane per meshlet, up to 64
 // meshlets per wave, seen set in LDS after the staged words (an HBM-streamed
 // cursor measured slower: its refills sit on the serial token chain); the
-// walks go to HBM and the triangles to the output.  Pass 2: the split groups' axis lanes stream
-// their substreams and the walk from HBM (split_axis_g) and keep only the
-// code arrays in LDS, then the dequantisation.  With neither the token work
-// nor staged bits in their LDS, many more meshlets are resident per CU
-// during the chains.
+// walks go to HBM and the triangles to the output.  Pass 2: the split
+// groups' axis lanes.  The wave stages only each meshlet's residual section
+// (anchor, coder pairs, substreams: about half of its bits) in LDS after its
+// code arrays, then each axis lane runs its chain from there (split_axis_g)
+// with the walk read from HBM four entries ahead, then the dequantisation.
+// With no token bits or walk in LDS, every meshlet of the mesh is resident
+// at once; with the substreams in LDS, the only global loads of the chains
+// are the walk reads, issued a group of four codes before their use.
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(64) void stride_tokens_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -495,14 +498,42 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
   MeshletEntry e;
   if (active) e = tab[m];
   const uint32_t nl = active ? e.n_local : 0u;
+  const MeshParams* Pm = meshes + (active ? e.mesh : 0u);
+  const uint64_t bb = active ? Pm->blob_byte * 8ull : 0ull;
+  // LDS per meshlet: its three code arrays, then its axis words (the walk
+  // stays in HBM)
+  const uint64_t alo = active ? axis_stage_lo(bb, e) : 0ull;
+  const uint32_t an = active ? axis_stage_words(bb, e) : 0u;
+  const uint32_t cw = split_code_words(nl, NARROW);
   uint32_t total = 0;
-  // LDS holds only the code arrays: the bits stream from HBM, the walk too
-  const uint32_t pre0 = wave_excl_scan(role == 0 ? split_code_words(nl, NARROW) : 0u, total);
+  const uint32_t pre0 = wave_excl_scan(role == 0 ? cw + an : 0u, total);
   const uint32_t pre = __shfl(pre0, slot, 64);
+  uint32_t amax = an;   // the same in every role: the max over the 16 slots
+  for (int o = 8; o >= 1; o >>= 1) amax = max(amax, (uint32_t)__shfl_xor((int)amax, o, 64));
+  // stage the axis words of every meshlet of the group: all the loads of a
+  // 64-word row first, then the stores (slots past the group load word 0)
+  for (uint32_t b0 = 0; b0 < amax; b0 += 64) {
+    const uint32_t i = b0 + (uint32_t)lane;
+    uint32_t t[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t nj = __shfl(an, j, 64);
+      const uint64_t lj = ((uint64_t)(uint32_t)__shfl((int)(alo >> 32), j, 64) << 32) |
+                          (uint32_t)__shfl((int)(uint32_t)alo, j, 64);
+      t[j] = words[i < nj ? lj + i : lj];
+    }
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      const uint32_t nj = __shfl(an, j, 64);
+      const uint32_t oj = __shfl(pre0, j, 64) + __shfl(cw, j, 64);
+      if (i < nj) lds32[oj + i] = __builtin_bswap32(t[j]);
+    }
+  }
+  __syncthreads();
   CodeT* codes0 = reinterpret_cast<CodeT*>(lds32 + pre);
   const uint32_t cstride = split_code_stride(nl, NARROW);
   if (active && role > 0) {
-    const MeshParams& P = meshes[e.mesh];
+    const MeshParams& P = *Pm;
     CodeT* codes = codes0 + cstride * (uint32_t)(role - 1);
     const int d = role - 1;
     const int32_t n0 = pick3(d, P.pred_n[0], P.pred_n[3], P.pred_n[6]);
@@ -510,10 +541,11 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
     const int32_t n2 = pick3(d, P.pred_n[2], P.pred_n[5], P.pred_n[8]);
     const int32_t K = pick3(d, P.pred_K[0], P.pred_K[1], P.pred_K[2]);
     const Walk4* wv = reinterpret_cast<const Walk4*>(walk_g + walk_slot(P, e, m));
+    const LdsStaged src{lds32 + pre + cw, alo};
     // one loop per wave: 32-bit IRLP when it is exact for every lane
     const bool ok = __all(NARROW && irlp32_exact(n0, n1, n2, K))
-                        ? split_axis_g<true>(words, P.blob_byte * 8ull, e, P, d, wv, codes)
-                        : split_axis_g<false>(words, P.blob_byte * 8ull, e, P, d, wv, codes);
+                        ? split_axis_g<true>(src, bb, e, P, d, wv, codes)
+                        : split_axis_g<false>(src, bb, e, P, d, wv, codes);
     if (!ok && overflow) overflow[g] = 1u;
   }
   __syncthreads();

@@ -496,27 +496,48 @@ STRIDE_DEV uint32_t load_be(const uint32_t* __restrict__ g, uint64_t i) {
   return __builtin_bswap32(g[i]);
 }
 
-// n <= 32 bits at absolute bit position p of the little-endian words g.
-STRIDE_DEV uint32_t bits_at(const uint32_t* __restrict__ g, uint64_t p, uint32_t n) {
+// Word sources of a bit stream, indexed by absolute word of the device
+// buffer: the little-endian buffer itself (byte-swapped per read), or words
+// staged MSB-first in LDS from word `lo` on.
+struct BufferWords {
+  const uint32_t* __restrict__ g;
+  STRIDE_DEV uint32_t operator()(uint64_t i) const { return load_be(g, i); }
+};
+struct LdsStaged {
+  const uint32_t* __restrict__ s;
+  uint64_t lo;
+  STRIDE_DEV uint32_t operator()(uint64_t i) const { return s[i - lo]; }
+};
+
+// n <= 32 bits at absolute bit position p of a word source.
+template <class Src>
+STRIDE_DEV uint32_t bits_at_src(const Src& src, uint64_t p, uint32_t n) {
   const uint64_t w = p >> 5;
-  const uint64_t x = (((uint64_t)load_be(g, w) << 32) | load_be(g, w + 1)) << (p & 31);
+  const uint64_t x = (((uint64_t)src(w) << 32) | src(w + 1)) << (p & 31);
   return n ? (uint32_t)(x >> (64 - n)) : 0u;
 }
+// n <= 32 bits at absolute bit position p of the little-endian words g.
+STRIDE_DEV uint32_t bits_at(const uint32_t* __restrict__ g, uint64_t p, uint32_t n) {
+  return bits_at_src(BufferWords{g}, p, n);
+}
 
-struct StreamBits {
-  const uint32_t* __restrict__ g;
+// A sequential reader: a 64-bit window with >= 33 valid bits and the next
+// word read ahead, so a refill never waits on the read it needs.
+template <class Src>
+struct BitStream {
+  Src src;
   uint64_t win;      // valid bits, top aligned; zeros below
   uint64_t next;     // index of the next word to request
   uint32_t valid;    // 33..64 between reads
-  uint32_t ahead;    // word next - 1, byte-swapped, not yet in win
+  uint32_t ahead;    // word next - 1, not yet in win
 
-  STRIDE_DEV void init(const uint32_t* __restrict__ words, uint64_t bit) {
-    g = words;
+  STRIDE_DEV void init(const Src& s, uint64_t bit) {
+    src = s;
     const uint64_t w0 = bit >> 5;
     const uint32_t sh = (uint32_t)(bit & 31);
-    win = (((uint64_t)load_be(g, w0) << 32) | load_be(g, w0 + 1)) << sh;
+    win = (((uint64_t)src(w0) << 32) | src(w0 + 1)) << sh;
     valid = 64 - sh;
-    ahead = load_be(g, w0 + 2);
+    ahead = src(w0 + 2);
     next = w0 + 3;
     fill();
   }
@@ -526,7 +547,7 @@ struct StreamBits {
       if (valid <= 32) {
         win |= (uint64_t)ahead << (32 - valid);
         valid += 32;
-        ahead = load_be(g, next++);
+        ahead = src(next++);
       }
   }
   // drops n <= 33 bits
@@ -545,8 +566,16 @@ struct StreamBits {
   STRIDE_DEV void advance(uint32_t n) { drop(n); }
 };
 
+// The stream over the little-endian device buffer.
+struct StreamBits : BitStream<BufferWords> {
+  STRIDE_DEV void init(const uint32_t* __restrict__ words, uint64_t bit) {
+    BitStream<BufferWords>::init(BufferWords{words}, bit);
+  }
+};
+
 // codeword() on a stream.
-STRIDE_DEV int64_t codeword_s(StreamBits& s, bool eg, int k) {
+template <class Stream>
+STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
   const uint64_t W = s.win;
   const uint32_t q = W ? (uint32_t)__builtin_clzll(W) : 64u;
   const uint32_t nb = (eg ? q : 0u) + (uint32_t)k;
@@ -603,6 +632,16 @@ STRIDE_DEV bool irlp32_exact(int32_t n0, int32_t n1, int32_t n2, int32_t K) {
   return small && K >= 0 && K <= 30 && top <= 0x7FFFFFFFll;
 }
 
+// The words an axis lane pass reads of meshlet e: from the word holding its
+// residual header (resid_bit) through 5 words past the one holding end_bit
+// (the three read-ahead words of a stream that ends there, and one more).
+STRIDE_DEV uint64_t axis_stage_lo(uint64_t blob_bit, const MeshletEntry& e) {
+  return (blob_bit + e.resid_bit) >> 5;
+}
+STRIDE_DEV uint32_t axis_stage_words(uint64_t blob_bit, const MeshletEntry& e) {
+  return (uint32_t)(((blob_bit + e.end_bit) >> 5) + 5 - axis_stage_lo(blob_bit, e));
+}
+
 // Walk slot of global meshlet m in the two-pass walk buffer: its first
 // vertex rounded up to 4 words, + 4 words per preceding meshlet, so every
 // slot is 16-byte aligned and a 4-entry read past a slot's end stays inside
@@ -620,11 +659,12 @@ STRIDE_DEV uint32_t split_code_words(uint32_t n_local, bool narrow) {
   return narrow ? 3 * ((n_local + 1) / 2) : 3 * n_local;
 }
 
-// split_axis over the stream in global memory: `words` are the stream words,
-// `base` the bit offset of the mesh's blob in them, `walk` the meshlet's walk
+// split_axis over a word source indexed by absolute buffer word (the device
+// buffer, or the meshlet's axis words staged in LDS: axis_stage_words),
+// `base` the bit offset of the mesh's blob, `walk` the meshlet's walk slot
 // in HBM (read four entries ahead).
-template <bool I32, class CodeT>
-STRIDE_DEV bool split_axis_g(const uint32_t* __restrict__ words, uint64_t base,
+template <bool I32, class Src, class CodeT>
+STRIDE_DEV bool split_axis_g(const Src& words, uint64_t base,
                              const MeshletEntry& e, const MeshParams& P, int d,
                              const Walk4* __restrict__ walk, CodeT* __restrict__ codes) {
   const uint32_t n_local = e.n_local;
@@ -632,14 +672,14 @@ STRIDE_DEV bool split_axis_g(const uint32_t* __restrict__ words, uint64_t base,
                  gb2 = (uint32_t)P.g_bits[2];
   const uint32_t gb = pick3(d, gb0, gb1, gb2);
   const uint64_t r0 = base + e.resid_bit;
-  const int64_t anchor = (int64_t)bits_at(words, r0 + (d > 0 ? gb0 : 0u) + (d > 1 ? gb1 : 0u), gb);
+  const int64_t anchor = (int64_t)bits_at_src(words, r0 + (d > 0 ? gb0 : 0u) + (d > 1 ? gb1 : 0u), gb);
   // (tag, k) of delta-d at tag byte pair d, of para-d at pair 3 + d
   const uint64_t tb = r0 + gb0 + gb1 + gb2;
-  const uint32_t TD = bits_at(words, tb + 16u * (uint32_t)d, 16);
-  const uint32_t TP = bits_at(words, tb + 16u * (uint32_t)(3 + d), 16);
+  const uint32_t TD = bits_at_src(words, tb + 16u * (uint32_t)d, 16);
+  const uint32_t TP = bits_at_src(words, tb + 16u * (uint32_t)(3 + d), 16);
   const bool deg = (TD >> 8) != 1u, peg = (TP >> 8) != 1u;
   const int dk = (int)(TD & 31u), pk = (int)(TP & 31u);
-  StreamBits sd, sp;
+  BitStream<Src> sd, sp;
   sd.init(words, base + pick3(d, e.sub_bit[0], e.sub_bit[1], e.sub_bit[2]));
   sp.init(words, base + pick3(d, e.sub_bit[3], e.sub_bit[4], e.sub_bit[5]));
   const int32_t n0 = (int32_t)pick3(d, (uint32_t)P.pred_n[0], (uint32_t)P.pred_n[3], (uint32_t)P.pred_n[6]);

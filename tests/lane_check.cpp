@@ -120,21 +120,27 @@ int main(int argc, char** argv) {
       std::vector<uint32_t> blk(split_code_words(e.n_local, narrow), 0xA5A5A5A5u);
       const uint32_t cs = split_code_stride(e.n_local, narrow);
       const Walk4* wv = reinterpret_cast<const Walk4*>(walk_g + walk_slot(P, e, m));
+      // the kernel's LDS copy of the meshlet's axis words, exactly sized
+      const uint64_t alo = axis_stage_lo(0, e);
+      std::vector<uint32_t> ax(st.begin() + alo, st.begin() + alo + axis_stage_words(0, e));
+      const LdsStaged sw{ax.data(), alo};
+      const BufferWords gw{le.data()};
       for (int d = 0; d < 3; ++d) {
         bool okd;
         if (narrow) {
           uint16_t* c = reinterpret_cast<uint16_t*>(blk.data()) + cs * d;
-          okd = split_axis_g<false>(le.data(), 0, e, P, d, wv, c);
-          if (irlp32_exact(P.pred_n[3 * d], P.pred_n[3 * d + 1], P.pred_n[3 * d + 2], P.pred_K[d])) {
-            // the 32-bit IRLP loop gives the same codes
-            std::vector<uint16_t> c2(e.n_local, 0x5A5Au);
-            const bool ok2 = split_axis_g<true>(le.data(), 0, e, P, d, wv, c2.data());
-            if (ok2 != okd || (okd && !std::equal(c2.begin(), c2.end(), c))) rc = 4;
-          }
+          okd = split_axis_g<false>(sw, 0, e, P, d, wv, c);
+          // the 32-bit IRLP loop and the device-buffer reader give the same codes
+          std::vector<uint16_t> c2(e.n_local, 0x5A5Au);
+          const bool i32 =
+              irlp32_exact(P.pred_n[3 * d], P.pred_n[3 * d + 1], P.pred_n[3 * d + 2], P.pred_K[d]);
+          const bool ok2 = i32 ? split_axis_g<true>(gw, 0, e, P, d, wv, c2.data())
+                               : split_axis_g<false>(gw, 0, e, P, d, wv, c2.data());
+          if (ok2 != okd || (okd && !std::equal(c2.begin(), c2.end(), c))) rc = 4;
           for (uint32_t v = 0; v < e.n_local; ++v) codes3[3 * ((size_t)e.v_off + v) + d] = c[v];
         } else {
           int32_t* c = reinterpret_cast<int32_t*>(blk.data()) + cs * d;
-          okd = split_axis_g<false>(le.data(), 0, e, P, d, wv, c);
+          okd = split_axis_g<false>(sw, 0, e, P, d, wv, c);
           for (uint32_t v = 0; v < e.n_local; ++v) codes3[3 * ((size_t)e.v_off + v) + d] = c[v];
         }
         if (!okd) rc = 3;
