@@ -603,16 +603,23 @@ STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
 // The short form of codeword_s on a window W (>= 33 valid bits): the
 // codeword's length with u = its zigzagged value, or 0 for a codeword longer
 // than 33 bits (decoded by codeword_s instead).
-STRIDE_DEV uint32_t codeword_win(uint64_t W, bool eg, int k, uint64_t& u) {
+// Within 33 bits everything fits 32-bit arithmetic: the body (nb <= 32
+// bits) is the low nb bits of the top len bits of W, and u < 2^32 (a Rice
+// code has (q + 1) << k <= 2^32, an EG code u < 2^(q + 1 + k) <= 2^32).
+STRIDE_DEV uint32_t codeword_win(uint64_t W, bool eg, int k, uint32_t& u) {
   const uint32_t q = W ? (uint32_t)__builtin_clzll(W) : 64u;
   const uint32_t nb = (eg ? q : 0u) + (uint32_t)k;
   const uint32_t len = q + 1 + nb;
-  const uint64_t body = nb ? ((W << (q + 1)) >> (64 - nb)) : 0ull;
-  const uint64_t off = body >> k;
-  const uint64_t low = body & ((1ull << k) - 1);
-  const uint64_t shifted = eg ? (((1ull << q) - 1) + off) : (uint64_t)q;
+  const uint32_t x = (uint32_t)(W >> (64 - (len <= 33 ? len : 33u)));
+  const uint32_t body = nb < 32 ? x & ((1u << nb) - 1u) : x;
+  const uint32_t off = body >> k;
+  const uint32_t low = body & ((1u << k) - 1u);
+  const uint32_t shifted = eg ? ((1u << (q & 31u)) - 1u) + off : q;
   u = (shifted << k) | low;
   return len <= 33 ? len : 0u;
+}
+STRIDE_DEV int64_t unzigzag32(uint32_t u) {
+  return (int64_t)(int32_t)((u >> 1) ^ (0u - (u & 1u)));
 }
 
 // IRLP in 32-bit arithmetic (24-bit multiplies): equal to irlp64 when the
@@ -704,13 +711,13 @@ STRIDE_DEV bool split_axis_g(const Src& words, uint64_t base,
         const bool root = (w & 0xFFFFFF00u) == kRootMark;
         // the code comes from the delta stream (strip roots) or the
         // parallelogram stream; only the chosen one advances
-        uint64_t u;
+        uint32_t u;
         const uint32_t len = codeword_win(root ? sd.win : sp.win, root ? deg : peg, root ? dk : pk, u);
         int64_t r;
         if (len) {
           sd.drop(root ? len : 0u);
           sp.drop(root ? 0u : len);
-          r = unzigzag(u);
+          r = unzigzag32(u);
         } else {
           r = root ? codeword_s(sd, deg, dk) : codeword_s(sp, peg, pk);
         }

@@ -1,0 +1,98 @@
+// Host build of the axis lanes' codeword fast path: codeword_win (32-bit,
+// codewords of at most 33 bits) and irlp24 against the general stream
+// decoder codeword_s and irlp64 (meshpress_amd/csrc/stride_lane.h), over
+// every unary length q, coder parameter k and coder, with random bodies and
+// trailing bits, and over the edge windows (all zeros, all ones).
+//
+//   codeword_check      exit 0 = all equal, 1 = a mismatch (printed)
+#include <cstdio>
+#include <cstdlib>
+#include <vector>
+
+#include "../meshpress_amd/csrc/stride_lane.h"
+
+using namespace stride;
+
+static uint64_t rng = 0x9E3779B97F4A7C15ull;
+static uint64_t next64() {
+  rng ^= rng << 13;
+  rng ^= rng >> 7;
+  rng ^= rng << 17;
+  return rng;
+}
+
+int main() {
+  long checked = 0, fast = 0;
+  for (int eg = 0; eg < 2; ++eg)
+    for (int k = 0; k < 32; ++k)
+      for (uint32_t q = 0; q <= 40; ++q)
+        for (int rep = 0; rep < 64; ++rep) {
+          // window: q zeros, a one, then random bits (or the edge patterns)
+          uint64_t W = next64();
+          if (rep == 0) W = 0;
+          if (rep == 1) W = ~0ull;
+          if (q < 64 && rep > 1) W = (W >> (q + 1)) | (1ull << (63 - q));
+          if (q >= 64) W = 0;
+          // the same bits as a big-endian word stream, then more random words
+          std::vector<uint32_t> le(8);
+          const uint32_t hi = (uint32_t)(W >> 32), lo = (uint32_t)W;
+          le[0] = __builtin_bswap32(hi);
+          le[1] = __builtin_bswap32(lo);
+          for (int i = 2; i < 8; ++i) le[i] = (uint32_t)next64();
+          StreamBits s;
+          s.init(le.data(), 0);
+          uint32_t u = 0;
+          const uint32_t len = codeword_win(s.win, eg != 0, k, u);
+          const uint32_t nb = (eg ? (W ? (uint32_t)__builtin_clzll(W) : 64u) : 0u) + (uint32_t)k;
+          const uint32_t qq = W ? (uint32_t)__builtin_clzll(W) : 64u;
+          if (qq + 1 + nb > 33) {
+            if (len != 0) {
+              std::printf("long codeword not rejected: eg %d k %d W %016llx len %u\n", eg, k,
+                          (unsigned long long)W, len);
+              return 1;
+            }
+            ++checked;
+            continue;
+          }
+          if (W == 0) continue;   // no terminating one inside the window: long path only
+          const int64_t r = codeword_s(s, eg != 0, k);
+          const uint64_t used = 64 - s.valid + 32 * (s.next - 3);   // bits consumed
+          if (len == 0 || unzigzag32(u) != r || used != len) {
+            std::printf("mismatch: eg %d k %d W %016llx fast (%u, %lld) general (%llu, %lld)\n", eg,
+                        k, (unsigned long long)W, len, (long long)unzigzag32(u),
+                        (unsigned long long)used, (long long)r);
+            return 1;
+          }
+          ++checked;
+          ++fast;
+        }
+  // irlp24 against irlp64 wherever irlp32_exact holds
+  long irl = 0;
+  for (int rep = 0; rep < 2000000; ++rep) {
+    const int32_t K = (int32_t)(next64() % 32);
+    int32_t n[3];
+    for (int j = 0; j < 3; ++j) {
+      const uint64_t t = next64();
+      n[j] = (t & 1) ? (int32_t)(int16_t)(t >> 8) : (int32_t)((int64_t)((t >> 8) % 2049) - 1024);
+    }
+    if (!irlp32_exact(n[0], n[1], n[2], K)) continue;
+    const int32_t a = (int32_t)(next64() & 0xFFFF), b = (int32_t)(next64() & 0xFFFF),
+                  c = (int32_t)(next64() & 0xFFFF);
+    const int32_t rnd = K > 0 ? (1 << (K - 1)) : 0;
+    const int64_t want = irlp64(a, b, c, n[0], n[1], n[2], K);
+    const int64_t got = irlp24(a, b, c, n[0], n[1], n[2], K, rnd);
+    if (want != got) {
+      std::printf("irlp mismatch: n %d %d %d K %d codes %d %d %d: %lld vs %lld\n", n[0], n[1],
+                  n[2], K, a, b, c, (long long)want, (long long)got);
+      return 1;
+    }
+    ++irl;
+  }
+  // the bound itself: the largest int16 numerators at K = 0 overflow int32
+  if (irlp32_exact(-32768, -32768, -32768, 0) || !irlp32_exact(512, 512, -512, 9)) {
+    std::printf("irlp32_exact bound wrong\n");
+    return 1;
+  }
+  std::printf("codewords %ld (fast path %ld), irlp %ld\n", checked, fast, irl);
+  return 0;
+}

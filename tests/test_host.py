@@ -211,6 +211,24 @@ def lane_check(tmp_path_factory):
     return exe
 
 
+def test_codeword_fast_path_and_irlp24(tmp_path):
+    """The axis lanes' 32-bit codeword fast path equals the general stream
+    decoder (value and length, every q, k and coder, long codewords
+    rejected), and irlp24 equals irlp64 wherever irlp32_exact allows it."""
+    import shutil
+    import subprocess
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    exe = tmp_path / "codeword_check"
+    subprocess.run([gxx, "-O1", "-g", "-std=c++17", "-fsanitize=address,undefined",
+                    "-fno-sanitize-recover=undefined", str(ROOT / "tests" / "codeword_check.cpp"),
+                    "-o", str(exe)], check=True)
+    r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "fast path" in r.stdout
+
+
 def run_lane_check(exe, data, tmp_path, split=False):
     """split: False (one lane per meshlet), True or "split" (int32 codes),
     "split16" (uint16 codes), "twopass" / "twopass16" (token pass, then the
