@@ -183,9 +183,13 @@ struct Mtf {
   uint64_t lo, hi;
   uint32_t n;
   STRIDE_DEV void clear() { lo = 0; hi = 0; n = 0; }
+  // entry j < 16.  A mask blend, not a select of the two halves: a select of
+  // two loads folds into one load through a computed address, which keeps
+  // a stepper's state out of registers.
   STRIDE_DEV uint32_t rank(uint32_t j) const {
-    const uint64_t w = j < 8 ? lo : hi;
-    return (uint32_t)(w >> (8 * (j & 7))) & 0xFFu;
+    const uint32_t sh = 8 * (j & 7);
+    const uint64_t m = 0ull - (uint64_t)(j >> 3);   // ones for j in [8, 16)
+    return (uint32_t)(((lo >> sh) & ~m) | ((hi >> sh) & m)) & 0xFFu;
   }
   // 128-bit mask of bytes [0, j), j in [0, 17]
   STRIDE_DEV static void mask(uint32_t j, uint64_t& mlo, uint64_t& mhi) {
@@ -227,9 +231,13 @@ struct MtfLite {
   uint64_t lo, hi;
   uint32_t n;
   STRIDE_DEV void clear() { lo = 0; hi = 0; n = 0; }
+  // entry j < 16.  A mask blend, not a select of the two halves: a select of
+  // two loads folds into one load through a computed address, which keeps
+  // a stepper's state out of registers.
   STRIDE_DEV uint32_t rank(uint32_t j) const {
-    const uint64_t w = j < 8 ? lo : hi;
-    return (uint32_t)(w >> (8 * (j & 7))) & 0xFFu;
+    const uint32_t sh = 8 * (j & 7);
+    const uint64_t m = 0ull - (uint64_t)(j >> 3);   // ones for j in [8, 16)
+    return (uint32_t)(((lo >> sh) & ~m) | ((hi >> sh) & m)) & 0xFFu;
   }
   STRIDE_DEV void to_front(uint32_t j, uint32_t v) {
     // bytes [0, j] of the low half: all ones >> (56 - 8 min(j, 7))

@@ -12,13 +12,13 @@ namespace stride {
 struct DecodeConfig {
   int mode = 4;           // 0 = one wave per meshlet, 1 = groups (one lane per meshlet),
                           // 2 = split groups (token lane + three axis lanes per meshlet),
-                          // 4 = two passes: token lanes (64 meshlets per wave, walks to
-                          //     HBM), then the axis lanes of the split groups
+                          // 4 = two passes: token lanes (tok_group meshlets per wave,
+                          //     walks to HBM), then the axis lanes of the split groups
   int waves = 1;          // mode 0: waves per workgroup (1, 2, 4); one meshlet per wave
   int stage_words = 512;  // mode 0: LDS words staged per meshlet (256, 512, 1024)
   int grid = 0;           // 0 = one workgroup per unit of work, else persistent grid
   int use_queue = 0;      // mode 0: persistent grid pulls meshlets from an atomic counter
-  int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; mode 2: 1..16
+  int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; modes 2, 4: 1..16
   int lds_bytes = 16384;  // modes 1, 2, 4: LDS per group (stage words + codes / walk blocks)
   int tok_group = 32;     // mode 4: meshlets per token-pass wave, 1..64
 };

@@ -340,40 +340,46 @@ struct LdsCursor {            // staged words in LDS
 
 // Token lane: strip tokens -> triangles (tout) and the walk.  `cur` starts
 // at the first strip length.
-template <class Cursor, class Seen>
-STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ walk,
-                               Seen& seen, uint32_t* __restrict__ tout);
-
-template <class Seen>
-STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
-                             uint32_t* __restrict__ walk, Seen& seen,
-                             uint32_t* __restrict__ tout) {
-  LdsCursor cur{st, rel + 48};
-  split_tokens_c(cur, e, walk, seen, tout);
-}
-
-template <class Cursor, class Seen>
-STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ walk,
-                               Seen& seen, uint32_t* __restrict__ tout) {
-  const uint32_t n_local = e.n_local, n_tris = e.n_tris;
-  const uint32_t n_strips = e.n_strips & kStripMask;
-  const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
-  uint32_t ib = 1;
-  while ((1u << ib) < n_local + 1) ++ib;
+// The token lane as a stepper: one strip token per step().  A first
+// sighting hands its walk entry (local id + parallelogram refs, first-seen
+// order) to sink(index, entry); the triangles go to `o`.  The split and
+// two-pass kernels run steps back to back (split_tokens_c).
+struct TokenLane {
+  uint32_t n_tok, it;      // tokens in the meshlet, tokens done
+  uint32_t ib, v_off;
+  uint32_t nw;             // walk entries emitted
+  uint32_t q0, q1, q2, root_j, steps_left;
+  uint32_t Wn;             // the next token's 32-bit window, read one token ahead
+  bool raw_in_fifo;
   MtfLite fifo;
-  fifo.clear();
-  uint32_t nw = 0, q0 = 0, q1 = 0, q2 = 0;
-  uint32_t root_j = 3, steps_left = 0;
-  const uint32_t v_off = e.v_off;
-  const uint32_t n_tok = n_tris + 2 * n_strips;
-  uint32_t* o = tout;
-  // strip length (16 bits) + edge bit + flag + id (<= 9 bits): one 32-bit
-  // window per token.  The next token's window is read as soon as this
-  // token's length is known, so its latency hides under the FIFO work; the
-  // read after the last token stays inside the meshlet's anchor and
-  // substream bits or the buffer space after them.
-  uint32_t Wn = cur.peek32();
-  for (uint32_t it = 0; it < n_tok; ++it) {
+  uint32_t* __restrict__ o;
+
+  template <class Cursor>
+  STRIDE_DEV void init(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ tout) {
+    const uint32_t n_local = e.n_local;
+    n_tok = e.n_tris + 2 * (e.n_strips & kStripMask);
+    raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
+    ib = 1;
+    while ((1u << ib) < n_local + 1) ++ib;
+    fifo.clear();
+    it = 0;
+    nw = 0;
+    q0 = q1 = q2 = 0;
+    root_j = 3;
+    steps_left = 0;
+    v_off = e.v_off;
+    o = tout;
+    // strip length (16 bits) + edge bit + flag + id (<= 9 bits): one 32-bit
+    // window per token.  The next token's window is read as soon as this
+    // token's length is known, so its latency hides under the FIFO work; the
+    // read after the last token stays inside the meshlet's anchor and
+    // substream bits or the buffer space after them.
+    Wn = cur.peek32();
+  }
+  STRIDE_DEV bool done() const { return it >= n_tok; }
+
+  template <class Cursor, class Seen, class Sink>
+  STRIDE_DEV void step(Cursor& cur, Seen& seen, Sink& sink) {
     uint32_t W = Wn;
     const bool start = root_j == 3 && steps_left == 0;
     steps_left = start ? (W >> 16) - 1 : steps_left;
@@ -403,7 +409,7 @@ STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __r
     const bool fresh = seen.fresh(v);
     const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
     const uint32_t entry = is_root ? (kRootMark | v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
-    if (fresh) walk[nw & 255u] = entry;
+    if (fresh) sink(nw, entry);
     nw += fresh ? 1u : 0u;
     // triangle (a, b, v) and the next previous triangle
     const bool emit = !is_root || root_j == 2;
@@ -420,7 +426,31 @@ STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __r
     q2 = is_root ? (root_j == 2 ? v : q2) : v;
     root_j += is_root ? 1u : 0u;
     steps_left -= is_root ? 0u : 1u;
+    ++it;
   }
+};
+
+// Walk sink of the split and two-pass kernels: the whole walk in one array.
+struct WalkArray {
+  uint32_t* __restrict__ walk;
+  STRIDE_DEV void operator()(uint32_t i, uint32_t entry) const { walk[i & 255u] = entry; }
+};
+
+template <class Cursor, class Seen>
+STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ walk,
+                               Seen& seen, uint32_t* __restrict__ tout) {
+  TokenLane t;
+  t.init(cur, e, tout);
+  WalkArray sink{walk};
+  for (uint32_t it = 0; it < t.n_tok; ++it) t.step(cur, seen, sink);
+}
+
+template <class Seen>
+STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
+                             uint32_t* __restrict__ walk, Seen& seen,
+                             uint32_t* __restrict__ tout) {
+  LdsCursor cur{st, rel + 48};
+  split_tokens_c(cur, e, walk, seen, tout);
 }
 
 STRIDE_DEV uint32_t pick3(int d, uint32_t x0, uint32_t x1, uint32_t x2) {
@@ -666,79 +696,105 @@ STRIDE_DEV uint32_t split_code_words(uint32_t n_local, bool narrow) {
   return narrow ? 3 * ((n_local + 1) / 2) : 3 * n_local;
 }
 
-// split_axis over a word source indexed by absolute buffer word (the device
-// buffer, or the meshlet's axis words staged in LDS: axis_stage_words),
-// `base` the bit offset of the mesh's blob, `walk` the meshlet's walk slot
-// in HBM (read four entries ahead).
+// One axis lane's chain as a stepper over the walk: init() reads the
+// anchor and the (tag, k) pairs of axis d and opens its delta and
+// parallelogram substreams; first(w0) stores the anchor at the walk's first
+// vertex, step(w) decodes the code of each later walk entry.  `words` is a
+// word source indexed by absolute buffer word (the device buffer, or the
+// meshlet's axis words staged in LDS: axis_stage_words), `base` the bit
+// offset of the mesh's blob.  I32: IRLP in 32-bit arithmetic (the caller
+// checks irlp32_exact; uint16 codes only).  ok turns false when a code
+// leaves the range of CodeT.
+template <bool I32, class Src, class CodeT>
+struct AxisChain {
+  BitStream<Src> sd, sp;
+  bool deg, peg, ok;
+  int dk, pk;
+  int32_t n0, n1, n2, K, rnd;
+  int64_t anchor, prev;
+
+  STRIDE_DEV void init(const Src& words, uint64_t base, const MeshletEntry& e, const MeshParams& P,
+                       int d) {
+    const uint32_t gb0 = (uint32_t)P.g_bits[0], gb1 = (uint32_t)P.g_bits[1],
+                   gb2 = (uint32_t)P.g_bits[2];
+    const uint32_t gb = pick3(d, gb0, gb1, gb2);
+    const uint64_t r0 = base + e.resid_bit;
+    anchor = (int64_t)bits_at_src(words, r0 + (d > 0 ? gb0 : 0u) + (d > 1 ? gb1 : 0u), gb);
+    // (tag, k) of delta-d at tag byte pair d, of para-d at pair 3 + d
+    const uint64_t tb = r0 + gb0 + gb1 + gb2;
+    const uint32_t TD = bits_at_src(words, tb + 16u * (uint32_t)d, 16);
+    const uint32_t TP = bits_at_src(words, tb + 16u * (uint32_t)(3 + d), 16);
+    deg = (TD >> 8) != 1u;
+    peg = (TP >> 8) != 1u;
+    dk = (int)(TD & 31u);
+    pk = (int)(TP & 31u);
+    sd.init(words, base + pick3(d, e.sub_bit[0], e.sub_bit[1], e.sub_bit[2]));
+    sp.init(words, base + pick3(d, e.sub_bit[3], e.sub_bit[4], e.sub_bit[5]));
+    n0 = (int32_t)pick3(d, (uint32_t)P.pred_n[0], (uint32_t)P.pred_n[3], (uint32_t)P.pred_n[6]);
+    n1 = (int32_t)pick3(d, (uint32_t)P.pred_n[1], (uint32_t)P.pred_n[4], (uint32_t)P.pred_n[7]);
+    n2 = (int32_t)pick3(d, (uint32_t)P.pred_n[2], (uint32_t)P.pred_n[5], (uint32_t)P.pred_n[8]);
+    K = (int32_t)pick3(d, (uint32_t)P.pred_K[0], (uint32_t)P.pred_K[1], (uint32_t)P.pred_K[2]);
+    rnd = (I32 && K > 0) ? (1 << (K - 1)) : 0;
+    ok = true;
+    prev = anchor;
+  }
+  STRIDE_DEV void first(uint32_t w0, CodeT* __restrict__ codes) {
+    codes[w0 & 0xFFu] = (CodeT)anchor;
+    ok &= code_fits<CodeT>(anchor);
+  }
+  STRIDE_DEV void step(uint32_t w, CodeT* __restrict__ codes) {
+    const bool root = (w & 0xFFFFFF00u) == kRootMark;
+    // the code comes from the delta stream (strip roots) or the
+    // parallelogram stream; only the chosen one advances
+    uint32_t u;
+    const uint32_t len = codeword_win(root ? sd.win : sp.win, root ? deg : peg, root ? dk : pk, u);
+    int64_t r;
+    if (len) {
+      sd.drop(root ? len : 0u);
+      sp.drop(root ? 0u : len);
+      r = unzigzag32(u);
+    } else {
+      r = root ? codeword_s(sd, deg, dk) : codeword_s(sp, peg, pk);
+    }
+    const uint32_t ia = root ? 0u : (w >> 8) & 0xFFu, ib = root ? 0u : (w >> 16) & 0xFFu,
+                   ic = root ? 0u : w >> 24;
+    int64_t pred;
+    if constexpr (I32 && sizeof(CodeT) == 2) {
+      pred = (int64_t)irlp24((int32_t)codes[ia], (int32_t)codes[ib], (int32_t)codes[ic], n0, n1, n2,
+                             K, rnd);
+    } else {
+      pred = irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1, n2, K);
+    }
+    const int64_t c = (root ? prev : pred) + r;
+    ok &= code_fits<CodeT>(c);
+    codes[w & 0xFFu] = (CodeT)c;
+    prev = c;
+  }
+};
+
+// The two-pass axis lane: the chain along the meshlet's walk slot in HBM
+// (read four entries ahead).  Returns false when a code leaves CodeT.
 template <bool I32, class Src, class CodeT>
 STRIDE_DEV bool split_axis_g(const Src& words, uint64_t base,
                              const MeshletEntry& e, const MeshParams& P, int d,
                              const Walk4* __restrict__ walk, CodeT* __restrict__ codes) {
   const uint32_t n_local = e.n_local;
-  const uint32_t gb0 = (uint32_t)P.g_bits[0], gb1 = (uint32_t)P.g_bits[1],
-                 gb2 = (uint32_t)P.g_bits[2];
-  const uint32_t gb = pick3(d, gb0, gb1, gb2);
-  const uint64_t r0 = base + e.resid_bit;
-  const int64_t anchor = (int64_t)bits_at_src(words, r0 + (d > 0 ? gb0 : 0u) + (d > 1 ? gb1 : 0u), gb);
-  // (tag, k) of delta-d at tag byte pair d, of para-d at pair 3 + d
-  const uint64_t tb = r0 + gb0 + gb1 + gb2;
-  const uint32_t TD = bits_at_src(words, tb + 16u * (uint32_t)d, 16);
-  const uint32_t TP = bits_at_src(words, tb + 16u * (uint32_t)(3 + d), 16);
-  const bool deg = (TD >> 8) != 1u, peg = (TP >> 8) != 1u;
-  const int dk = (int)(TD & 31u), pk = (int)(TP & 31u);
-  BitStream<Src> sd, sp;
-  sd.init(words, base + pick3(d, e.sub_bit[0], e.sub_bit[1], e.sub_bit[2]));
-  sp.init(words, base + pick3(d, e.sub_bit[3], e.sub_bit[4], e.sub_bit[5]));
-  const int32_t n0 = (int32_t)pick3(d, (uint32_t)P.pred_n[0], (uint32_t)P.pred_n[3], (uint32_t)P.pred_n[6]);
-  const int32_t n1 = (int32_t)pick3(d, (uint32_t)P.pred_n[1], (uint32_t)P.pred_n[4], (uint32_t)P.pred_n[7]);
-  const int32_t n2 = (int32_t)pick3(d, (uint32_t)P.pred_n[2], (uint32_t)P.pred_n[5], (uint32_t)P.pred_n[8]);
-  const int32_t K = (int32_t)pick3(d, (uint32_t)P.pred_K[0], (uint32_t)P.pred_K[1], (uint32_t)P.pred_K[2]);
-  const int32_t rnd = (I32 && K > 0) ? (1 << (K - 1)) : 0;
-  bool ok = true;
-  int64_t prev = anchor;
+  AxisChain<I32, Src, CodeT> ax;
+  ax.init(words, base, e, P, d);
   // the walk slot is 16-byte aligned, with readable words past its end
   Walk4 wq = walk[0];
-  codes[wq.x & 0xFFu] = (CodeT)anchor;
-  ok &= code_fits<CodeT>(anchor);
+  ax.first(wq.x, codes);
   for (uint32_t i0 = 0; i0 < n_local; i0 += 4) {
     const Walk4 nq = walk[(i0 >> 2) + 1];
     const uint32_t wb[4] = {wq.x, wq.y, wq.z, wq.w};
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t i = i0 + (uint32_t)j;
-      if (i >= 1 && i < n_local) {
-        const uint32_t w = wb[j];
-        const bool root = (w & 0xFFFFFF00u) == kRootMark;
-        // the code comes from the delta stream (strip roots) or the
-        // parallelogram stream; only the chosen one advances
-        uint32_t u;
-        const uint32_t len = codeword_win(root ? sd.win : sp.win, root ? deg : peg, root ? dk : pk, u);
-        int64_t r;
-        if (len) {
-          sd.drop(root ? len : 0u);
-          sp.drop(root ? 0u : len);
-          r = unzigzag32(u);
-        } else {
-          r = root ? codeword_s(sd, deg, dk) : codeword_s(sp, peg, pk);
-        }
-        const uint32_t ia = root ? 0u : (w >> 8) & 0xFFu, ib = root ? 0u : (w >> 16) & 0xFFu,
-                       ic = root ? 0u : w >> 24;
-        int64_t pred;
-        if constexpr (I32 && sizeof(CodeT) == 2) {
-          pred = (int64_t)irlp24((int32_t)codes[ia], (int32_t)codes[ib], (int32_t)codes[ic], n0, n1,
-                                 n2, K, rnd);
-        } else {
-          pred = irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1, n2, K);
-        }
-        const int64_t c = (root ? prev : pred) + r;
-        ok &= code_fits<CodeT>(c);
-        codes[w & 0xFFu] = (CodeT)c;
-        prev = c;
-      }
+      if (i >= 1 && i < n_local) ax.step(wb[j], codes);
     }
     wq = nq;
   }
-  return ok;
+  return ax.ok;
 }
 
 // Index pre-pass for one meshlet (one lane): parses the whole meshlet once,

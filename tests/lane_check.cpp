@@ -3,7 +3,7 @@
 // the lane decoder (decode_lane) run here on the CPU, under the sanitizers
 // when built with them, and their output is compared with the oracle.
 //
-//   lane_check <stream.dpr4> <verts.f32 out> <tris.u32 out> [split]
+//   lane_check <stream.dpr4> <verts.f32 out> <tris.u32 out> [split|split16|twopass|twopass16]
 //
 // With "split" the split decoder runs instead: every meshlet's token lane
 // (split_tokens), then its three axis lanes (split_axis), as the kernel's
@@ -11,7 +11,8 @@
 // lanes first with register seen sets, then the axis lanes on blocks that
 // hold only the copied walk.
 // Exit status: 0 ok, 2 stream rejected by the pre-pass, 3 code overflow
-// (the kernels route such groups to the wave-per-meshlet decoder).
+// (the kernels route such groups to the wave-per-meshlet decoder), 4 the
+// 32-bit and 64-bit axis loops disagree.
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>

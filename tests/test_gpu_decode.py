@@ -52,19 +52,27 @@ def stream_fixtures():
     return out
 
 
+# the default launch shape (two passes) and the split kernel
+SHAPES = [dict(), dict(mode=2, group_size=16, lds_bytes=24576)]
+
+
+@pytest.mark.parametrize("shape", SHAPES, ids=["default", "split"])
 @pytest.mark.parametrize("path", stream_fixtures(), ids=lambda p: p.stem)
-def test_golden_streams(path):
+def test_golden_streams(path, shape):
+    if shape and path.stem.startswith("pdlt_"):
+        pytest.skip("PDLT v5 streams have their own launch shapes (tests/test_gpu_pdlt.py)")
     z = np.load(path)
-    V, T = gpu_decode(z["data"].tobytes())
+    V, T = gpu_decode(z["data"].tobytes(), **shape)
     assert sha(V) == z["V_sha"].tobytes().hex()
     assert sha(T.astype(np.int64)) == z["T_sha"].tobytes().hex()
 
 
+@pytest.mark.parametrize("shape", SHAPES, ids=["default", "split"])
 @pytest.mark.parametrize("pred", ["canonical", "fit"])
-def test_small_meshes(oracle, pred):
+def test_small_meshes(oracle, pred, shape):
     for name, V, T in small_meshes():
         _, data = encode_with_oracle(V, T, pred=pred)
-        Vg, Tg = gpu_decode(data)
+        Vg, Tg = gpu_decode(data, **shape)
         Vo, To = oracle.decode_dup(data)
         assert_same(Vg, Tg, Vo, To)
 
@@ -237,8 +245,8 @@ def test_icosphere_l8_full_size(coracle, bits):
 @pytest.mark.parametrize("bits", [17, 20, 24])
 def test_wide_codes(coracle, bits):
     """Meshes quantised to more than 16 bits per axis take the int32 code
-    path of the split decoder (and the 21-bit packed path of mode 1 where
-    the codes fit); both equal the oracle."""
+    path of the split and two-pass decoders (and the 21-bit packed
+    path of mode 1 where the codes fit); all equal the oracle."""
     from meshpress_amd import workloads
     from meshpress_amd.encoder import STRIDEEncoder
     from meshpress_amd.utils.types import Model
@@ -248,6 +256,7 @@ def test_wide_codes(coracle, bits):
     assert data[50] >= 17
     Vo, To = coracle.decode(data)
     for tune in (dict(mode=2, group_size=16, lds_bytes=40960), dict(mode=2, group_size=3, lds_bytes=8192),
+                 dict(mode=4, group_size=16, lds_bytes=24576), dict(mode=4, group_size=3, lds_bytes=8192),
                  dict(mode=1, group_size=64, lds_bytes=49152), dict(mode=0, waves=1, stage_words=512)):
         Vg, Tg = gpu_decode(data, **tune)
         assert_same(Vg, Tg, Vo, To)
