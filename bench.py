@@ -3,8 +3,8 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
                     [--batch N] [--quick] [--no-cpu]
 
-A step is one decode of the whole workload (one kernel launch for a single
-mesh).  The headline workload is the 1900 x 1900 heightfield (7,212,402
+A step is one decode of the whole workload (one launch of the decode: the
+token pass and the axis pass for a single mesh by default).  The headline workload is the 1900 x 1900 heightfield (7,212,402
 triangles) with 12-bit bbox quantisation, encoded by this package's
 STRIDEEncoder; inputs are resident in HBM before timing starts.  The same
 run also decodes
@@ -22,9 +22,11 @@ run also decodes
   predictor), decoded by the same class with the float64 chain of
   decode_paradelta_v5 and with the encoder's float32 chain;
 
-Prints one JSON line with the headline throughput in Gtris/s, the dominant
-kernel's HBM roofline figures, a CPU baseline (the C oracle, one thread,
-timed on this host) and the other workloads under "details".
+Prints one JSON line with the headline throughput in Gtris/s, the decode's
+HBM roofline figures (with the VALU issue bound of the committed counter
+profile: roofline.issue_model), a CPU baseline (the C oracle on one thread,
+and under cpu_baseline.all_threads on the host's thread share, both timed on
+this host) and the other workloads under "details".
 """
 from __future__ import annotations
 
@@ -100,15 +102,16 @@ def autotune(dec, mode_kw=None):
                    c["use_queue"], round(ms, 4)] for c, ms in res]
 
 
-def cpu_baseline(data, n_t, gpu_out, budget_s=10.0):
-    """The C oracle decode on one host thread over the full stream; its
-    output also checks the GPU decode that was timed."""
+def cpu_baseline(data, n_t, gpu_out, budget_s=10.0, n_threads=1):
+    """The C oracle decode on n_threads host threads (meshlets split across
+    them) over the full stream; its output also checks the GPU decode that
+    was timed."""
     from tests.oracle_c import load
     co = load()
     reps, t_total, out = 0, 0.0, None
     while t_total < budget_s and reps < 20:
         t0 = time.perf_counter()
-        out = co.decode(data, n_threads=1)
+        out = co.decode(data, n_threads=n_threads)
         t_total += time.perf_counter() - t0
         reps += 1
     V, T = gpu_out
@@ -118,15 +121,47 @@ def cpu_baseline(data, n_t, gpu_out, budget_s=10.0):
     return n_t / (t_total / reps) / 1e9, reps, t_total, same
 
 
-def read_traffic(name):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+def host_threads():
+    """The host threads a run may use: OMP_NUM_THREADS when set (the GPU
+    machine's share of cores), else the visible cores."""
+    try:
+        return max(1, int(os.environ.get("OMP_NUM_THREADS", "")))
+    except ValueError:
+        return max(1, os.cpu_count() or 1)
+
+
+def read_profile(name):
+    """The committed rocprofv3 --pmc summary of the workload, or {}."""
     p = ROOT / "profiles" / f"{name}_traffic.json"
     if p.exists():
         try:
-            return json.loads(p.read_text()).get("hbm_bytes_per_launch")
+            return json.loads(p.read_text())
         except Exception:
-            return None
-    return None
+            return {}
+    return {}
+
+
+def read_traffic(name):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary."""
+    return read_profile(name).get("hbm_bytes_per_launch")
+
+
+MI355X_SIMDS = 256 * 4          # 256 CUs x 4 SIMD units
+MI355X_CLOCK_GHZ = 2.4          # peak engine clock
+
+
+def issue_model(name, kernel_ms):
+    """The decode's VALU issue bound from the committed counter profile: every
+    wave64 VALU instruction takes a 16-lane SIMD 4 cycles, so
+    sum(VALU) * 4 / (SIMDs * clock) is the time at full issue; frac = that
+    time / the measured kernel time."""
+    valu = read_profile(name).get("valu_insts_per_launch")
+    if not valu:
+        return None
+    n = sum(valu.values())
+    bound_ms = n * 4 / MI355X_SIMDS / (MI355X_CLOCK_GHZ * 1e9) * 1e3
+    return {"valu_insts": n, "simds": MI355X_SIMDS, "clock_ghz": MI355X_CLOCK_GHZ,
+            "bound_ms": bound_ms, "frac": bound_ms / kernel_ms}
 
 
 def single_mesh(name, V, T, bits, steps, warmup, tune=True):
@@ -387,6 +422,15 @@ def main():
         log(f"cpu baseline: {v:.4f} Gtris/s, GPU output {'matches' if same else 'DIFFERS'}")
         if not same:
             sys.exit("GPU decode differs from the oracle on the bench workload")
+        nt = host_threads()
+        vm, reps_m, tt_m, same_m = cpu_baseline(data, head["n_tris"], gpu_out, budget_s=5.0,
+                                                n_threads=nt)
+        cpu["all_threads"] = {"value": vm, "unit": "Gtris/s", "cores": nt,
+                              "sample": f"same stream x{reps_m} ({tt_m:.1f} s), meshlets split "
+                                        f"over {nt} threads", "gpu_output_matches": same_m}
+        log(f"cpu baseline, {nt} threads: {vm:.4f} Gtris/s")
+        if not same_m:
+            sys.exit("GPU decode differs from the threaded oracle on the bench workload")
     del dec
     if not a.quick:
         details[f"{name}_pdlt_v5"] = pdlt_v5(head_prep, name, a.steps, a.warmup)
@@ -416,7 +460,8 @@ def main():
                                         "stream_bytes", "side_bytes", "bpv", "tune")},
         "roofline": {"bound": "hbm", "achieved": head["hbm_gb_s"], "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": head["hbm_frac"], "traffic": read_traffic(name),
-                     "algorithmic_bytes": head["algorithmic_bytes"], "kernel_ms": head["ms_kernel"]},
+                     "algorithmic_bytes": head["algorithmic_bytes"], "kernel_ms": head["ms_kernel"],
+                     "issue_model": issue_model(name, head["ms_kernel"])},
         "cpu_baseline": cpu,
         "details": details,
     }

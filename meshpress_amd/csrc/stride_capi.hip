@@ -481,8 +481,10 @@ struct DecodeSet {
       if (open) tg.push_back(cur);
     }
     tg_base[n_meshes] = (uint32_t)tg.size();
-    // staged words + an 8-word seen set per token lane
-    for (const GroupEntry& G : tg) tl = std::max(tl, 4 * (G.n_words + 8 * (G.m1 - G.m0)));
+    // staged words + the token lanes' seen sets, 8 words each at stride
+    // kTokenSeenStride
+    for (const GroupEntry& G : tg)
+      tl = std::max(tl, 4 * (G.n_words + 7 * kTokenSeenStride + (G.m1 - G.m0)));
     (void)hipFree(d_tgroups);
     d_tgroups = nullptr;
     n_tgroups = (uint32_t)tg.size();

@@ -197,6 +197,59 @@ __device__ __forceinline__ float dequant(int32_t code, float gmin, float grange,
   return (float)__dadd_rn(__dmul_rn(vn, (double)scale), (double)center);
 }
 
+// dequant for codes below 2^16 and g_bits <= 16, with r = RN(1 / mx) per
+// axis: the quotient by quot_mx (equal to the division in that range), the
+// rest as dequant.
+__device__ __forceinline__ float dequant16(uint32_t code, float gmin, float grange, double mx,
+                                           double r, float scale, float center) {
+  const double vn = mx == 0.0 ? (double)gmin
+                              : __dadd_rn((double)gmin, __dmul_rn(quot_mx((double)code, mx, r),
+                                                                  (double)grange));
+  return (float)__dadd_rn(__dmul_rn(vn, (double)scale), (double)center);
+}
+
+// Dequantisation of the split kernels' code arrays: the wave writes meshlet
+// e's vertices (x, y, z) one per lane.  With uint16 codes, one reciprocal
+// per axis and mesh (kept across a group's meshlets) replaces the division
+// per coordinate.
+struct SplitDequant {
+  uint32_t mesh = ~0u;
+  double mx[3] = {0.0, 0.0, 0.0}, r[3] = {0.0, 0.0, 0.0};
+
+  template <bool NARROW, class CodeT>
+  __device__ __forceinline__ void meshlet(const CodeT* __restrict__ cb, const MeshletEntry& e,
+                                          const MeshParams& P, float* __restrict__ V, int lane) {
+    const uint32_t n = e.n_local, cs = split_code_stride(n, NARROW);
+    float* vo = V + 3 * (P.v_base + e.v_off);
+    if (NARROW && e.mesh != mesh) {
+      mesh = e.mesh;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        mx[a] = (double)((1u << P.g_bits[a]) - 1u);
+        r[a] = mx[a] == 0.0 ? 0.0 : __ddiv_rn(1.0, mx[a]);
+      }
+    }
+    for (uint32_t v = (uint32_t)lane; v < n; v += 64) {
+      float x, y, z;
+      if constexpr (NARROW) {
+        x = dequant16(cb[v], P.g_min[0], P.g_range[0], mx[0], r[0], P.scale, P.center[0]);
+        y = dequant16(cb[cs + v], P.g_min[1], P.g_range[1], mx[1], r[1], P.scale, P.center[1]);
+        z = dequant16(cb[2 * cs + v], P.g_min[2], P.g_range[2], mx[2], r[2], P.scale,
+                      P.center[2]);
+      } else {
+        x = dequant((int32_t)cb[v], P.g_min[0], P.g_range[0], P.g_bits[0], P.scale, P.center[0]);
+        y = dequant((int32_t)cb[cs + v], P.g_min[1], P.g_range[1], P.g_bits[1], P.scale,
+                    P.center[1]);
+        z = dequant((int32_t)cb[2 * cs + v], P.g_min[2], P.g_range[2], P.g_bits[2], P.scale,
+                    P.center[2]);
+      }
+      vo[3 * v] = x;
+      vo[3 * v + 1] = y;
+      vo[3 * v + 2] = z;
+    }
+  }
+};
+
 template <int WAVES, int SW>
 __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -423,24 +476,12 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
     }
     __syncthreads();
     // dequantise, one meshlet at a time, one vertex (x, y, z) per lane
+    SplitDequant dq;
     for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
       const MeshletEntry ej = tab[G.m0 + j];
-      const MeshParams& Pj = meshes[ej.mesh];
       const uint32_t pj = __shfl(pre0, (int)j, 64);
-      const uint32_t n = ej.n_local, cs = split_code_stride(n, NARROW);
-      const CodeT* cb = reinterpret_cast<const CodeT*>(lds32 + G.codes_off + pj + n);
-      float* vo = V + 3 * (Pj.v_base + ej.v_off);
-      for (uint32_t v = lane; v < n; v += 64) {
-        const float x = dequant((int32_t)cb[v], Pj.g_min[0], Pj.g_range[0], Pj.g_bits[0], Pj.scale,
-                                Pj.center[0]);
-        const float y = dequant((int32_t)cb[cs + v], Pj.g_min[1], Pj.g_range[1], Pj.g_bits[1],
-                                Pj.scale, Pj.center[1]);
-        const float z = dequant((int32_t)cb[2 * cs + v], Pj.g_min[2], Pj.g_range[2], Pj.g_bits[2],
-                                Pj.scale, Pj.center[2]);
-        vo[3 * v] = x;
-        vo[3 * v + 1] = y;
-        vo[3 * v + 2] = z;
-      }
+      dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + G.codes_off + pj + ej.n_local), ej,
+                         meshes[ej.mesh], V, lane);
     }
     __syncthreads();
   }
@@ -475,7 +516,8 @@ __global__ __launch_bounds__(64) void stride_tokens_kernel(
     const MeshParams& P = meshes[e.mesh];
     const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
     const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
-    BitSeen seen{lds32 + G.n_words + lane, G.m1 - G.m0};
+    // a constant stride: the word index scales by a shift, not a multiply
+    BitSeen seen{lds32 + G.n_words + lane, kTokenSeenStride};
     seen.preset(0);
     split_tokens(lds32, rel, e, walk_g + walk_slot(P, e, m), seen, T + 3 * (P.t_base + e.t_off));
   }
@@ -549,24 +591,11 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
     if (!ok && overflow) overflow[g] = 1u;
   }
   __syncthreads();
+  SplitDequant dq;
   for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
     const MeshletEntry ej = tab[G.m0 + j];
-    const MeshParams& Pj = meshes[ej.mesh];
     const uint32_t pj = __shfl(pre0, (int)j, 64);
-    const uint32_t n = ej.n_local, cs = split_code_stride(n, NARROW);
-    const CodeT* cb = reinterpret_cast<const CodeT*>(lds32 + pj);
-    float* vo = V + 3 * (Pj.v_base + ej.v_off);
-    for (uint32_t v = lane; v < n; v += 64) {
-      const float x = dequant((int32_t)cb[v], Pj.g_min[0], Pj.g_range[0], Pj.g_bits[0], Pj.scale,
-                              Pj.center[0]);
-      const float y = dequant((int32_t)cb[cs + v], Pj.g_min[1], Pj.g_range[1], Pj.g_bits[1],
-                              Pj.scale, Pj.center[1]);
-      const float z = dequant((int32_t)cb[2 * cs + v], Pj.g_min[2], Pj.g_range[2], Pj.g_bits[2],
-                              Pj.scale, Pj.center[2]);
-      vo[3 * v] = x;
-      vo[3 * v + 1] = y;
-      vo[3 * v + 2] = z;
-    }
+    dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + pj), ej, meshes[ej.mesh], V, lane);
   }
 }
 

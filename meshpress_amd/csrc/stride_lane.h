@@ -2,6 +2,7 @@
 // harness that checks it against the oracle (compiled as host C++ there).
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 
 #include "stride_bits.h"
@@ -280,35 +281,11 @@ struct ScratchSeen {
     return f;
   }
 };
-// In registers: 256 bits as eight words, read and updated with selects and
-// masks only (no indexing, no branches).
-struct RegSeen {
-  uint32_t w[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  // ids [0, n) start as seen
-  STRIDE_DEV void preset(uint32_t n) {
-#pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const uint32_t lo = 32u * (uint32_t)q;
-      w[q] = n >= lo + 32 ? ~0u : (n <= lo ? 0u : ((1u << (n - lo)) - 1u));
-    }
-  }
-  STRIDE_DEV bool fresh(uint32_t v) {
-    const uint32_t q = (v >> 5) & 7u;
-    const uint32_t bit = 1u << (v & 31u);
-    uint32_t cur = 0;
-#pragma unroll
-    for (int i = 0; i < 8; ++i) {
-      const uint32_t m = (q == (uint32_t)i) ? bit : 0u;
-      cur |= w[i] & m;
-      w[i] |= m;
-    }
-    return cur == 0;
-  }
-};
-
 // In memory owned by one lane (LDS in the kernels): 8 words, updated with an
 // atomic or whose returned old word tells whether v was seen -- one LDS
 // instruction per token instead of a select over registers.
+// Lanes of a token wave interleave their seen sets at this stride (words).
+constexpr uint32_t kTokenSeenStride = 64;
 struct BitSeen {
   uint32_t* __restrict__ w;
   // words between consecutive 32-id blocks: a wave's lanes interleave their
@@ -677,6 +654,17 @@ STRIDE_DEV uint64_t axis_stage_lo(uint64_t blob_bit, const MeshletEntry& e) {
 }
 STRIDE_DEV uint32_t axis_stage_words(uint64_t blob_bit, const MeshletEntry& e) {
   return (uint32_t)(((blob_bit + e.end_bit) >> 5) + 5 - axis_stage_lo(blob_bit, e));
+}
+
+// code / mx rounded to nearest in float64, from r = RN(1 / mx): q0 = RN(code
+// r) lies within an ulp of the quotient, the remainder code - q0 mx is exact
+// (an fma), and one correction step rounds it correctly.  Checked against
+// the division for every mx = 2^b - 1 with b <= 16 and every code < 2^16
+// (tests/codeword_check.cpp), the range of the uint16 code kernels.
+STRIDE_DEV double quot_mx(double code, double mx, double r) {
+  const double q0 = code * r;
+  const double e = std::fma(-q0, mx, code);
+  return std::fma(e, r, q0);
 }
 
 // Walk slot of global meshlet m in the two-pass walk buffer: its first

@@ -2,7 +2,9 @@
 // codewords of at most 33 bits) and irlp24 against the general stream
 // decoder codeword_s and irlp64 (meshpress_amd/csrc/stride_lane.h), over
 // every unary length q, coder parameter k and coder, with random bodies and
-// trailing bits, and over the edge windows (all zeros, all ones).
+// trailing bits, and over the edge windows (all zeros, all ones); and the
+// dequantisation quotient quot_mx against the float64 division for every
+// g_bits <= 16 and every 16-bit code.
 //
 //   codeword_check      exit 0 = all equal, 1 = a mismatch (printed)
 #include <cstdio>
@@ -93,6 +95,19 @@ int main() {
     std::printf("irlp32_exact bound wrong\n");
     return 1;
   }
-  std::printf("codewords %ld (fast path %ld), irlp %ld\n", checked, fast, irl);
+  // quot_mx against the division: every g_bits <= 16, every code < 2^16
+  long quot = 0;
+  for (int b = 1; b <= 16; ++b) {
+    const double mx = (double)((1u << b) - 1u), r = 1.0 / mx;
+    for (uint32_t c = 0; c < 65536u; ++c) {
+      const double want = (double)c / mx, got = quot_mx((double)c, mx, r);
+      if (want != got) {
+        std::printf("quot_mx mismatch: b %d code %u: %.17g vs %.17g\n", b, c, got, want);
+        return 1;
+      }
+      ++quot;
+    }
+  }
+  std::printf("codewords %ld (fast path %ld), irlp %ld, quotients %ld\n", checked, fast, irl, quot);
   return 0;
 }

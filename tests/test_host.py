@@ -214,7 +214,8 @@ def lane_check(tmp_path_factory):
 def test_codeword_fast_path_and_irlp24(tmp_path):
     """The axis lanes' 32-bit codeword fast path equals the general stream
     decoder (value and length, every q, k and coder, long codewords
-    rejected), and irlp24 equals irlp64 wherever irlp32_exact allows it."""
+    rejected), irlp24 equals irlp64 wherever irlp32_exact allows it, and
+    quot_mx equals the float64 division for every 16-bit code and g_bits."""
     import shutil
     import subprocess
     gxx = shutil.which("g++")
@@ -226,7 +227,7 @@ def test_codeword_fast_path_and_irlp24(tmp_path):
                     "-o", str(exe)], check=True)
     r = subprocess.run([str(exe)], capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
-    assert "fast path" in r.stdout
+    assert "fast path" in r.stdout and "quotients 1048576" in r.stdout
 
 
 def run_lane_check(exe, data, tmp_path, split=False):
