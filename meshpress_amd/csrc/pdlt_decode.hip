@@ -101,7 +101,7 @@ __global__ __launch_bounds__(64) void pdlt_decode_kernel(
   const int lane = threadIdx.x;
   const int slot = lane & 15, role = lane >> 4;
   const PdltGroup G = groups[blockIdx.x];
-  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
+  stage_words(lds, words, G.w_lo, G.n_words, (uint32_t)lane, 64);
   const uint32_t m = G.m0 + (uint32_t)slot;
   const bool active = m < G.m1;
   PdltEntry e = {};
@@ -129,7 +129,7 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
   uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
   const int lane = threadIdx.x;
   const PdltGroup G = groups[blockIdx.x];
-  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
+  stage_words(lds, words, G.w_lo, G.n_words, (uint32_t)lane, 64);
   __syncthreads();
   const uint32_t m = G.m0 + (uint32_t)lane;
   if (m < G.m1) {
@@ -158,7 +158,7 @@ __global__ __launch_bounds__(64) void pdlt_chain_kernel(
   const int lane = threadIdx.x;
   const int slot = lane & 15, role = lane >> 4;
   const PdltGroup G = groups[blockIdx.x];
-  for (uint32_t i = lane; i < G.n_words; i += 64) lds[i] = __builtin_bswap32(words[G.w_lo + i]);
+  stage_words(lds, words, G.w_lo, G.n_words, (uint32_t)lane, 64);
   const uint32_t m = G.m0 + (uint32_t)slot;
   const bool active = m < G.m1;
   PdltEntry e = {};

@@ -275,7 +275,7 @@ __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
     const uint32_t nw = ((e.end_bit + 31u) >> 5) + 2u - w0;
     const bool staged = nw <= (uint32_t)SW;
     if (staged) {
-      for (uint32_t i = lane; i < nw; i += 64) L.stage[i] = __builtin_bswap32(blob[w0 + i]);
+      stage_words(L.stage, blob, w0, nw, lane, 64);
     }
     wave_sync();
     const uint32_t base = w0 << 5;
@@ -373,7 +373,7 @@ __global__ __launch_bounds__(64) void stride_decode_group_kernel(
   const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
   for (uint32_t g = blockIdx.x; g < n_groups; g += gridDim.x) {
     const GroupEntry G = groups[g];
-    for (uint32_t i = lane; i < G.n_words; i += 64) stage[i] = __builtin_bswap32(words[G.w_lo + i]);
+    stage_words(stage, words, G.w_lo, G.n_words, lane, 64);
     uint64_t* codes = lds64 + G.codes_off;
     const uint32_t m = G.m0 + lane;
     const bool active = m < G.m1;
@@ -443,7 +443,7 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
   };
   for (uint32_t g = queue ? next(0) : blockIdx.x; g < n_groups; g = next(g)) {
     const GroupEntry G = groups[g];
-    for (uint32_t i = lane; i < G.n_words; i += 64) lds32[i] = __builtin_bswap32(words[G.w_lo + i]);
+    stage_words(lds32, words, G.w_lo, G.n_words, lane, 64);
     const uint32_t m = G.m0 + (uint32_t)slot;
     const bool active = m < G.m1;
     MeshletEntry e;
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(64) void stride_tokens_kernel(
   const int lane = threadIdx.x;
   const GroupEntry G = groups[blockIdx.x];
   const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
-  for (uint32_t i = lane; i < G.n_words; i += 64) lds32[i] = __builtin_bswap32(words[G.w_lo + i]);
+  stage_words(lds32, words, G.w_lo, G.n_words, lane, 64);
   __syncthreads();
   const uint32_t m = G.m0 + (uint32_t)lane;
   if (m < G.m1) {

@@ -667,6 +667,32 @@ STRIDE_DEV double quot_mx(double code, double mx, double r) {
   return std::fma(e, r, q0);
 }
 
+#if defined(__HIPCC__)
+// Copies words [lo, lo + n) of the little-endian buffer into dst (LDS),
+// byte-swapped to MSB-first, by the block's nthr threads with kStageDepth
+// loads in flight per thread before the stores (a load-then-store loop waits
+// a full memory round trip per row of words).  Threads past the end re-read
+// word lo, in bounds, and store nothing.
+constexpr uint32_t kStageDepth = 8;
+__device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst,
+                                            const uint32_t* __restrict__ words, uint64_t lo,
+                                            uint32_t n, uint32_t tid, uint32_t nthr) {
+  for (uint32_t i0 = 0; i0 < n; i0 += kStageDepth * nthr) {
+    uint32_t t[kStageDepth];
+#pragma unroll
+    for (uint32_t u = 0; u < kStageDepth; ++u) {
+      const uint32_t i = i0 + u * nthr + tid;
+      t[u] = words[lo + (i < n ? i : 0u)];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kStageDepth; ++u) {
+      const uint32_t i = i0 + u * nthr + tid;
+      if (i < n) dst[i] = __builtin_bswap32(t[u]);
+    }
+  }
+}
+#endif
+
 // Walk slot of global meshlet m in the two-pass walk buffer: its first
 // vertex rounded up to 4 words, + 4 words per preceding meshlet, so every
 // slot is 16-byte aligned and a 4-entry read past a slot's end stays inside
