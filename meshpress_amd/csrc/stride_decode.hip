@@ -216,13 +216,15 @@ struct SplitDequant {
   uint32_t mesh = ~0u;
   double mx[3] = {0.0, 0.0, 0.0}, r[3] = {0.0, 0.0, 0.0};
 
+  // a meshlet of n local vertices at v_off in mesh mesh_id (parameters P)
   template <bool NARROW, class CodeT>
-  __device__ __forceinline__ void meshlet(const CodeT* __restrict__ cb, const MeshletEntry& e,
-                                          const MeshParams& P, float* __restrict__ V, int lane) {
-    const uint32_t n = e.n_local, cs = split_code_stride(n, NARROW);
-    float* vo = V + 3 * (P.v_base + e.v_off);
-    if (NARROW && e.mesh != mesh) {
-      mesh = e.mesh;
+  __device__ __forceinline__ void meshlet(const CodeT* __restrict__ cb, uint32_t n, uint32_t v_off,
+                                          uint32_t mesh_id, const MeshParams& P,
+                                          float* __restrict__ V, int lane) {
+    const uint32_t cs = split_code_stride(n, NARROW);
+    float* vo = V + 3 * (P.v_base + v_off);
+    if (NARROW && mesh_id != mesh) {
+      mesh = mesh_id;
 #pragma unroll
       for (int a = 0; a < 3; ++a) {
         mx[a] = (double)((1u << P.g_bits[a]) - 1u);
@@ -475,13 +477,17 @@ __global__ __launch_bounds__(64) void stride_decode_split_kernel(
         overflow[g] = 1u;
     }
     __syncthreads();
-    // dequantise, one meshlet at a time, one vertex (x, y, z) per lane
+    // dequantise, one meshlet at a time, one vertex (x, y, z) per lane; a
+    // group lies in one mesh, and each meshlet's size and output offset come
+    // from its slot lane
     SplitDequant dq;
+    const uint32_t gmesh = (uint32_t)__shfl((int)(active ? e.mesh : 0u), 0, 64);
+    const MeshParams& Pg = meshes[gmesh];
+    const uint32_t voff = active ? e.v_off : 0u;
     for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
-      const MeshletEntry ej = tab[G.m0 + j];
-      const uint32_t pj = __shfl(pre0, (int)j, 64);
-      dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + G.codes_off + pj + ej.n_local), ej,
-                         meshes[ej.mesh], V, lane);
+      const uint32_t pj = __shfl(pre0, (int)j, 64), nj = __shfl(nl, (int)j, 64);
+      dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + G.codes_off + pj + nj), nj,
+                         __shfl(voff, (int)j, 64), gmesh, Pg, V, lane);
     }
     __syncthreads();
   }
@@ -508,12 +514,15 @@ __global__ __launch_bounds__(64) void stride_tokens_kernel(
   const int lane = threadIdx.x;
   const GroupEntry G = groups[blockIdx.x];
   const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
+  // the lane's entry and the group's mesh (a token group lies in one mesh)
+  // are read while the staging loads are in flight
+  const uint32_t m = G.m0 + (uint32_t)lane;
+  MeshletEntry e{};
+  if (m < G.m1) e = tab[m];
+  const MeshParams& P = meshes[tab[G.m0].mesh];
   stage_words(lds32, words, G.w_lo, G.n_words, lane, 64);
   __syncthreads();
-  const uint32_t m = G.m0 + (uint32_t)lane;
   if (m < G.m1) {
-    const MeshletEntry e = tab[m];
-    const MeshParams& P = meshes[e.mesh];
     const uint64_t abs_bit = P.blob_byte * 8ull + e.start_bit;
     const uint32_t rel = (uint32_t)(abs_bit - 32ull * G.w_lo);
     // a constant stride: the word index scales by a shift, not a multiply
@@ -591,11 +600,16 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
     if (!ok && overflow) overflow[g] = 1u;
   }
   __syncthreads();
+  // a group lies in one mesh; each meshlet's size and output offset come
+  // from its slot lane, not from memory
   SplitDequant dq;
+  const uint32_t gmesh = (uint32_t)__shfl((int)(active ? e.mesh : 0u), 0, 64);
+  const MeshParams& Pg = meshes[gmesh];
+  const uint32_t voff = active ? e.v_off : 0u;
   for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
-    const MeshletEntry ej = tab[G.m0 + j];
     const uint32_t pj = __shfl(pre0, (int)j, 64);
-    dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + pj), ej, meshes[ej.mesh], V, lane);
+    dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + pj), __shfl(nl, (int)j, 64),
+                       __shfl(voff, (int)j, 64), gmesh, Pg, V, lane);
   }
 }
 
