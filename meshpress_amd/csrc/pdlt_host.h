@@ -127,7 +127,7 @@ inline int pdlt_parse(const uint8_t* d, size_t len, PdltHost& H, std::string& er
     err = "stream shorter than its boundary and offset tables";
     return STRIDE_E_TRUNCATED;
   }
-  const size_t nw = (len + 3) / 4 + 4;
+  const size_t nw = (len + 3) / 4 + 8;   // the staged ranges read up to 6 words past a meshlet
   H.words.assign(nw, 0);
   std::memcpy(H.words.data(), d, len);
   GlobalWords src{H.words.data(), (uint32_t)nw};
@@ -230,8 +230,9 @@ inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vec
     g.w_lo = tab[m].start_bit >> 5;
     uint32_t blk = 0, words = 0, j = m;
     while (j < n_m && (int)(j - m) < gs) {
-      // the words holding [start, end) plus one word read ahead, even count
-      const uint32_t w = ((((tab[j].end_bit - 1) >> 5) + 2 - g.w_lo) + 1) & ~1u;
+      // the words holding [start, end) plus the axis lanes' read-ahead (the
+      // stream window reads up to 3 words past its position), even count
+      const uint32_t w = ((((tab[j].end_bit - 1) >> 5) + 6 - g.w_lo) + 1) & ~1u;
       const uint32_t b = pdlt_block_words(tab[j].n_bnd, tab[j].n_int);
       if (j > m && 4ull * (w + blk + b) > (uint64_t)budget) break;
       tab[j].blk_off = blk;

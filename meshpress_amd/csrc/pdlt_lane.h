@@ -421,14 +421,28 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
   }
   const double w0 = P.w[0], w1 = P.w[1], w2 = P.w[2], delta = P.delta;
   const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, deltaf = (float)delta;
+  // the codes stream through a window with the next word read ahead, so no
+  // step waits on an LDS read for its bits (the staged range holds 6 words
+  // past the meshlet's end for the read-ahead); a codeword of at most 33
+  // bits decodes from the window in one step (tag 1 Rice, 2 Exp-Golomb: the
+  // coders of DPR4's residuals), a longer one bit-serially
+  BitStream<LdsStaged> bs;
+  bs.init(LdsStaged{st, 0}, p);
+  const bool eg = tag != 1;
   for (uint32_t i = 0; i < n_emit; ++i) {
     int32_t code;
     if (tag == 0) {
-      code = (int32_t)((int64_t)mn + (int64_t)v5_bits(st, p, bw));
-      p += bw;
+      code = (int32_t)((int64_t)mn + (int64_t)(bw ? (uint32_t)(bs.win >> (64 - bw)) : 0u));
+      bs.drop(bw);
     } else {
-      const uint32_t u = tag == 1 ? v5_rice(st, p, k) : v5_eg(st, p, k);
-      code = (int32_t)((u >> 1) ^ (0u - (u & 1u)));
+      uint32_t u;
+      const uint32_t len = codeword_win(bs.win, eg, (int)k, u);
+      if (len) {
+        bs.drop(len);
+        code = (int32_t)((u >> 1) ^ (0u - (u & 1u)));
+      } else {
+        code = (int32_t)codeword_s(bs, eg, (int)k);
+      }
     }
     const uint32_t ent = walk[i];
     const uint32_t v = ent & 0xFFu, a = (ent >> 8) & 0xFFu;
