@@ -252,60 +252,47 @@ STRIDE_DEV uint32_t pdlt_scan(const Src& src, uint32_t start, uint64_t end_all,
 // stream bit to a staged bit: staged = stream - 32 * w_lo).
 // ---------------------------------------------------------------------------
 
-// n <= 32 bits at p
-STRIDE_DEV uint32_t v5_bits(const uint32_t* __restrict__ st, uint32_t p, uint32_t n) {
-  return n ? (uint32_t)(win64(st, p) >> (64 - n)) : 0u;
-}
-
-// unary prefix at p (runs of any length), p moves past the terminating 1
-STRIDE_DEV uint32_t v5_unary(const uint32_t* __restrict__ st, uint32_t& p) {
-  uint32_t q = 0;
-  uint64_t W = win64(st, p);
-  while ((W >> 32) == 0) {
-    q += 32;
-    p += 32;
-    W = win64(st, p);
+// Unsigned Rice(k) value from a stream: one window step for codewords of at
+// most 33 bits, bit-serial beyond (the pre-pass bounds the value below 2^32).
+template <class Stream>
+STRIDE_DEV uint32_t rice_raw(Stream& s, uint32_t k) {
+  uint32_t u;
+  const uint32_t len = codeword_win(s.win, false, (int)k, u);
+  if (len) {
+    s.drop(len);
+    return u;
   }
-  const uint32_t z = (uint32_t)__builtin_clzll(W);
-  p += z + 1;
-  return q + z;
+  const uint32_t q = unary_s(s);
+  return (q << k) | (k ? s.bits(k) : 0u);
 }
 
-// Rice(k) / Exp-Golomb(k) value (the pre-pass bounds it below 2^32)
-STRIDE_DEV uint32_t v5_rice(const uint32_t* __restrict__ st, uint32_t& p, uint32_t k) {
-  const uint32_t q = v5_unary(st, p);
-  const uint32_t low = v5_bits(st, p, k);
-  p += k;
-  return (q << k) | low;
-}
-STRIDE_DEV uint32_t v5_eg(const uint32_t* __restrict__ st, uint32_t& p, uint32_t k) {
-  const uint32_t lb = v5_unary(st, p);
-  const uint32_t off = v5_bits(st, p, lb);
-  p += lb;
-  const uint32_t shifted = ((lb ? (1u << lb) : 1u) - 1u) + off;
-  const uint32_t low = v5_bits(st, p, k);
-  p += k;
-  return (shifted << k) | low;
-}
-
-// Boundary refs of a meshlet into refs[] (Rice-coded differences minus one).
 STRIDE_DEV void pdlt_refs(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e,
                           uint32_t* __restrict__ refs) {
   const uint32_t n_bnd = e.n_bnd;
-  uint32_t p = (uint32_t)((int64_t)e.start_bit + rel) + 64u;
   if (n_bnd == 0) return;
-  uint32_t prev = v5_bits(st, p, 32);
-  p += 32;
+  // the refs stream through a read-ahead window (the strip tokens follow)
+  BitStream<LdsStaged> bs;
+  bs.init(LdsStaged{st, 0}, (uint64_t)((int64_t)e.start_bit + rel) + 64u);
+  uint32_t prev = bs.bits(32);
   refs[0] = prev;
   if (n_bnd > 1) {
-    const uint32_t k = v5_bits(st, p, 8);
-    p += 8;
+    const uint32_t k = bs.bits(8);
     for (uint32_t i = 1; i < n_bnd; ++i) {
-      prev = prev + v5_rice(st, p, k) + 1u;
+      prev = prev + rice_raw(bs, k) + 1u;
       refs[i] = prev;
     }
   }
 }
+
+// Output ids of a PDLT meshlet's locals: boundary locals through their
+// global refs, interior locals after the meshlet's interior base; strip
+// roots enter the walk as v in every byte (the chain's a == v test).
+struct PdltIds {
+  const uint32_t* __restrict__ refs;
+  uint32_t n_bnd, ibase;
+  STRIDE_DEV uint32_t operator()(uint32_t x) const { return x < n_bnd ? refs[x] : ibase + x; }
+  STRIDE_DEV static uint32_t root(uint32_t v) { return v * 0x01010101u; }
+};
 
 // Token lane: boundary refs into refs[], strips into the walk (interior
 // locals in first-use order: v * 0x01010101 for a strip-root vertex,
@@ -318,64 +305,15 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
                             uint32_t* __restrict__ tout) {
   const uint32_t n_bnd = e.n_bnd, n_local = e.n_bnd + e.n_int;
   pdlt_refs(st, rel, e, refs);
-  const uint32_t n_tris = e.n_tris, n_strips = e.n_strips & kStripMask;
-  const bool raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
-  uint32_t pc = (uint32_t)((int64_t)e.strips_bit + rel);
-  const uint32_t ib = pdlt_idx_bits(n_local);
-  MtfLite fifo;
-  fifo.clear();
-  // seen set, boundary locals preset
+  // seen set, boundary locals preset: only interior locals enter the walk
   BitSeen seen{seen_out};
   seen.preset(n_bnd);
-  uint32_t nw = 0, q0 = 0, q1 = 0, q2 = 0;
-  uint32_t root_j = 3, steps_left = 0;
-  const uint32_t ibase = e.int_base - n_bnd;
-  auto gid = [&](uint32_t x) { return x < n_bnd ? refs[x] : ibase + x; };
-  const uint32_t n_tok = n_tris + 2 * n_strips;
-  uint32_t* o = tout;
-  for (uint32_t it = 0; it < n_tok; ++it) {
-    uint32_t W = win32(st, pc);
-    const bool start = root_j == 3 && steps_left == 0;
-    const uint32_t L = W >> 16;
-    steps_left = start ? (L ? L - 1 : 0u) : steps_left;
-    W = start ? (W << 16) : W;
-    root_j = start ? 0u : root_j;
-    const bool is_root = root_j < 3;
-    const uint32_t ec = is_root ? 0u : (W >> 31);
-    W = is_root ? W : (W << 1);
-    const uint32_t flag = W >> 31;
-    const uint32_t val = flag ? ((W << 1) >> (32 - ib)) : ((W >> 26) & 31u);
-    pc += (start ? 16u : 0u) + (is_root ? 1u : 2u) + (flag ? ib : 5u);
-    const bool hit = flag == 0;
-    uint32_t j = hit ? ((fifo.n - 1 - val) & 15u) : fifo.n;
-    const uint32_t v = hit ? fifo.rank(j) : (val & 0xFFu);
-    if (raw_in_fifo && !hit) {
-      const int pf = fifo.find(v);
-      j = pf >= 0 ? (uint32_t)pf : j;
-    }
-    fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
-    fifo.to_front(j, v);
-    const bool fresh = seen.fresh(v);
-    const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
-    const uint32_t entry = is_root ? v * 0x01010101u : (v | (ra << 8) | (q2 << 16) | (rc << 24));
-    if (fresh) walk[nw & 255u] = entry;
-    nw += fresh ? 1u : 0u;
-    const bool emit = !is_root || root_j == 2;
-    const uint32_t ta = is_root ? q0 : ra;
-    const uint32_t tb = is_root ? q1 : q2;
-    if (emit) {
-      o[0] = gid(ta);
-      o[1] = gid(tb);
-      o[2] = gid(v);
-      o += 3;
-    }
-    q0 = is_root ? (root_j == 0 ? v : q0) : ra;
-    q1 = is_root ? (root_j == 1 ? v : q1) : q2;
-    q2 = is_root ? (root_j == 2 ? v : q2) : v;
-    root_j += is_root ? 1u : 0u;
-    steps_left -= is_root ? 0u : 1u;
-  }
-  *n_emit = nw;
+  LdsCursor cur{st, (uint32_t)((int64_t)e.strips_bit + rel)};
+  TokenLane<PdltIds> t;
+  t.init(cur, n_local, e.n_tris, e.n_strips, tout, PdltIds{refs, n_bnd, e.int_base - n_bnd});
+  WalkArray sink{walk};
+  for (uint32_t it = 0; it < t.n_tok; ++it) t.step(cur, seen, sink);
+  *n_emit = t.nw;
 }
 
 // Axis lane d: boundary centroid of axis d (the boundary rows summed in
@@ -405,7 +343,10 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
     for (uint32_t i = 0; i < n_bnd; ++i) s = s + rc[i];
     fb = n_bnd ? s / (double)n_bnd : 0.0;
   }
-  const uint32_t ab = d == 0 ? e.axis_bit[0] : (d == 1 ? e.axis_bit[1] : e.axis_bit[2]);
+  // a mask blend, not a select of three loads: that select folds into an
+  // indexed load, which keeps the caller's entry out of registers
+  const uint32_t m1 = 0u - (uint32_t)(d == 1), m2 = 0u - (uint32_t)(d == 2);
+  const uint32_t ab = (e.axis_bit[0] & ~(m1 | m2)) | (e.axis_bit[1] & m1) | (e.axis_bit[2] & m2);
   uint32_t p = (uint32_t)((int64_t)ab + rel);
   const uint64_t H = win64(st, p);
   const uint32_t tag = (uint32_t)(H >> 56);

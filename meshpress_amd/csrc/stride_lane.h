@@ -317,25 +317,37 @@ struct LdsCursor {            // staged words in LDS
 
 // Token lane: strip tokens -> triangles (tout) and the walk.  `cur` starts
 // at the first strip length.
+// Output ids of a DPR4 meshlet's locals (v_off + local) and its walk's root
+// entries (kRootMark | local).
+struct MeshletIds {
+  uint32_t v_off;
+  STRIDE_DEV uint32_t operator()(uint32_t x) const { return v_off + x; }
+  STRIDE_DEV static uint32_t root(uint32_t v) { return kRootMark | v; }
+};
+
 // The token lane as a stepper: one strip token per step().  A first
 // sighting hands its walk entry (local id + parallelogram refs, first-seen
-// order) to sink(index, entry); the triangles go to `o`.  The split and
-// two-pass kernels run steps back to back (split_tokens_c).
+// order; Ids::root(v) for strip roots) to sink(index, entry); the triangles
+// go to `o` as ids(local).  The split, two-pass and PDLT token kernels run
+// steps back to back (split_tokens_c, pdlt_tokens).
+template <class Ids>
 struct TokenLane {
   uint32_t n_tok, it;      // tokens in the meshlet, tokens done
-  uint32_t ib, v_off;
+  uint32_t ib;
   uint32_t nw;             // walk entries emitted
   uint32_t q0, q1, q2, root_j, steps_left;
   uint32_t Wn;             // the next token's 32-bit window, read one token ahead
   bool raw_in_fifo;
   MtfLite fifo;
+  Ids ids;
   uint32_t* __restrict__ o;
 
+  // strips_flags: the strip count | kFlagRawInFifo
   template <class Cursor>
-  STRIDE_DEV void init(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ tout) {
-    const uint32_t n_local = e.n_local;
-    n_tok = e.n_tris + 2 * (e.n_strips & kStripMask);
-    raw_in_fifo = (e.n_strips & kFlagRawInFifo) != 0;
+  STRIDE_DEV void init(Cursor& cur, uint32_t n_local, uint32_t n_tris, uint32_t strips_flags,
+                       uint32_t* __restrict__ tout, const Ids& id_map) {
+    n_tok = n_tris + 2 * (strips_flags & kStripMask);
+    raw_in_fifo = (strips_flags & kFlagRawInFifo) != 0;
     ib = 1;
     while ((1u << ib) < n_local + 1) ++ib;
     fifo.clear();
@@ -344,7 +356,7 @@ struct TokenLane {
     q0 = q1 = q2 = 0;
     root_j = 3;
     steps_left = 0;
-    v_off = e.v_off;
+    ids = id_map;
     o = tout;
     // strip length (16 bits) + edge bit + flag + id (<= 9 bits): one 32-bit
     // window per token.  The next token's window is read as soon as this
@@ -359,7 +371,9 @@ struct TokenLane {
   STRIDE_DEV void step(Cursor& cur, Seen& seen, Sink& sink) {
     uint32_t W = Wn;
     const bool start = root_j == 3 && steps_left == 0;
-    steps_left = start ? (W >> 16) - 1 : steps_left;
+    // a strip of length L has max(L, 1) triangles (PDLT allows L = 0)
+    const uint32_t L = W >> 16;
+    steps_left = start ? (L ? L - 1 : 0u) : steps_left;
     W = start ? (W << 16) : W;
     root_j = start ? 0u : root_j;
     const bool is_root = root_j < 3;
@@ -385,7 +399,7 @@ struct TokenLane {
     // ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
     const bool fresh = seen.fresh(v);
     const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
-    const uint32_t entry = is_root ? (kRootMark | v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
+    const uint32_t entry = is_root ? Ids::root(v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
     if (fresh) sink(nw, entry);
     nw += fresh ? 1u : 0u;
     // triangle (a, b, v) and the next previous triangle
@@ -393,9 +407,9 @@ struct TokenLane {
     const uint32_t ta = is_root ? q0 : ra;
     const uint32_t tb = is_root ? q1 : q2;
     if (emit) {
-      o[0] = v_off + ta;
-      o[1] = v_off + tb;
-      o[2] = v_off + v;
+      o[0] = ids(ta);
+      o[1] = ids(tb);
+      o[2] = ids(v);
       o += 3;
     }
     q0 = is_root ? (root_j == 0 ? v : q0) : ra;
@@ -416,8 +430,8 @@ struct WalkArray {
 template <class Cursor, class Seen>
 STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ walk,
                                Seen& seen, uint32_t* __restrict__ tout) {
-  TokenLane t;
-  t.init(cur, e, tout);
+  TokenLane<MeshletIds> t;
+  t.init(cur, e.n_local, e.n_tris, e.n_strips, tout, MeshletIds{e.v_off});
   WalkArray sink{walk};
   for (uint32_t it = 0; it < t.n_tok; ++it) t.step(cur, seen, sink);
 }
@@ -581,6 +595,23 @@ struct StreamBits : BitStream<BufferWords> {
 };
 
 // codeword() on a stream.
+// A unary prefix (zeros, then a one) of any length from a stream: the
+// window's top 32 bits (always valid) at a time.  Returns the zero count.
+template <class Stream>
+STRIDE_DEV uint32_t unary_s(Stream& s) {
+  uint32_t q = 0;
+  for (;;) {
+    const uint32_t top = (uint32_t)(s.win >> 32);
+    if (top) {
+      const uint32_t z = (uint32_t)__builtin_clz(top);
+      s.drop(z + 1);
+      return q + z;
+    }
+    q += 32;
+    s.drop(32);
+  }
+}
+
 template <class Stream>
 STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
   const uint64_t W = s.win;
@@ -596,9 +627,8 @@ STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
     u = (shifted << k) | low;
     s.drop(len);
   } else {
-    // long codeword (outliers): the unary part bit by bit, then the rest
-    uint32_t z = 0;
-    while (s.bits(1) == 0) ++z;
+    // long codeword (outliers): the unary part 32 bits at a time, then the rest
+    const uint32_t z = unary_s(s);
     uint64_t shifted = z;
     if (eg) shifted = z ? (((1ull << z) - 1) + s.bits(z)) : 0ull;
     const uint64_t low = k ? s.bits((uint32_t)k) : 0ull;
