@@ -195,8 +195,22 @@ def single_mesh(name, V, T, bits, steps, warmup, tune=True):
 
 # PDLT v5 launch shapes: (mode, group_size, lds_bytes, token-pass LDS words)
 PDLT_LAYOUTS = [(4, 16, 24576, 4096), (4, 8, 24576, 4096), (4, 16, 16384, 4096),
-                (4, 16, 49152, 4096), (4, 16, 24576, 8192), (3, 16, 49152, 4096),
-                (3, 16, 24576, 4096)]
+                (4, 16, 49152, 4096), (4, 16, 24576, 8192), (4, 16, 16384, 8192),
+                (4, 8, 12288, 8192), (3, 16, 49152, 4096), (3, 16, 24576, 4096)]
+
+
+def tune_layout(dec):
+    """The fastest PDLT_LAYOUTS entry for a v5 decoder (its own chain: the
+    float32 chain's smaller LDS blocks favour other group sizes)."""
+    sweep = []
+    for mode, gs, lds, sw in PDLT_LAYOUTS:
+        dec.set_tune(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
+        _, g = time_calls(dec.decode_world, 10, 3)
+        sweep.append([mode, gs, lds, sw, round(g * 1e3, 4)])
+    mode, gs, lds, sw, _ = min(sweep, key=lambda x: x[-1])
+    best = dict(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
+    dec.set_tune(**best)
+    return best, sweep
 
 
 def pdlt_v5(prep, name, steps, warmup):
@@ -214,19 +228,12 @@ def pdlt_v5(prep, name, steps, warmup):
     create_s = time.perf_counter() - t0
     log(f"{name} v5: {len(data):,} bytes ({8 * len(data) / prep['n_v']:.2f} bpv), "
         f"{dec.n_boundary:,} boundary vertices, encode {enc_s:.1f}s, create {create_s:.2f}s")
-    sweep = []
-    for mode, gs, lds, sw in PDLT_LAYOUTS:
-        dec.set_tune(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
-        _, g = time_calls(dec.decode_world, 10, 3)
-        sweep.append([mode, gs, lds, sw, round(g * 1e3, 4)])
-    mode, gs, lds, sw, _ = min(sweep, key=lambda x: x[-1])
-    best = dict(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
-    dec.set_tune(**best)
+    best, sweep = tune_layout(dec)
     s_wall, s_gpu = time_calls(dec.decode_world, steps, warmup)
     v64, t64 = dec.decode_world()
     v64 = v64.clone()
     dec32 = ParaDeltaV5GpuDecoder(data, chain="f32")
-    dec32.set_tune(**best)
+    best32, _ = tune_layout(dec32)
     w32, g32 = time_calls(dec32.decode_world, steps, warmup)
     v32, t32 = dec32.decode_world()
     torch.cuda.synchronize()
@@ -240,7 +247,7 @@ def pdlt_v5(prep, name, steps, warmup):
          "tune": best, "tune_sweep_ms": sweep,
          "ms_wall": s_wall * 1e3, "ms_kernels": s_gpu * 1e3, "gtris_s": dec.n_t_total / s_wall / 1e9,
          "hbm_gb_s": bytes_alg / s_gpu / 1e9, "algorithmic_bytes": bytes_alg,
-         "f32_chain": {"ms_kernels": g32 * 1e3, "gtris_s": dec.n_t_total / w32 / 1e9,
+         "f32_chain": {"ms_kernels": g32 * 1e3, "gtris_s": dec.n_t_total / w32 / 1e9, "tune": best32,
                        "tris_equal": bool(torch.equal(t64, t32)),
                        "max_f64_f32_drift_steps": drift}}
     log(f"{name} v5: {s_gpu * 1e3:.3f} ms, {r['gtris_s']:.3f} Gtris/s (f32 chain {g32 * 1e3:.3f} ms), "

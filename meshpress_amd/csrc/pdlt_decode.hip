@@ -35,29 +35,34 @@ __global__ __launch_bounds__(256) void pdlt_boundary_kernel(const double* __rest
 
 // Phases B-D of a group (after the walk, seen set, walk length and refs of
 // every meshlet are in its LDS block): boundary gather, axis chains, output.
+// T: the reconstruction type, double (float64 chain) or float (float32 chain,
+// half the LDS per vertex); a block holds three T arrays of n_local, then
+// refs, walk and seen set.
+template <class T>
 __device__ __forceinline__ void pdlt_group_tail(const uint32_t* __restrict__ lds_c, uint32_t* lds,
                                                 const PdltGroup& G, const PdltEntry& e, bool active,
                                                 int role, int lane, int64_t rel,
                                                 const PdltEntry* __restrict__ tab,
                                                 const double* __restrict__ bnd, const PdltParams& P,
                                                 float* __restrict__ V) {
+  constexpr uint32_t RW = 3 * sizeof(T) / 4;   // block words of reconstruction per local
   const uint32_t nl = e.n_bnd + e.n_int;
   uint32_t* blk = lds + G.blk_base + e.blk_off;
-  double* recon = reinterpret_cast<double*>(blk);
-  const uint32_t* refs = blk + 6 * nl;
+  T* recon = reinterpret_cast<T*>(blk);
+  const uint32_t* refs = blk + RW * nl;
   const uint32_t* walk = refs + e.n_bnd;
   const uint32_t* n_emit = walk + e.n_int + 8;
   // boundary reconstructions: the four lanes of a slot gather its refs
   if (active) {
     const uint32_t nb3 = 3 * e.n_bnd;
     for (uint32_t base = (uint32_t)role; base < nb3; base += 32) {
-      double x[8];
+      T x[8];
 #pragma unroll
       for (uint32_t u = 0; u < 8; ++u) {
         const uint32_t idx = base + 4 * u;
         if (idx < nb3) {
           const uint32_t i = idx / 3, d = idx - 3 * i;
-          x[u] = pdlt_bnd(bnd[3ull * refs[i] + d], P);
+          x[u] = (T)pdlt_bnd(bnd[3ull * refs[i] + d], P);
         }
       }
 #pragma unroll
@@ -79,19 +84,21 @@ __device__ __forceinline__ void pdlt_group_tail(const uint32_t* __restrict__ lds
     const PdltEntry ej = tab[G.m0 + j];
     const uint32_t nlj = ej.n_bnd + ej.n_int;
     const uint32_t* bj = lds + G.blk_base + ej.blk_off;
-    const double* rj = reinterpret_cast<const double*>(bj);
-    const uint32_t* sj = bj + 6 * nlj + ej.n_bnd + ej.n_int;
+    const T* rj = reinterpret_cast<const T*>(bj);
+    const uint32_t* sj = bj + RW * nlj + ej.n_bnd + ej.n_int;
     float* vo = V + 3ull * ej.int_base;
     const uint32_t n3 = 3 * ej.n_int;
     for (uint32_t t = lane; t < n3; t += 64) {
       const uint32_t i = t / 3, d = t - 3 * i, v = ej.n_bnd + i;
       const bool reached = ((sj[v >> 5] >> (v & 31)) & 1u) != 0;
-      vo[t] = pdlt_out(reached ? rj[d * nlj + v] : 0.0, P, (int)d);
+      vo[t] = pdlt_out(reached ? (double)rj[d * nlj + v] : 0.0, P, (int)d);
     }
   }
 }
 
-// Single-pass decode: the token lanes run inside the group.
+// Single-pass decode: the token lanes run inside the group.  R: the
+// reconstruction type (pdlt_group_tail).
+template <class R>
 __global__ __launch_bounds__(64) void pdlt_decode_kernel(
     const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
     const PdltGroup* __restrict__ groups, const double* __restrict__ bnd, PdltParams P,
@@ -108,14 +115,14 @@ __global__ __launch_bounds__(64) void pdlt_decode_kernel(
   if (active) e = tab[m];
   const uint32_t nl = e.n_bnd + e.n_int;
   uint32_t* blk = lds + G.blk_base + e.blk_off;
-  uint32_t* refs = blk + 6 * nl;
+  uint32_t* refs = blk + (3 * sizeof(R) / 4) * nl;
   uint32_t* walk = refs + e.n_bnd;
   uint32_t* seen = walk + e.n_int;
   const int64_t rel = -32 * (int64_t)G.w_lo;
   __syncthreads();
   if (active && role == 0) pdlt_tokens(lds, rel, e, refs, walk, seen, seen + 8, T + 3ull * e.t_off);
   __syncthreads();
-  pdlt_group_tail(lds, lds, G, e, active, role, lane, rel, tab, bnd, P, V);
+  pdlt_group_tail<R>(lds, lds, G, e, active, role, lane, rel, tab, bnd, P, V);
 }
 
 // Two-pass decode, pass 1: one token lane per meshlet, up to 64 meshlets
@@ -148,6 +155,7 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
 
 // Two-pass decode, pass 2: the group's blocks take the walk, seen set and
 // walk length from HBM and the refs decoded again, then phases B-D.
+template <class R>
 __global__ __launch_bounds__(64) void pdlt_chain_kernel(
     const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
     const PdltGroup* __restrict__ groups, const double* __restrict__ bnd, PdltParams P,
@@ -165,7 +173,7 @@ __global__ __launch_bounds__(64) void pdlt_chain_kernel(
   if (active) e = tab[m];
   const uint32_t nl = e.n_bnd + e.n_int;
   uint32_t* blk = lds + G.blk_base + e.blk_off;
-  uint32_t* refs = blk + 6 * nl;
+  uint32_t* refs = blk + (3 * sizeof(R) / 4) * nl;
   uint32_t* walk = refs + e.n_bnd;
   uint32_t* seen = walk + e.n_int;
   const int64_t rel = -32 * (int64_t)G.w_lo;
@@ -178,7 +186,7 @@ __global__ __launch_bounds__(64) void pdlt_chain_kernel(
   __syncthreads();
   if (active && role == 0) pdlt_refs(lds, rel, e, refs);
   __syncthreads();
-  pdlt_group_tail(lds, lds, G, e, active, role, lane, rel, tab, bnd, P, V);
+  pdlt_group_tail<R>(lds, lds, G, e, active, role, lane, rel, tab, bnd, P, V);
 }
 
 // ---------------------------------------------------------------------------
@@ -233,7 +241,8 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   std::vector<PdltGroup> gr, tg;
   uint32_t lds = 0, tlds = 0;
   std::vector<PdltEntry> tab = H.tab;
-  int rc = pdlt_layout(tab, gs, budget, gr, lds, err);
+  // the float32 chain keeps three float32 per vertex, the float64 chain three float64
+  int rc = pdlt_layout(tab, gs, budget, gr, lds, err, chain32 ? 3u : 6u);
   if (!rc) rc = pdlt_layout_tokens(tab, tok_budget, tok_group, tg, tlds, err);
   if (rc) return set_error(rc, err);
   H.tab.swap(tab);
@@ -284,11 +293,19 @@ int PdltSet::decode(float* V, uint32_t* T, int world, hipStream_t s) {
     hipLaunchKernelGGL(pdlt_tokens_kernel, dim3((uint32_t)tok_groups.size()), dim3(64), tok_lds_bytes,
                        s, d_words, d_tab, d_tok_groups, H.n_b, d_walk, d_seen, T);
     PDLT_HIP(hipGetLastError());
-    hipLaunchKernelGGL(pdlt_chain_kernel, dim3((uint32_t)groups.size()), dim3(64), lds_bytes, s,
-                       d_words, d_tab, d_groups, d_bnd, p, H.n_b, d_walk, d_seen, V);
+    if (chain32)
+      hipLaunchKernelGGL(pdlt_chain_kernel<float>, dim3((uint32_t)groups.size()), dim3(64),
+                         lds_bytes, s, d_words, d_tab, d_groups, d_bnd, p, H.n_b, d_walk, d_seen, V);
+    else
+      hipLaunchKernelGGL(pdlt_chain_kernel<double>, dim3((uint32_t)groups.size()), dim3(64),
+                         lds_bytes, s, d_words, d_tab, d_groups, d_bnd, p, H.n_b, d_walk, d_seen, V);
   } else {
-    hipLaunchKernelGGL(pdlt_decode_kernel, dim3((uint32_t)groups.size()), dim3(64), lds_bytes, s,
-                       d_words, d_tab, d_groups, d_bnd, p, V, T);
+    if (chain32)
+      hipLaunchKernelGGL(pdlt_decode_kernel<float>, dim3((uint32_t)groups.size()), dim3(64),
+                         lds_bytes, s, d_words, d_tab, d_groups, d_bnd, p, V, T);
+    else
+      hipLaunchKernelGGL(pdlt_decode_kernel<double>, dim3((uint32_t)groups.size()), dim3(64),
+                         lds_bytes, s, d_words, d_tab, d_groups, d_bnd, p, V, T);
   }
   PDLT_HIP(hipGetLastError());
   return STRIDE_OK;

@@ -96,9 +96,11 @@ enum : uint32_t {
 // n_local float64, axis major), refs (n_bnd), walk (n_int), the seen set
 // (8 words) and the walk length (1 word); even, so blocks stay 8-byte
 // aligned.
-STRIDE_DEV uint32_t pdlt_block_words(uint32_t n_bnd, uint32_t n_int) {
+// rw = words of reconstruction per local: 6 (three float64, the float64
+// chain) or 3 (three float32, the float32 chain)
+STRIDE_DEV uint32_t pdlt_block_words(uint32_t n_bnd, uint32_t n_int, uint32_t rw = 6) {
   const uint32_t n_local = n_bnd + n_int;
-  return (6 * n_local + n_bnd + n_int + 9 + 1) & ~1u;
+  return (rw * n_local + n_bnd + n_int + 9 + 1) & ~1u;
 }
 
 STRIDE_DEV uint32_t pdlt_idx_bits(uint32_t n_local) {
@@ -326,9 +328,11 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
 // computed its codes against (encoder/paradelta_v5_nb.py:44-59,113-176:
 // float32 boundary positions, float32 centroid, float32 LIN3 terms summed
 // left to right, code * float32 delta), so every position is the encoder's
-// own reconstruction; values stay in the float64 array, exactly.
+// own reconstruction; T is float for it (the values are float32 numbers), or
+// double.
+template <class T>
 STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e, int d,
-                          const PdltParams& P, double* __restrict__ rc,
+                          const PdltParams& P, T* __restrict__ rc,
                           const uint32_t* __restrict__ walk, uint32_t n_emit) {
   if (n_emit == 0) return;   // no residual section is read (n_int may be 0)
   const uint32_t n_bnd = e.n_bnd;
@@ -340,7 +344,7 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
     fb = n_bnd ? (double)(s / (float)n_bnd) : 0.0;
   } else {
     double s = 0.0;
-    for (uint32_t i = 0; i < n_bnd; ++i) s = s + rc[i];
+    for (uint32_t i = 0; i < n_bnd; ++i) s = s + (double)rc[i];
     fb = n_bnd ? s / (double)n_bnd : 0.0;
   }
   // a mask blend, not a select of three loads: that select folds into an
@@ -396,16 +400,16 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
         const float t2 = w2f * (float)rc[c];
         pred = (t0 + t1) + t2;
       }
-      rc[v] = (double)(pred + (float)code * deltaf);
+      rc[v] = (T)(pred + (float)code * deltaf);
     } else {
       double pred = fb;
       if (a != v) {
-        const double t0 = w0 * rc[a];
-        const double t1 = w1 * rc[b];
-        const double t2 = w2 * rc[c];
+        const double t0 = w0 * (double)rc[a];
+        const double t1 = w1 * (double)rc[b];
+        const double t2 = w2 * (double)rc[c];
         pred = (t0 + t1) + t2;
       }
-      rc[v] = pred + (double)code * delta;
+      rc[v] = (T)(pred + (double)code * delta);
     }
   }
 }

@@ -849,8 +849,15 @@ int stride_decoder_set_precision(stride_decoder* h, int32_t chain_bits) {
     if (chain_bits == 64) return STRIDE_OK;   // DPR4 decodes integer codes exactly
     return fail(STRIDE_E_ARG, "the float32 chain is defined for PDLT v5 streams only");
   }
-  h->v5->chain32 = chain_bits == 32 ? 1 : 0;
-  return STRIDE_OK;
+  const int c32 = chain_bits == 32 ? 1 : 0;
+  if (c32 == h->v5->chain32) return STRIDE_OK;
+  // the chain's LDS block is three float32 or three float64 per vertex; a
+  // failed layout leaves the previous one (and precision) in place
+  const int prev = h->v5->chain32;
+  h->v5->chain32 = c32;
+  const int rc = h->v5->set_layout(h->v5->group_size, h->v5->lds_budget, h->v5->tok_lds_budget);
+  if (rc) h->v5->chain32 = prev;
+  return rc;
 }
 
 int stride_decoder_format(const stride_decoder* h, int32_t* version, float* lin3,

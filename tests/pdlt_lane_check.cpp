@@ -46,7 +46,7 @@ int main(int argc, char** argv) {
   }
   std::vector<PdltGroup> groups;
   uint32_t lds_bytes = 0;
-  rc = pdlt_layout(H.tab, gs, budget, groups, lds_bytes, err);
+  rc = pdlt_layout(H.tab, gs, budget, groups, lds_bytes, err, chain32 ? 3u : 6u);
   if (rc) {
     std::fprintf(stderr, "layout (%d): %s\n", rc, err.c_str());
     return 2;
@@ -84,6 +84,11 @@ int main(int argc, char** argv) {
       }
     }
   }
+  // the kernels' block: three arrays of Tt per local (double for the float64
+  // chain, float for the float32 chain), then refs, walk, seen set
+  auto run_groups = [&](auto tag) {
+  using Tt = decltype(tag);
+  constexpr uint32_t RW = 3 * sizeof(Tt) / 4;
   for (const PdltGroup& G : groups) {
     std::vector<uint64_t> lds64((lds_bytes + 7) / 8, 0x5A5A5A5A5A5A5A5Aull);   // stale LDS
     uint32_t* lds = reinterpret_cast<uint32_t*>(lds64.data());
@@ -95,7 +100,7 @@ int main(int argc, char** argv) {
       const PdltEntry& e = H.tab[m];
       uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
-      uint32_t* refs = b + 6 * nl;
+      uint32_t* refs = b + RW * nl;
       uint32_t* walk = refs + e.n_bnd;
       uint32_t* seen = walk + e.n_int;
       if (!two_pass) {
@@ -111,18 +116,19 @@ int main(int argc, char** argv) {
       const PdltEntry& e = H.tab[m];
       uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
-      double* recon = reinterpret_cast<double*>(b);
-      const uint32_t* refs = b + 6 * nl;
+      Tt* recon = reinterpret_cast<Tt*>(b);
+      const uint32_t* refs = b + RW * nl;
       for (uint32_t i = 0; i < e.n_bnd; ++i)
-        for (int d = 0; d < 3; ++d) recon[d * nl + i] = pdlt_bnd(H.bnd.at(3ull * refs[i] + d), P);
+        for (int d = 0; d < 3; ++d)
+          recon[d * nl + i] = (Tt)pdlt_bnd(H.bnd.at(3ull * refs[i] + d), P);
     }
     // phase C: axis lanes
     for (uint32_t m = G.m0; m < G.m1; ++m) {
       const PdltEntry& e = H.tab[m];
       uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
-      double* recon = reinterpret_cast<double*>(b);
-      const uint32_t* walk = b + 6 * nl + e.n_bnd;
+      Tt* recon = reinterpret_cast<Tt*>(b);
+      const uint32_t* walk = b + RW * nl + e.n_bnd;
       const uint32_t n_emit = walk[e.n_int + 8];
       for (int d = 0; d < 3; ++d) pdlt_axis(lds, rel, e, d, P, recon + d * nl, walk, n_emit);
     }
@@ -131,15 +137,20 @@ int main(int argc, char** argv) {
       const PdltEntry& e = H.tab[m];
       const uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
-      const double* rj = reinterpret_cast<const double*>(b);
-      const uint32_t* sj = b + 6 * nl + e.n_bnd + e.n_int;
+      const Tt* rj = reinterpret_cast<const Tt*>(b);
+      const uint32_t* sj = b + RW * nl + e.n_bnd + e.n_int;
       for (uint32_t t = 0; t < 3 * e.n_int; ++t) {
         const uint32_t i = t / 3, d = t - 3 * i, v = e.n_bnd + i;
         const bool reached = ((sj[v >> 5] >> (v & 31)) & 1u) != 0;
-        V.at(3ull * e.int_base + t) = pdlt_out(reached ? rj[d * nl + v] : 0.0, P, (int)d);
+        V.at(3ull * e.int_base + t) = pdlt_out(reached ? (double)rj[d * nl + v] : 0.0, P, (int)d);
       }
     }
   }
+  };
+  if (chain32)
+    run_groups(float{});
+  else
+    run_groups(double{});
   FILE* fv = std::fopen(argv[2], "wb");
   FILE* ft = std::fopen(argv[3], "wb");
   if (!fv || !ft) return 1;
