@@ -393,23 +393,27 @@ struct TokenLane {
       const int pf = fifo.find(v);
       j = pf >= 0 ? (uint32_t)pf : j;
     }
+    // triangle (a, b, v): its output ids are looked up as soon as v is known
+    // (for PDLT v5 a boundary local's id is an LDS read), so the lookups
+    // overlap the FIFO and seen-set work before the store
+    const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
+    const uint32_t ta = is_root ? q0 : ra;
+    const uint32_t tb = is_root ? q1 : q2;
+    const uint32_t ia = ids(ta), ibb = ids(tb), iv = ids(v);
     fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
     fifo.to_front(j, v);
     // first sighting -> walk entry; parallelogram refs (a, b, c):
     // ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
     const bool fresh = seen.fresh(v);
-    const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
     const uint32_t entry = is_root ? Ids::root(v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
     if (fresh) sink(nw, entry);
     nw += fresh ? 1u : 0u;
-    // triangle (a, b, v) and the next previous triangle
+    // the triangle, and the next previous triangle
     const bool emit = !is_root || root_j == 2;
-    const uint32_t ta = is_root ? q0 : ra;
-    const uint32_t tb = is_root ? q1 : q2;
     if (emit) {
-      o[0] = ids(ta);
-      o[1] = ids(tb);
-      o[2] = ids(v);
+      o[0] = ia;
+      o[1] = ibb;
+      o[2] = iv;
       o += 3;
     }
     q0 = is_root ? (root_j == 0 ? v : q0) : ra;
