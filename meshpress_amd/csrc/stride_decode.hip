@@ -528,7 +528,8 @@ __global__ __launch_bounds__(64) void stride_tokens_kernel(
     // a constant stride: the word index scales by a shift, not a multiply
     BitSeen seen{lds32 + G.n_words + lane, kTokenSeenStride};
     seen.preset(0);
-    split_tokens(lds32, rel, e, walk_g + walk_slot(P, e, m), seen, T + 3 * (P.t_base + e.t_off));
+    split_tokens<WalkSlot>(lds32, rel, e, walk_g + walk_slot(P, e, m), seen,
+                           T + 3 * (P.t_base + e.t_off));
   }
 }
 

@@ -406,16 +406,20 @@ struct TokenLane {
     // ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
     const bool fresh = seen.fresh(v);
     const uint32_t entry = is_root ? Ids::root(v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
-    if (fresh) sink(nw, entry);
-    nw += fresh ? 1u : 0u;
-    // the triangle, and the next previous triangle
-    const bool emit = !is_root || root_j == 2;
-    if (emit) {
-      o[0] = ia;
-      o[1] = ibb;
-      o[2] = iv;
-      o += 3;
+    if constexpr (Sink::kEveryToken) {
+      sink(nw, entry);   // a stale entry at nw is overwritten by the next first sighting
+    } else {
+      if (fresh) sink(nw, entry);
     }
+    nw += fresh ? 1u : 0u;
+    // the triangle, and the next previous triangle.  Every token stores one:
+    // the first two roots of a strip store into the slot the third root
+    // then fills, so only emitting tokens advance o.
+    const bool emit = !is_root || root_j == 2;
+    o[0] = ia;
+    o[1] = ibb;
+    o[2] = iv;
+    o += emit ? 3 : 0;
     q0 = is_root ? (root_j == 0 ? v : q0) : ra;
     q1 = is_root ? (root_j == 1 ? v : q1) : q2;
     q2 = is_root ? (root_j == 2 ? v : q2) : v;
@@ -425,27 +429,36 @@ struct TokenLane {
   }
 };
 
-// Walk sink of the split and two-pass kernels: the whole walk in one array.
+// Walk sinks: the whole walk in one array.  WalkArray stores first
+// sightings only (the split kernel's LDS block has no room past n_local);
+// WalkSlot stores every token's entry at the next index, which needs one
+// word past the walk (the two-pass slots have it: walk_slot).
 struct WalkArray {
+  static constexpr bool kEveryToken = false;
   uint32_t* __restrict__ walk;
   STRIDE_DEV void operator()(uint32_t i, uint32_t entry) const { walk[i & 255u] = entry; }
 };
+struct WalkSlot {
+  static constexpr bool kEveryToken = true;
+  uint32_t* __restrict__ walk;
+  STRIDE_DEV void operator()(uint32_t i, uint32_t entry) const { walk[i] = entry; }
+};
 
-template <class Cursor, class Seen>
+template <class Sink = WalkArray, class Cursor, class Seen>
 STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __restrict__ walk,
                                Seen& seen, uint32_t* __restrict__ tout) {
   TokenLane<MeshletIds> t;
   t.init(cur, e.n_local, e.n_tris, e.n_strips, tout, MeshletIds{e.v_off});
-  WalkArray sink{walk};
+  Sink sink{walk};
   for (uint32_t it = 0; it < t.n_tok; ++it) t.step(cur, seen, sink);
 }
 
-template <class Seen>
+template <class Sink = WalkArray, class Seen>
 STRIDE_DEV void split_tokens(const uint32_t* __restrict__ st, uint32_t rel, const MeshletEntry& e,
                              uint32_t* __restrict__ walk, Seen& seen,
                              uint32_t* __restrict__ tout) {
   LdsCursor cur{st, rel + 48};
-  split_tokens_c(cur, e, walk, seen, tout);
+  split_tokens_c<Sink>(cur, e, walk, seen, tout);
 }
 
 STRIDE_DEV uint32_t pick3(int d, uint32_t x0, uint32_t x1, uint32_t x2) {

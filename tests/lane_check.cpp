@@ -111,7 +111,8 @@ int main(int argc, char** argv) {
       seen.preset(0);
       StreamBits cur;
       cur.init(le.data(), e.start_bit + 48);
-      split_tokens_c(cur, e, walk_g + walk_slot(P, e, m), seen, tris.data() + 3ull * e.t_off);
+      split_tokens_c<WalkSlot>(cur, e, walk_g + walk_slot(P, e, m), seen,
+                               tris.data() + 3ull * e.t_off);
     }
   for (uint32_t m = 0; m < n_m; ++m) {
     const MeshletEntry& e = tab[m];
