@@ -120,14 +120,17 @@ __global__ __launch_bounds__(64) void pdlt_decode_kernel(
   uint32_t* seen = walk + e.n_int;
   const int64_t rel = -32 * (int64_t)G.w_lo;
   __syncthreads();
-  if (active && role == 0) pdlt_tokens(lds, rel, e, refs, walk, seen, seen + 8, T + 3ull * e.t_off);
+  if (active && role == 0)
+    pdlt_tokens(lds, rel, e, refs, walk, BitSeen{seen}, seen + 8, T + 3ull * e.t_off);
   __syncthreads();
   pdlt_group_tail<R>(lds, lds, G, e, active, role, lane, rel, tab, bnd, P, V);
 }
 
 // Two-pass decode, pass 1: one token lane per meshlet, up to 64 meshlets
-// per wave (refs in LDS for the global triangle ids, seen set in LDS); the
-// walk, seen set and walk length go to HBM.
+// per wave (refs in LDS for the global triangle ids; the lanes' seen sets
+// interleaved in LDS, lane i's word q at seen_base + 64 q + i, so one access
+// of the wave touches distinct banks); the walk, seen set and walk length go
+// to HBM.
 __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
     const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
     const PdltGroup* __restrict__ groups, uint32_t n_b, uint32_t* __restrict__ walk_g,
@@ -142,13 +145,13 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
   if (m < G.m1) {
     const PdltEntry e = tab[m];
     uint32_t* refs = lds + G.blk_base + e.tok_ref_off;
-    uint32_t* seen = refs + e.n_bnd;
+    uint32_t* seen = lds + G.seen_base + lane;
     uint32_t n_emit = 0;
-    pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g + (e.int_base - n_b), seen, &n_emit,
-                T + 3ull * e.t_off);
+    pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g + (e.int_base - n_b),
+                BitSeen{seen, kTokenSeenStride}, &n_emit, T + 3ull * e.t_off);
     uint32_t* so = seen_g + (uint64_t)kSeenWords * m;
 #pragma unroll
-    for (int q = 0; q < 8; ++q) so[q] = seen[q];
+    for (int q = 0; q < 8; ++q) so[q] = seen[q * kTokenSeenStride];
     so[8] = n_emit;
   }
 }

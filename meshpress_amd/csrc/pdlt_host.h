@@ -253,8 +253,8 @@ inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vec
 }
 
 // Token groups of the two-pass decode: up to gmax (<= 64) consecutive
-// meshlets per wave (one token lane each) whose staged words, refs and seen
-// sets fit `budget` bytes.  Sets each entry's tok_ref_off; lds = the largest group's
+// meshlets per wave (one token lane each) whose staged words, refs and
+// interleaved seen sets (7 kTokenSeenStride + lanes words) fit `budget` bytes.  Sets each entry's tok_ref_off; lds = the largest group's
 // LDS bytes.
 inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, int gmax,
                               std::vector<PdltGroup>& groups, uint32_t& lds, std::string& err) {
@@ -271,17 +271,19 @@ inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, int gmax,
     uint32_t refs = 0, words = 0, j = m;
     while (j < n_m && j - m < (uint32_t)gmax) {
       const uint32_t w = ((((tab[j].end_bit - 1) >> 5) + 2 - g.w_lo) + 1) & ~1u;
-      // per meshlet: refs (n_bnd words), then the seen set (8 words)
-      if (j > m && 4ull * (w + refs + tab[j].n_bnd + 8) > (uint64_t)budget) break;
+      // per meshlet: refs (n_bnd words); then the group's seen sets
+      const uint64_t need = w + refs + tab[j].n_bnd + 7 * kTokenSeenStride + (j - m + 1);
+      if (j > m && 4 * need > (uint64_t)budget) break;
       tab[j].tok_ref_off = refs;
-      refs += tab[j].n_bnd + 8;
+      refs += tab[j].n_bnd;
       words = w;
       ++j;
     }
     g.m1 = j;
     g.n_words = words;
     g.blk_base = words;
-    max_lds = std::max(max_lds, 4 * (words + refs));
+    g.seen_base = words + refs;
+    max_lds = std::max(max_lds, 4 * (words + refs + 7 * kTokenSeenStride + (j - m)));
     groups.push_back(g);
     m = j;
   }

@@ -60,7 +60,8 @@ struct PdltGroup {
   uint32_t w_lo;         // first staged stream word
   uint32_t n_words;      // staged words (even)
   uint32_t blk_base;     // LDS word offset of the meshlet blocks (even)
-  uint32_t pad[3];
+  uint32_t seen_base;    // token groups: LDS word offset of the interleaved seen sets
+  uint32_t pad[2];
 };
 static_assert(sizeof(PdltGroup) == 32, "PdltGroup is 32 bytes");
 
@@ -289,26 +290,32 @@ STRIDE_DEV void pdlt_refs(const uint32_t* __restrict__ st, int64_t rel, const Pd
 // Output ids of a PDLT meshlet's locals: boundary locals through their
 // global refs, interior locals after the meshlet's interior base; strip
 // roots enter the walk as v in every byte (the chain's a == v test).
+// The lookup is branch-free: an interior local reads refs[0], which lies in
+// the LDS block in both layouts (the seen set follows the refs), and drops it.
 struct PdltIds {
+  static constexpr bool kCarry = true;
   const uint32_t* __restrict__ refs;
   uint32_t n_bnd, ibase;
-  STRIDE_DEV uint32_t operator()(uint32_t x) const { return x < n_bnd ? refs[x] : ibase + x; }
+  STRIDE_DEV uint32_t operator()(uint32_t x) const {
+    const bool b = x < n_bnd;
+    const uint32_t r = refs[b ? x : 0u];
+    return b ? r : ibase + x;
+  }
   STRIDE_DEV static uint32_t root(uint32_t v) { return v * 0x01010101u; }
 };
 
 // Token lane: boundary refs into refs[], strips into the walk (interior
 // locals in first-use order: v * 0x01010101 for a strip-root vertex,
 // v | a << 8 | b << 16 | c << 24 for a parallelogram step, a != v) and the
-// triangles into tout (global ids).  seen_out (8 words, LDS in the kernels)
-// is the live seen set; the walk length goes to *n_emit.
+// triangles into tout (global ids).  `seen` (LDS in the kernels) is the live
+// seen set; the walk length goes to *n_emit.
 STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e,
                             uint32_t* __restrict__ refs, uint32_t* __restrict__ walk,
-                            uint32_t* __restrict__ seen_out, uint32_t* __restrict__ n_emit,
+                            BitSeen seen, uint32_t* __restrict__ n_emit,
                             uint32_t* __restrict__ tout) {
   const uint32_t n_bnd = e.n_bnd, n_local = e.n_bnd + e.n_int;
   pdlt_refs(st, rel, e, refs);
-  // seen set, boundary locals preset: only interior locals enter the walk
-  BitSeen seen{seen_out};
+  // boundary locals preset: only interior locals enter the walk
   seen.preset(n_bnd);
   LdsCursor cur{st, (uint32_t)((int64_t)e.strips_bit + rel)};
   TokenLane<PdltIds> t;

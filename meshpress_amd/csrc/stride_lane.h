@@ -320,6 +320,7 @@ struct LdsCursor {            // staged words in LDS
 // Output ids of a DPR4 meshlet's locals (v_off + local) and its walk's root
 // entries (kRootMark | local).
 struct MeshletIds {
+  static constexpr bool kCarry = false;   // ids recomputed per token (an add)
   uint32_t v_off;
   STRIDE_DEV uint32_t operator()(uint32_t x) const { return v_off + x; }
   STRIDE_DEV static uint32_t root(uint32_t v) { return kRootMark | v; }
@@ -336,6 +337,7 @@ struct TokenLane {
   uint32_t ib;
   uint32_t nw;             // walk entries emitted
   uint32_t q0, q1, q2, root_j, steps_left;
+  uint32_t g0, g1, g2;     // ids(q0..q2) when Ids::kCarry (an id lookup reads LDS)
   uint32_t Wn;             // the next token's 32-bit window, read one token ahead
   bool raw_in_fifo;
   MtfLite fifo;
@@ -354,6 +356,7 @@ struct TokenLane {
     it = 0;
     nw = 0;
     q0 = q1 = q2 = 0;
+    g0 = g1 = g2 = 0;
     root_j = 3;
     steps_left = 0;
     ids = id_map;
@@ -394,12 +397,20 @@ struct TokenLane {
       j = pf >= 0 ? (uint32_t)pf : j;
     }
     // triangle (a, b, v): its output ids are looked up as soon as v is known
-    // (for PDLT v5 a boundary local's id is an LDS read), so the lookups
-    // overlap the FIFO and seen-set work before the store
+    // (for PDLT v5 a boundary local's id is an LDS read, so a and b carry
+    // theirs from the tokens that named them), and the lookup overlaps the
+    // FIFO and seen-set work before the store
     const uint32_t ra = ec ? q0 : q1, rc = ec ? q1 : q0;
-    const uint32_t ta = is_root ? q0 : ra;
-    const uint32_t tb = is_root ? q1 : q2;
-    const uint32_t ia = ids(ta), ibb = ids(tb), iv = ids(v);
+    const uint32_t iv = ids(v);
+    uint32_t ia, ibb, ga = 0;
+    if constexpr (Ids::kCarry) {
+      ga = ec ? g0 : g1;
+      ia = is_root ? g0 : ga;
+      ibb = is_root ? g1 : g2;
+    } else {
+      ia = ids(is_root ? q0 : ra);
+      ibb = ids(is_root ? q1 : q2);
+    }
     fifo.n += (!hit && j == fifo.n && fifo.n < 16) ? 1u : 0u;
     fifo.to_front(j, v);
     // first sighting -> walk entry; parallelogram refs (a, b, c):
@@ -423,6 +434,11 @@ struct TokenLane {
     q0 = is_root ? (root_j == 0 ? v : q0) : ra;
     q1 = is_root ? (root_j == 1 ? v : q1) : q2;
     q2 = is_root ? (root_j == 2 ? v : q2) : v;
+    if constexpr (Ids::kCarry) {
+      g0 = is_root ? (root_j == 0 ? iv : g0) : ga;
+      g1 = is_root ? (root_j == 1 ? iv : g1) : g2;
+      g2 = is_root ? (root_j == 2 ? iv : g2) : iv;
+    }
     root_j += is_root ? 1u : 0u;
     steps_left -= is_root ? 0u : 1u;
     ++it;

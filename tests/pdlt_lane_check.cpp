@@ -74,12 +74,12 @@ int main(int argc, char** argv) {
       for (uint32_t m = G.m0; m < G.m1; ++m) {
         const PdltEntry& e = H.tab[m];
         uint32_t* refs = lds + G.blk_base + e.tok_ref_off;
-        uint32_t* seen = refs + e.n_bnd;
+        uint32_t* seen = lds + G.seen_base + (m - G.m0);
         uint32_t n_emit = 0;
-        pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g.data() + (e.int_base - H.n_b), seen,
-                    &n_emit, T.data() + 3ull * e.t_off);
+        pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g.data() + (e.int_base - H.n_b),
+                    BitSeen{seen, kTokenSeenStride}, &n_emit, T.data() + 3ull * e.t_off);
         uint32_t* so = seen_g.data() + (size_t)kSeenWords * m;
-        for (int q = 0; q < 8; ++q) so[q] = seen[q];
+        for (int q = 0; q < 8; ++q) so[q] = seen[q * kTokenSeenStride];
         so[8] = n_emit;
       }
     }
@@ -104,7 +104,7 @@ int main(int argc, char** argv) {
       uint32_t* walk = refs + e.n_bnd;
       uint32_t* seen = walk + e.n_int;
       if (!two_pass) {
-        pdlt_tokens(lds, rel, e, refs, walk, seen, seen + 8, T.data() + 3ull * e.t_off);
+        pdlt_tokens(lds, rel, e, refs, walk, BitSeen{seen}, seen + 8, T.data() + 3ull * e.t_off);
         continue;
       }
       for (uint32_t i = 0; i < e.n_int; ++i) walk[i] = walk_g.at(e.int_base - H.n_b + i);
