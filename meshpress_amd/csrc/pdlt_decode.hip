@@ -147,8 +147,9 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
     uint32_t* refs = lds + G.blk_base + e.tok_ref_off;
     uint32_t* seen = lds + G.seen_base + lane;
     uint32_t n_emit = 0;
-    pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g + (e.int_base - n_b),
-                BitSeen{seen, kTokenSeenStride}, &n_emit, T + 3ull * e.t_off);
+    pdlt_tokens<WalkSlot>(lds, -32 * (int64_t)G.w_lo, e, refs,
+                          walk_g + pdlt_walk_off(e.int_base, m, n_b),
+                          BitSeen{seen, kTokenSeenStride}, &n_emit, T + 3ull * e.t_off);
     uint32_t* so = seen_g + (uint64_t)kSeenWords * m;
 #pragma unroll
     for (int q = 0; q < 8; ++q) so[q] = seen[q * kTokenSeenStride];
@@ -181,7 +182,7 @@ __global__ __launch_bounds__(64) void pdlt_chain_kernel(
   uint32_t* seen = walk + e.n_int;
   const int64_t rel = -32 * (int64_t)G.w_lo;
   if (active) {
-    const uint32_t* ws = walk_g + (e.int_base - n_b);
+    const uint32_t* ws = walk_g + pdlt_walk_off(e.int_base, m, n_b);
     for (uint32_t i = (uint32_t)role; i < e.n_int; i += 4) walk[i] = ws[i];
     const uint32_t* ss = seen_g + (uint64_t)kSeenWords * m;
     for (uint32_t i = (uint32_t)role; i < kSeenWords; i += 4) seen[i] = ss[i];
@@ -224,10 +225,10 @@ int PdltSet::build(const uint8_t* d, size_t len) {
   std::string err;
   int rc = pdlt_parse(d, len, H, err);
   if (rc) return set_error(rc, err);
-  // two-pass scratch: the walks (one word per interior vertex) and the seen
-  // sets with walk lengths
-  const uint64_t n_int = H.n_v - H.n_b;
-  PDLT_HIP(hipMalloc(&d_walk, 4 * std::max<uint64_t>(1, n_int)));
+  // two-pass scratch: the walks (one word per interior vertex and one per
+  // meshlet) and the seen sets with walk lengths
+  const uint64_t n_walk = H.n_v - H.n_b + H.n_m;
+  PDLT_HIP(hipMalloc(&d_walk, 4 * std::max<uint64_t>(1, n_walk)));
   PDLT_HIP(hipMalloc(&d_seen, 4ull * kSeenWords * std::max<uint32_t>(1, H.n_m)));
   PDLT_HIP(hipMalloc(&d_words, H.words.size() * 4));
   PDLT_HIP(hipMemcpy(d_words, H.words.data(), H.words.size() * 4, hipMemcpyHostToDevice));

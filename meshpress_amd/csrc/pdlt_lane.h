@@ -65,10 +65,16 @@ struct PdltGroup {
 };
 static_assert(sizeof(PdltGroup) == 32, "PdltGroup is 32 bytes");
 
-// Two-pass layout: the walk of meshlet m (n_int words) goes to the walk
-// buffer at int_base - n_boundary; its seen set (8 words) and walk length
+// Two-pass layout: the walk of meshlet m (n_int words and one spare word,
+// which the token lane's unconditional entry stores may reach) goes to the
+// walk buffer at pdlt_walk_off; its seen set (8 words) and walk length
 // (1 word) to the seen buffer at kSeenWords * m.
 constexpr uint32_t kSeenWords = 9;
+
+// Walk buffer words: n_int + n_meshlets.
+STRIDE_DEV uint64_t pdlt_walk_off(uint32_t int_base, uint32_t m, uint32_t n_boundary) {
+  return (uint64_t)(int_base - n_boundary) + m;
+}
 
 // Decode constants, widened to float64 as the reference decoder does.
 struct PdltParams {
@@ -308,7 +314,9 @@ struct PdltIds {
 // locals in first-use order: v * 0x01010101 for a strip-root vertex,
 // v | a << 8 | b << 16 | c << 24 for a parallelogram step, a != v) and the
 // triangles into tout (global ids).  `seen` (LDS in the kernels) is the live
-// seen set; the walk length goes to *n_emit.
+// seen set; the walk length goes to *n_emit.  Sink = WalkSlot stores every
+// token's entry (the walk needs one spare word after n_int).
+template <class Sink = WalkArray>
 STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e,
                             uint32_t* __restrict__ refs, uint32_t* __restrict__ walk,
                             BitSeen seen, uint32_t* __restrict__ n_emit,
@@ -320,7 +328,7 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
   LdsCursor cur{st, (uint32_t)((int64_t)e.strips_bit + rel)};
   TokenLane<PdltIds> t;
   t.init(cur, n_local, e.n_tris, e.n_strips, tout, PdltIds{refs, n_bnd, e.int_base - n_bnd});
-  WalkArray sink{walk};
+  Sink sink{walk};
   for (uint32_t it = 0; it < t.n_tok; ++it) t.step(cur, seen, sink);
   *n_emit = t.nw;
 }

@@ -58,7 +58,7 @@ int main(int argc, char** argv) {
   std::vector<uint32_t> T(3 * H.n_t);
   for (size_t i = 0; i < H.bnd.size(); ++i) V[i] = pdlt_out(pdlt_bnd(H.bnd[i], P), P, (int)(i % 3));
   // two-pass: the token pass first, into exactly sized global arrays
-  std::vector<uint32_t> walk_g(H.n_v - H.n_b), seen_g((size_t)kSeenWords * H.n_m);
+  std::vector<uint32_t> walk_g(H.n_v - H.n_b + H.n_m), seen_g((size_t)kSeenWords * H.n_m);
   if (two_pass) {
     std::vector<PdltGroup> tg;
     uint32_t tlds = 0;
@@ -76,8 +76,9 @@ int main(int argc, char** argv) {
         uint32_t* refs = lds + G.blk_base + e.tok_ref_off;
         uint32_t* seen = lds + G.seen_base + (m - G.m0);
         uint32_t n_emit = 0;
-        pdlt_tokens(lds, -32 * (int64_t)G.w_lo, e, refs, walk_g.data() + (e.int_base - H.n_b),
-                    BitSeen{seen, kTokenSeenStride}, &n_emit, T.data() + 3ull * e.t_off);
+        pdlt_tokens<WalkSlot>(lds, -32 * (int64_t)G.w_lo, e, refs,
+                              walk_g.data() + pdlt_walk_off(e.int_base, m, H.n_b),
+                              BitSeen{seen, kTokenSeenStride}, &n_emit, T.data() + 3ull * e.t_off);
         uint32_t* so = seen_g.data() + (size_t)kSeenWords * m;
         for (int q = 0; q < 8; ++q) so[q] = seen[q * kTokenSeenStride];
         so[8] = n_emit;
@@ -107,7 +108,7 @@ int main(int argc, char** argv) {
         pdlt_tokens(lds, rel, e, refs, walk, BitSeen{seen}, seen + 8, T.data() + 3ull * e.t_off);
         continue;
       }
-      for (uint32_t i = 0; i < e.n_int; ++i) walk[i] = walk_g.at(e.int_base - H.n_b + i);
+      for (uint32_t i = 0; i < e.n_int; ++i) walk[i] = walk_g.at(pdlt_walk_off(e.int_base, m, H.n_b) + i);
       for (uint32_t i = 0; i < kSeenWords; ++i) seen[i] = seen_g.at((size_t)kSeenWords * m + i);
       pdlt_refs(lds, rel, e, refs);
     }
