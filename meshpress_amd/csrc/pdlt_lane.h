@@ -262,7 +262,7 @@ STRIDE_DEV uint32_t pdlt_scan(const Src& src, uint32_t start, uint64_t end_all,
 // ---------------------------------------------------------------------------
 
 // Unsigned Rice(k) value from a stream: one window step for codewords of at
-// most 33 bits, bit-serial beyond (the pre-pass bounds the value below 2^32).
+// most 32 bits, bit-serial beyond (the pre-pass bounds the value below 2^32).
 template <class Stream>
 STRIDE_DEV uint32_t rice_raw(Stream& s, uint32_t k) {
   uint32_t u;
@@ -383,7 +383,7 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
   const float w0f = (float)w0, w1f = (float)w1, w2f = (float)w2, deltaf = (float)delta;
   // the codes stream through a window with the next word read ahead, so no
   // step waits on an LDS read for its bits (the staged range holds 6 words
-  // past the meshlet's end for the read-ahead); a codeword of at most 33
+  // past the meshlet's end for the read-ahead); a codeword of at most 32
   // bits decodes from the window in one step (tag 1 Rice, 2 Exp-Golomb: the
   // coders of DPR4's residuals), a longer one bit-serially
   BitStream<LdsStaged> bs;

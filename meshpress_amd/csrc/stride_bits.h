@@ -251,20 +251,23 @@ struct MtfLite {
     lo = (lo & ~ml) | (sl & ml);
     hi = (hi & ~mh) | (sh & mh);
   }
-  // rank of v among the n entries, or -1
+  // 0x80 in each zero byte of x
+  STRIDE_DEV static uint32_t zero_bytes(uint32_t x) {
+    const uint32_t low7 = 0x7F7F7F7Fu;
+    return ~(((x & low7) + low7) | x | low7);
+  }
+  // rank of v among the n entries, or -1.  Branch-free: the first byte equal
+  // to v over all 16 bytes.  Bytes [n, 16) sit above every entry, so the
+  // first match is an entry whenever any entry matches.
   STRIDE_DEV int find(uint32_t v) const {
-    const uint64_t ones = 0x0101010101010101ull, low7 = 0x7F7F7F7F7F7F7F7Full;
-    const uint64_t pat = (uint64_t)(v & 0xFFu) * ones;
-    const uint64_t t0 = lo ^ pat, t1 = hi ^ pat;
-    uint64_t z0 = ~(((t0 & low7) + low7) | t0 | low7);
-    uint64_t z1 = ~(((t1 & low7) + low7) | t1 | low7);
-    const uint64_t v0 = n >= 8 ? ~0ull : (n ? (~0ull >> (64 - 8 * n)) : 0ull);
-    const uint64_t v1 = n <= 8 ? 0ull : (n >= 16 ? ~0ull : (~0ull >> (64 - 8 * (n - 8))));
-    z0 &= v0;
-    z1 &= v1;
-    if (z0) return __builtin_ctzll(z0) >> 3;
-    if (z1) return 8 + (__builtin_ctzll(z1) >> 3);
-    return -1;
+    const uint32_t pat = (v & 0xFFu) * 0x01010101u;
+    const uint64_t z0 = (uint64_t)zero_bytes((uint32_t)lo ^ pat) |
+                        ((uint64_t)zero_bytes((uint32_t)(lo >> 32) ^ pat) << 32);
+    const uint64_t z1 = (uint64_t)zero_bytes((uint32_t)hi ^ pat) |
+                        ((uint64_t)zero_bytes((uint32_t)(hi >> 32) ^ pat) << 32);
+    uint32_t i = z1 ? 8u + ((uint32_t)__builtin_ctzll(z1) >> 3) : 16u;
+    i = z0 ? ((uint32_t)__builtin_ctzll(z0) >> 3) : i;
+    return i < n ? (int)i : -1;
   }
 };
 

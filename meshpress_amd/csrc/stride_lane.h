@@ -595,16 +595,15 @@ struct BitStream {
     next = w0 + 3;
     fill();
   }
+  // one word at most: a drop of n <= 32 bits leaves valid >= 1
   STRIDE_DEV void fill() {
-#pragma unroll
-    for (int t = 0; t < 2; ++t)
-      if (valid <= 32) {
-        win |= (uint64_t)ahead << (32 - valid);
-        valid += 32;
-        ahead = src(next++);
-      }
+    if (valid <= 32) {
+      win |= (uint64_t)ahead << (32 - valid);
+      valid += 32;
+      ahead = src(next++);
+    }
   }
-  // drops n <= 33 bits
+  // drops n <= 32 bits
   STRIDE_DEV void drop(uint32_t n) {
     win = n < 64 ? (win << n) : 0ull;
     valid -= n;
@@ -652,7 +651,7 @@ STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
   const uint32_t nb = (eg ? q : 0u) + (uint32_t)k;
   const uint32_t len = q + 1 + nb;
   uint64_t u;
-  if (len <= 33) {
+  if (len <= 32) {
     const uint64_t body = nb ? ((W << (q + 1)) >> (64 - nb)) : 0ull;
     const uint64_t off = body >> k;
     const uint64_t low = body & ((1ull << k) - 1);
@@ -672,21 +671,22 @@ STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
 
 // The short form of codeword_s on a window W (>= 33 valid bits): the
 // codeword's length with u = its zigzagged value, or 0 for a codeword longer
-// than 33 bits (decoded by codeword_s instead).
-// Within 33 bits everything fits 32-bit arithmetic: the body (nb <= 32
+// than 32 bits (decoded by codeword_s instead; a stream then refills one
+// word at most per drop).
+// Within 32 bits everything fits 32-bit arithmetic: the body (nb <= 31
 // bits) is the low nb bits of the top len bits of W, and u < 2^32 (a Rice
-// code has (q + 1) << k <= 2^32, an EG code u < 2^(q + 1 + k) <= 2^32).
+// code has (q + 1) << k <= 2^31, an EG code u < 2^(q + 1 + k) <= 2^32).
 STRIDE_DEV uint32_t codeword_win(uint64_t W, bool eg, int k, uint32_t& u) {
   const uint32_t q = W ? (uint32_t)__builtin_clzll(W) : 64u;
   const uint32_t nb = (eg ? q : 0u) + (uint32_t)k;
   const uint32_t len = q + 1 + nb;
-  const uint32_t x = (uint32_t)(W >> (64 - (len <= 33 ? len : 33u)));
+  const uint32_t x = (uint32_t)(W >> (64 - (len <= 32 ? len : 32u)));
   const uint32_t body = nb < 32 ? x & ((1u << nb) - 1u) : x;
   const uint32_t off = body >> k;
   const uint32_t low = body & ((1u << k) - 1u);
   const uint32_t shifted = eg ? ((1u << (q & 31u)) - 1u) + off : q;
   u = (shifted << k) | low;
-  return len <= 33 ? len : 0u;
+  return len <= 32 ? len : 0u;
 }
 STRIDE_DEV int64_t unzigzag32(uint32_t u) {
   return (int64_t)(int32_t)((u >> 1) ^ (0u - (u & 1u)));

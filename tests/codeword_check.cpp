@@ -1,14 +1,19 @@
 // Host build of the axis lanes' codeword fast path: codeword_win (32-bit,
-// codewords of at most 33 bits) and irlp24 against the general stream
+// codewords of at most 32 bits) and irlp24 against the general stream
 // decoder codeword_s and irlp64 (meshpress_amd/csrc/stride_lane.h), over
 // every unary length q, coder parameter k and coder, with random bodies and
-// trailing bits, and over the edge windows (all zeros, all ones); and the
+// trailing bits, and over the edge windows (all zeros, all ones); the longer
+// codewords (codeword_s's general path) against the bit-serial BitReader
+// decode of the same words (meshpress_amd/csrc/stride_bits.h); and the
 // dequantisation quotient quot_mx against the float64 division for every
-// g_bits <= 16 and every 16-bit code.
+// g_bits <= 16 and every 16-bit code; and the token lanes' reuse FIFO
+// (MtfLite: rank, to_front, find) against a plain list over random hit /
+// raw sequences, raw ids present or not, ids 0 and 255 included.
 //
 //   codeword_check      exit 0 = all equal, 1 = a mismatch (printed)
 #include <cstdio>
 #include <cstdlib>
+#include <deque>
 #include <vector>
 
 #include "../meshpress_amd/csrc/stride_lane.h"
@@ -24,7 +29,7 @@ static uint64_t next64() {
 }
 
 int main() {
-  long checked = 0, fast = 0;
+  long checked = 0, fast = 0, longer = 0;
   for (int eg = 0; eg < 2; ++eg)
     for (int k = 0; k < 32; ++k)
       for (uint32_t q = 0; q <= 40; ++q)
@@ -47,11 +52,32 @@ int main() {
           const uint32_t len = codeword_win(s.win, eg != 0, k, u);
           const uint32_t nb = (eg ? (W ? (uint32_t)__builtin_clzll(W) : 64u) : 0u) + (uint32_t)k;
           const uint32_t qq = W ? (uint32_t)__builtin_clzll(W) : 64u;
-          if (qq + 1 + nb > 33) {
+          if (qq + 1 + nb > 32) {
             if (len != 0) {
               std::printf("long codeword not rejected: eg %d k %d W %016llx len %u\n", eg, k,
                           (unsigned long long)W, len);
               return 1;
+            }
+            // the general path against the bit-serial reader (W != 0: the
+            // unary run ends inside the first 64 bits; an Exp-Golomb prefix of
+            // more than 32 zeros is a value of 2^33 or more, outside every
+            // stream, which the reader does not decode)
+            if (W != 0 && !(eg && qq > 32)) {
+              uint32_t be[8];
+              for (int i = 0; i < 8; ++i) be[i] = __builtin_bswap32(le[i]);
+              BitReader<LdsWords> br;
+              br.init(LdsWords{be, 8}, 0);
+              const uint64_t want_u = eg ? br.exp_golomb(k, 0xFFFFFFu) : br.rice(k, 0xFFFFFFu);
+              const int64_t got = codeword_s(s, eg != 0, k);
+              const uint64_t used = 64 - s.valid + 32 * (s.next - 3);
+              if (got != unzigzag(want_u) || used != br.pos) {
+                std::printf("long codeword mismatch: eg %d k %d W %016llx: (%llu, %lld) vs "
+                            "(%u, %lld)\n", eg, k, (unsigned long long)W,
+                            (unsigned long long)used, (long long)got, br.pos,
+                            (long long)unzigzag(want_u));
+                return 1;
+              }
+              ++longer;
             }
             ++checked;
             continue;
@@ -108,6 +134,50 @@ int main() {
       ++quot;
     }
   }
-  std::printf("codewords %ld (fast path %ld), irlp %ld, quotients %ld\n", checked, fast, irl, quot);
+  // reuse FIFO: front = index 0 (the most recent entry), at most 16 entries
+  long fifo_ops = 0;
+  for (int trial = 0; trial < 4000; ++trial) {
+    std::deque<uint32_t> ref;
+    MtfLite f;
+    f.clear();
+    const uint32_t n_ids = trial % 4 == 0 ? 256u : 1u + (uint32_t)(next64() % 256);
+    for (int step = 0; step < 400; ++step) {
+      const uint64_t r = next64();
+      uint32_t j, v;
+      if (!ref.empty() && r % 3 != 0) {   // hit: entry j
+        j = (uint32_t)((r >> 8) % ref.size());
+        v = ref[j];
+        if (f.rank(j) != v) {
+          std::printf("fifo rank mismatch: trial %d step %d\n", trial, step);
+          return 1;
+        }
+      } else {                            // raw id: in the FIFO or not
+        v = (r >> 2) % 4 == 0 && !ref.empty() ? ref[(r >> 16) % ref.size()]
+                                             : (uint32_t)((r >> 24) % n_ids);
+        int want = -1;
+        for (size_t i = 0; i < ref.size(); ++i)
+          if (ref[i] == v) { want = (int)i; break; }
+        const int got = f.find(v);
+        if (got != want) {
+          std::printf("fifo find mismatch: trial %d step %d v %u: %d vs %d\n", trial, step, v,
+                      got, want);
+          return 1;
+        }
+        j = want >= 0 ? (uint32_t)want : f.n;
+        if (j == f.n && f.n < 16) f.n++;
+      }
+      f.to_front(j, v);
+      if (j < ref.size()) ref.erase(ref.begin() + j);
+      else if (ref.size() == 16) ref.pop_back();
+      ref.push_front(v);
+      if (f.n != ref.size()) {
+        std::printf("fifo size mismatch: trial %d step %d\n", trial, step);
+        return 1;
+      }
+      ++fifo_ops;
+    }
+  }
+  std::printf("codewords %ld (fast path %ld, longer %ld), irlp %ld, quotients %ld, fifo ops %ld\n",
+              checked, fast, longer, irl, quot, fifo_ops);
   return 0;
 }
