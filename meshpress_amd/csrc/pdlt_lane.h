@@ -266,7 +266,7 @@ STRIDE_DEV uint32_t pdlt_scan(const Src& src, uint32_t start, uint64_t end_all,
 template <class Stream>
 STRIDE_DEV uint32_t rice_raw(Stream& s, uint32_t k) {
   uint32_t u;
-  const uint32_t len = codeword_win(s.win, false, (int)k, u);
+  const uint32_t len = codeword_win(s.win, 0u, (int)k, u);
   if (len) {
     s.drop(len);
     return u;
@@ -396,7 +396,7 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
       bs.drop(bw);
     } else {
       uint32_t u;
-      const uint32_t len = codeword_win(bs.win, eg, (int)k, u);
+      const uint32_t len = codeword_win(bs.win, eg ? ~0u : 0u, (int)k, u);
       if (len) {
         bs.drop(len);
         code = (int32_t)((u >> 1) ^ (0u - (u & 1u)));

@@ -673,18 +673,24 @@ STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
 // codeword's length with u = its zigzagged value, or 0 for a codeword longer
 // than 32 bits (decoded by codeword_s instead; a stream then refills one
 // word at most per drop).
-// Within 32 bits everything fits 32-bit arithmetic: the body (nb <= 31
-// bits) is the low nb bits of the top len bits of W, and u < 2^32 (a Rice
-// code has (q + 1) << k <= 2^31, an EG code u < 2^(q + 1 + k) <= 2^32).
-STRIDE_DEV uint32_t codeword_win(uint64_t W, bool eg, int k, uint32_t& u) {
-  const uint32_t q = W ? (uint32_t)__builtin_clzll(W) : 64u;
-  const uint32_t nb = (eg ? q : 0u) + (uint32_t)k;
+// Within 32 bits everything fits 32-bit arithmetic on the window's top word
+// T: q <= 31, the body (nb <= 31 bits) is the low nb bits of the top len
+// bits of T, and u < 2^32 (a Rice code has (q + 1) << k <= 2^31, an EG code
+// u < 2^(q + 1 + k) <= 2^32).  The shift counts are masked to 5 bits: they
+// are exact whenever len <= 32, and the value is dropped otherwise.
+// egm: all ones for Exp-Golomb(k), 0 for Rice(k) (a mask, not a bool: the
+// coder choice then costs an and and a bit-field insert, no compares).
+STRIDE_DEV uint32_t codeword_win(uint64_t W, uint32_t egm, int k, uint32_t& u) {
+  const uint32_t T = (uint32_t)(W >> 32);
+  const uint32_t q = T ? (uint32_t)__builtin_clz(T) : 32u;
+  const uint32_t nb = (q & egm) + (uint32_t)k;
   const uint32_t len = q + 1 + nb;
-  const uint32_t x = (uint32_t)(W >> (64 - (len <= 32 ? len : 32u)));
-  const uint32_t body = nb < 32 ? x & ((1u << nb) - 1u) : x;
+  const uint32_t x = T >> ((32u - len) & 31u);
+  const uint32_t body = x & ((1u << (nb & 31u)) - 1u);
   const uint32_t off = body >> k;
   const uint32_t low = body & ((1u << k) - 1u);
-  const uint32_t shifted = eg ? ((1u << (q & 31u)) - 1u) + off : q;
+  const uint32_t eg_val = ((1u << (q & 31u)) - 1u) + off;
+  const uint32_t shifted = (eg_val & egm) | (q & ~egm);
   u = (shifted << k) | low;
   return len <= 32 ? len : 0u;
 }
@@ -786,6 +792,9 @@ template <bool I32, class Src, class CodeT>
 struct AxisChain {
   BitStream<Src> sd, sp;
   bool deg, peg, ok;
+  // each stream's coder in one word: k | (Exp-Golomb ? 0xFFFFFFE0 : 0), so a
+  // step selects one word and takes the mask as kp >> 5 (arithmetic)
+  uint32_t dkp, pkp;
   int dk, pk;
   int32_t n0, n1, n2, K, rnd;
   int64_t anchor, prev;
@@ -803,6 +812,8 @@ struct AxisChain {
     const uint32_t TP = bits_at_src(words, tb + 16u * (uint32_t)(3 + d), 16);
     deg = (TD >> 8) != 1u;
     peg = (TP >> 8) != 1u;
+    dkp = (TD & 31u) | (deg ? 0xFFFFFFE0u : 0u);
+    pkp = (TP & 31u) | (peg ? 0xFFFFFFE0u : 0u);
     dk = (int)(TD & 31u);
     pk = (int)(TP & 31u);
     sd.init(words, base + pick3(d, e.sub_bit[0], e.sub_bit[1], e.sub_bit[2]));
@@ -824,7 +835,9 @@ struct AxisChain {
     // the code comes from the delta stream (strip roots) or the
     // parallelogram stream; only the chosen one advances
     uint32_t u;
-    const uint32_t len = codeword_win(root ? sd.win : sp.win, root ? deg : peg, root ? dk : pk, u);
+    const uint32_t kp = root ? dkp : pkp;
+    const uint32_t len = codeword_win(root ? sd.win : sp.win, (uint32_t)((int32_t)kp >> 5),
+                                      (int)(kp & 31u), u);
     int64_t r;
     if (len) {
       sd.drop(root ? len : 0u);

@@ -49,7 +49,7 @@ int main() {
           StreamBits s;
           s.init(le.data(), 0);
           uint32_t u = 0;
-          const uint32_t len = codeword_win(s.win, eg != 0, k, u);
+          const uint32_t len = codeword_win(s.win, eg ? ~0u : 0u, k, u);
           const uint32_t nb = (eg ? (W ? (uint32_t)__builtin_clzll(W) : 64u) : 0u) + (uint32_t)k;
           const uint32_t qq = W ? (uint32_t)__builtin_clzll(W) : 64u;
           if (qq + 1 + nb > 32) {
