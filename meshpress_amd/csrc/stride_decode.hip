@@ -72,7 +72,7 @@ __device__ __forceinline__ uint32_t conn_phase(const Src& src, uint32_t bit, uin
   Seen256 seen;
   seen.clear();
   uint32_t walk = 0, tc = 0;
-  const uint32_t kRoot = 0xFFFFFF00u;  // a = b = c = 255 never occurs for a strip step
+  // strip roots enter the walk as v * 0x01010101 (walk_root)
   auto token = [&]() -> uint32_t {
     uint32_t flag = r.bits(1);
     uint32_t v;
@@ -89,9 +89,9 @@ __device__ __forceinline__ uint32_t conn_phase(const Src& src, uint32_t bit, uin
     uint32_t p0 = token(), p1 = token(), p2 = token();
     L.tris[tc & (kMaxTris - 1)] = p0 | (p1 << 8) | (p2 << 16);
     ++tc;
-    if (!seen.test_set(p0)) L.walk[walk++ & (kMaxLocal - 1)] = kRoot | p0;
-    if (!seen.test_set(p1)) L.walk[walk++ & (kMaxLocal - 1)] = kRoot | p1;
-    if (!seen.test_set(p2)) L.walk[walk++ & (kMaxLocal - 1)] = kRoot | p2;
+    if (!seen.test_set(p0)) L.walk[walk++ & (kMaxLocal - 1)] = p0 * 0x01010101u;
+    if (!seen.test_set(p1)) L.walk[walk++ & (kMaxLocal - 1)] = p1 * 0x01010101u;
+    if (!seen.test_set(p2)) L.walk[walk++ & (kMaxLocal - 1)] = p2 * 0x01010101u;
     for (uint32_t t = 1; t < len; ++t) {
       uint32_t ec = r.bits(1);
       uint32_t nv = token();
@@ -154,7 +154,7 @@ __device__ __forceinline__ bool chain_phase(const MeshletEntry& e, const MeshPar
     uint32_t pk = L.walk[i];
     uint32_t v = pk & 0xFFu;
     int32_t cx, cy, cz;
-    if ((pk >> 8) == 0xFFFFFFu) {
+    if (walk_root(pk)) {
       if (first) {
         cx = ax; cy = ay; cz = az;
         first = false;

@@ -247,11 +247,13 @@ STRIDE_DEV bool decode_lane(const uint32_t* __restrict__ st, uint32_t n_words,
 // code arrays: int32 (3 n_local words) or uint16 (split_code_words(n)
 // words, each axis array padded to an even count).
 //
-// Walk entry i: kRootMark | v for a strip-root vertex (anchor or delta),
-// v | a << 8 | b << 16 | c << 24 for a parallelogram vertex (the pre-pass
-// rejects references (255, 255, 255), so the two forms never collide).
+// Walk entry i: v * 0x01010101 for a strip-root vertex (anchor or delta),
+// v | a << 8 | b << 16 | c << 24 for a parallelogram vertex.  A walk entry
+// is a first sighting of v while a is a vertex of the previous triangle,
+// seen before, so a != v and the byte test of walk_root never confuses the
+// two forms.  A root's prediction reads codes[v] (in range, discarded).
 // ---------------------------------------------------------------------------
-constexpr uint32_t kRootMark = 0xFFFFFF00u;
+STRIDE_DEV bool walk_root(uint32_t w) { return ((w >> 8) & 0xFFu) == (w & 0xFFu); }
 
 // 32-bit words of one meshlet's split block (walk + three code arrays).
 STRIDE_DEV uint32_t split_block_words(uint32_t n_local, bool narrow) {
@@ -318,12 +320,12 @@ struct LdsCursor {            // staged words in LDS
 // Token lane: strip tokens -> triangles (tout) and the walk.  `cur` starts
 // at the first strip length.
 // Output ids of a DPR4 meshlet's locals (v_off + local) and its walk's root
-// entries (kRootMark | local).
+// entries (local * 0x01010101).
 struct MeshletIds {
   static constexpr bool kCarry = false;   // ids recomputed per token (an add)
   uint32_t v_off;
   STRIDE_DEV uint32_t operator()(uint32_t x) const { return v_off + x; }
-  STRIDE_DEV static uint32_t root(uint32_t v) { return kRootMark | v; }
+  STRIDE_DEV static uint32_t root(uint32_t v) { return v * 0x01010101u; }
 };
 
 // The token lane as a stepper: one strip token per step().  A first
@@ -521,13 +523,12 @@ STRIDE_DEV bool split_axis(const uint32_t* __restrict__ st, uint32_t n_words, ui
   ok &= code_fits<CodeT>(anchor);
   for (uint32_t i = 1; i < n_local; ++i) {
     const uint32_t w = walk[i];
-    const bool root = (w & 0xFFFFFF00u) == kRootMark;
+    const bool root = walk_root(w);
     uint32_t pos = root ? pd : pp;
     const int64_t r = codeword(st, n_words, pos, root ? deg : peg, root ? dk : pk, true);
     pd = root ? pos : pd;
     pp = root ? pp : pos;
-    const uint32_t ia = root ? 0u : (w >> 8) & 0xFFu, ib = root ? 0u : (w >> 16) & 0xFFu,
-                   ic = root ? 0u : w >> 24;
+    const uint32_t ia = (w >> 8) & 0xFFu, ib = (w >> 16) & 0xFFu, ic = w >> 24;
     const int64_t pred = irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1,
                                 n2, K);
     const int64_t c = (root ? prev : pred) + r;
@@ -798,6 +799,8 @@ struct AxisChain {
   int dk, pk;
   int32_t n0, n1, n2, K, rnd;
   int64_t anchor, prev;
+  int32_t prev32;   // the 32-bit chain's previous code
+  uint32_t bad;     // the 32-bit chain's range flags (bits 16..31)
 
   STRIDE_DEV void init(const Src& words, uint64_t base, const MeshletEntry& e, const MeshParams& P,
                        int d) {
@@ -825,13 +828,15 @@ struct AxisChain {
     rnd = (I32 && K > 0) ? (1 << (K - 1)) : 0;
     ok = true;
     prev = anchor;
+    prev32 = (int32_t)anchor;
+    bad = 0;
   }
   STRIDE_DEV void first(uint32_t w0, CodeT* __restrict__ codes) {
     codes[w0 & 0xFFu] = (CodeT)anchor;
     ok &= code_fits<CodeT>(anchor);
   }
   STRIDE_DEV void step(uint32_t w, CodeT* __restrict__ codes) {
-    const bool root = (w & 0xFFFFFF00u) == kRootMark;
+    const bool root = walk_root(w);
     // the code comes from the delta stream (strip roots) or the
     // parallelogram stream; only the chosen one advances
     uint32_t u;
@@ -845,21 +850,37 @@ struct AxisChain {
       r = unzigzag32(u);
     } else {
       r = root ? codeword_s(sd, deg, dk) : codeword_s(sp, peg, pk);
+      if constexpr (I32 && sizeof(CodeT) == 2) {
+        if (r != (int64_t)(int32_t)r) bad |= 0x80000000u;   // no 16-bit code then
+      }
     }
-    const uint32_t ia = root ? 0u : (w >> 8) & 0xFFu, ib = root ? 0u : (w >> 16) & 0xFFu,
-                   ic = root ? 0u : w >> 24;
-    int64_t pred;
+    const uint32_t ia = (w >> 8) & 0xFFu, ib = (w >> 16) & 0xFFu, ic = w >> 24;
     if constexpr (I32 && sizeof(CodeT) == 2) {
-      pred = (int64_t)irlp24((int32_t)codes[ia], (int32_t)codes[ib], (int32_t)codes[ic], n0, n1, n2,
-                             K, rnd);
+      // 32-bit chain.  The stored codes are uint16 and irlp32_exact holds,
+      // so pred and prev lie in int32; c = base + r wraps only on a signed
+      // overflow, which sets the sign bit of (base ^ c) & (r ^ c), and a
+      // code outside [0, 65535] has c's top 16 bits set.  bad collects both
+      // (and a general-path residual outside int32); the chain is exact
+      // while bad's top 16 bits stay clear.
+      const int32_t pred = irlp24((int32_t)codes[ia], (int32_t)codes[ib], (int32_t)codes[ic], n0,
+                                  n1, n2, K, rnd);
+      const uint32_t r32 = (uint32_t)r;
+      const uint32_t base = (uint32_t)(root ? prev32 : pred);
+      const uint32_t c = base + r32;
+      bad |= c | ((base ^ c) & (r32 ^ c) & 0x80000000u);
+      codes[w & 0xFFu] = (CodeT)c;
+      prev32 = (int32_t)c;
     } else {
-      pred = irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1, n2, K);
+      const int64_t pred =
+          irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1, n2, K);
+      const int64_t c = (root ? prev : pred) + r;
+      ok &= code_fits<CodeT>(c);
+      codes[w & 0xFFu] = (CodeT)c;
+      prev = c;
     }
-    const int64_t c = (root ? prev : pred) + r;
-    ok &= code_fits<CodeT>(c);
-    codes[w & 0xFFu] = (CodeT)c;
-    prev = c;
   }
+  // every code of the chain lies in CodeT
+  STRIDE_DEV bool good() const { return ok && (bad >> 16) == 0u; }
 };
 
 // The two-pass axis lane: the chain along the meshlet's walk slot in HBM
@@ -884,7 +905,7 @@ STRIDE_DEV bool split_axis_g(const Src& words, uint64_t base,
     }
     wq = nq;
   }
-  return ax.ok;
+  return ax.good();
 }
 
 // Index pre-pass for one meshlet (one lane): parses the whole meshlet once,
@@ -943,15 +964,10 @@ STRIDE_DEV void scan_meshlet(const Src& src, uint32_t start, uint32_t end, const
       uint32_t ec = r.bits(1);
       uint32_t v;
       if (!token(v)) break;
-      uint32_t a = ec ? p[0] : p[1], b = p[2], c = ec ? p[1] : p[0];
       if (ec == 0) p[0] = p[1];
       p[1] = p[2];
       p[2] = v;
-      if (!seen.test_set(v)) {
-        walk++;
-        // the decode kernel marks strip roots with refs (255, 255, 255)
-        if (a == 255 && b == 255 && c == 255) err = kScanErrWalk;
-      }
+      if (!seen.test_set(v)) walk++;
       if (over()) err = kScanErrOverrun;
     }
     if (over()) err = kScanErrOverrun;
