@@ -227,6 +227,16 @@ struct Mtf {
 // The newest-first reuse FIFO as two 64-bit halves with cheap masks: the
 // move-to-front of rank j keeps the bytes above j and shifts bytes 0..j up
 // by one, new = (old & ~M) | ((old << 8) & M) | v with M = bytes [0, j].
+// v (< 256) in every byte: one byte permute on the device (a multiply by
+// 0x01010101 compiles to a quarter-rate v_mul_lo_u32).
+STRIDE_DEV uint32_t byte4(uint32_t v) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __builtin_amdgcn_perm(0u, v, 0u);
+#else
+  return (v & 0xFFu) * 0x01010101u;
+#endif
+}
+
 struct MtfLite {
   uint64_t lo, hi;
   uint32_t n;
@@ -260,7 +270,7 @@ struct MtfLite {
   // to v over all 16 bytes.  Bytes [n, 16) sit above every entry, so the
   // first match is an entry whenever any entry matches.
   STRIDE_DEV int find(uint32_t v) const {
-    const uint32_t pat = (v & 0xFFu) * 0x01010101u;
+    const uint32_t pat = byte4(v);
     const uint64_t z0 = (uint64_t)zero_bytes((uint32_t)lo ^ pat) |
                         ((uint64_t)zero_bytes((uint32_t)(lo >> 32) ^ pat) << 32);
     const uint64_t z1 = (uint64_t)zero_bytes((uint32_t)hi ^ pat) |

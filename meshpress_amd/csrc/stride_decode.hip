@@ -89,9 +89,9 @@ __device__ __forceinline__ uint32_t conn_phase(const Src& src, uint32_t bit, uin
     uint32_t p0 = token(), p1 = token(), p2 = token();
     L.tris[tc & (kMaxTris - 1)] = p0 | (p1 << 8) | (p2 << 16);
     ++tc;
-    if (!seen.test_set(p0)) L.walk[walk++ & (kMaxLocal - 1)] = p0 * 0x01010101u;
-    if (!seen.test_set(p1)) L.walk[walk++ & (kMaxLocal - 1)] = p1 * 0x01010101u;
-    if (!seen.test_set(p2)) L.walk[walk++ & (kMaxLocal - 1)] = p2 * 0x01010101u;
+    if (!seen.test_set(p0)) L.walk[walk++ & (kMaxLocal - 1)] = byte4(p0);
+    if (!seen.test_set(p1)) L.walk[walk++ & (kMaxLocal - 1)] = byte4(p1);
+    if (!seen.test_set(p2)) L.walk[walk++ & (kMaxLocal - 1)] = byte4(p2);
     for (uint32_t t = 1; t < len; ++t) {
       uint32_t ec = r.bits(1);
       uint32_t nv = token();

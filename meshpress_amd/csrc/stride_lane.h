@@ -255,6 +255,7 @@ STRIDE_DEV bool decode_lane(const uint32_t* __restrict__ st, uint32_t n_words,
 // ---------------------------------------------------------------------------
 STRIDE_DEV bool walk_root(uint32_t w) { return ((w >> 8) & 0xFFu) == (w & 0xFFu); }
 
+
 // 32-bit words of one meshlet's split block (walk + three code arrays).
 STRIDE_DEV uint32_t split_block_words(uint32_t n_local, bool narrow) {
   return narrow ? n_local + 3 * ((n_local + 1) / 2) : 4 * n_local;
@@ -325,7 +326,7 @@ struct MeshletIds {
   static constexpr bool kCarry = false;   // ids recomputed per token (an add)
   uint32_t v_off;
   STRIDE_DEV uint32_t operator()(uint32_t x) const { return v_off + x; }
-  STRIDE_DEV static uint32_t root(uint32_t v) { return v * 0x01010101u; }
+  STRIDE_DEV static uint32_t root(uint32_t v) { return byte4(v); }
 };
 
 // The token lane as a stepper: one strip token per step().  A first
