@@ -329,7 +329,7 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
   TokenLane<PdltIds> t;
   t.init(cur, n_local, e.n_tris, e.n_strips, tout, PdltIds{refs, n_bnd, e.int_base - n_bnd});
   Sink sink{walk};
-  for (uint32_t it = 0; it < t.n_tok; ++it) t.step(cur, seen, sink);
+  tokens_run(t, cur, seen, sink);
   *n_emit = t.nw;
 }
 

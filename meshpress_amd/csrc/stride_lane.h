@@ -373,7 +373,9 @@ struct TokenLane {
   }
   STRIDE_DEV bool done() const { return it >= n_tok; }
 
-  template <class Cursor, class Seen, class Sink>
+  // kFind = false leaves out the FIFO search of raw ids, for waves none of
+  // whose meshlets sets kFlagRawInFifo (the loops choose with wave_any)
+  template <bool kFind = true, class Cursor, class Seen, class Sink>
   STRIDE_DEV void step(Cursor& cur, Seen& seen, Sink& sink) {
     uint32_t W = Wn;
     const bool start = root_j == 3 && steps_left == 0;
@@ -395,9 +397,11 @@ struct TokenLane {
     const bool hit = flag == 0;
     uint32_t j = hit ? ((fifo.n - 1 - val) & 15u) : fifo.n;
     const uint32_t v = hit ? fifo.rank(j) : (val & 0xFFu);
-    if (raw_in_fifo && !hit) {
-      const int pf = fifo.find(v);
-      j = pf >= 0 ? (uint32_t)pf : j;
+    if constexpr (kFind) {
+      if (raw_in_fifo && !hit) {
+        const int pf = fifo.find(v);
+        j = pf >= 0 ? (uint32_t)pf : j;
+      }
     }
     // triangle (a, b, v): its output ids are looked up as soon as v is known
     // (for PDLT v5 a boundary local's id is an LDS read, so a and b carry
@@ -448,6 +452,26 @@ struct TokenLane {
   }
 };
 
+// True when p holds on any active lane of the wave (on the host: this lane).
+STRIDE_DEV bool wave_any(bool p) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __any(p ? 1 : 0) != 0;
+#else
+  return p;
+#endif
+}
+
+// All steps of a token lane; the FIFO search is compiled in only when some
+// lane of the wave needs it (kFlagRawInFifo).
+template <class Lane, class Cursor, class Seen, class Sink>
+STRIDE_DEV void tokens_run(Lane& t, Cursor& cur, Seen& seen, Sink& sink) {
+  if (wave_any(t.raw_in_fifo)) {
+    for (uint32_t it = 0; it < t.n_tok; ++it) t.template step<true>(cur, seen, sink);
+  } else {
+    for (uint32_t it = 0; it < t.n_tok; ++it) t.template step<false>(cur, seen, sink);
+  }
+}
+
 // Walk sinks: the whole walk in one array.  WalkArray stores first
 // sightings only (the split kernel's LDS block has no room past n_local);
 // WalkSlot stores every token's entry at the next index, which needs one
@@ -469,7 +493,7 @@ STRIDE_DEV void split_tokens_c(Cursor& cur, const MeshletEntry& e, uint32_t* __r
   TokenLane<MeshletIds> t;
   t.init(cur, e.n_local, e.n_tris, e.n_strips, tout, MeshletIds{e.v_off});
   Sink sink{walk};
-  for (uint32_t it = 0; it < t.n_tok; ++it) t.step(cur, seen, sink);
+  tokens_run(t, cur, seen, sink);
 }
 
 template <class Sink = WalkArray, class Seen>
