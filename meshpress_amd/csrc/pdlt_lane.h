@@ -307,7 +307,6 @@ struct PdltIds {
     const uint32_t r = refs[b ? x : 0u];
     return b ? r : ibase + x;
   }
-  STRIDE_DEV static uint32_t root(uint32_t v) { return byte4(v); }
 };
 
 // Token lane: boundary refs into refs[], strips into the walk (interior

@@ -320,18 +320,16 @@ struct LdsCursor {            // staged words in LDS
 
 // Token lane: strip tokens -> triangles (tout) and the walk.  `cur` starts
 // at the first strip length.
-// Output ids of a DPR4 meshlet's locals (v_off + local) and its walk's root
-// entries (local * 0x01010101).
+// Output ids of a DPR4 meshlet's locals (v_off + local).
 struct MeshletIds {
   static constexpr bool kCarry = false;   // ids recomputed per token (an add)
   uint32_t v_off;
   STRIDE_DEV uint32_t operator()(uint32_t x) const { return v_off + x; }
-  STRIDE_DEV static uint32_t root(uint32_t v) { return byte4(v); }
 };
 
 // The token lane as a stepper: one strip token per step().  A first
 // sighting hands its walk entry (local id + parallelogram refs, first-seen
-// order; Ids::root(v) for strip roots) to sink(index, entry); the triangles
+// order; v * 0x01010101 for strip roots, walk_root) to sink(index, entry); the triangles
 // go to `o` as ids(local).  The split, two-pass and PDLT token kernels run
 // steps back to back (split_tokens_c, pdlt_tokens).
 template <class Ids>
@@ -423,7 +421,11 @@ struct TokenLane {
     // first sighting -> walk entry; parallelogram refs (a, b, c):
     // ec 0 -> (q1, q2, q0), ec 1 -> (q0, q2, q1)
     const bool fresh = seen.fresh(v);
-    const uint32_t entry = is_root ? Ids::root(v) : (v | (ra << 8) | (q2 << 16) | (rc << 24));
+    // the entry's three ref bytes are v for a root (v * 0x01010101),
+    // selected byte by byte: a select of two whole entries compiles to a
+    // branch
+    const uint32_t ea = is_root ? v : ra, eb = is_root ? v : q2, ec2 = is_root ? v : rc;
+    const uint32_t entry = v | (ea << 8) | (eb << 16) | (ec2 << 24);
     if constexpr (Sink::kEveryToken) {
       sink(nw, entry);   // a stale entry at nw is overwritten by the next first sighting
     } else {
