@@ -8,7 +8,9 @@
 // dequantisation quotient quot_mx against the float64 division for every
 // g_bits <= 16 and every 16-bit code; and the token lanes' reuse FIFO
 // (MtfLite: rank, to_front, find) against a plain list over random hit /
-// raw sequences, raw ids present or not, ids 0 and 255 included.
+// raw sequences, raw ids present or not, ids 0 and 255 included; and the
+// walk-entry root test: walk_root holds for every byte4(v) and for no
+// parallelogram entry (a != v), refs (255, 255, 255) and (0, 0, 0) included.
 //
 //   codeword_check      exit 0 = all equal, 1 = a mismatch (printed)
 #include <cstdio>
@@ -177,7 +179,32 @@ int main() {
       ++fifo_ops;
     }
   }
-  std::printf("codewords %ld (fast path %ld, longer %ld), irlp %ld, quotients %ld, fifo ops %ld\n",
-              checked, fast, longer, irl, quot, fifo_ops);
+  // walk entries: roots v * 0x01010101, parallelograms v | a << 8 | b << 16 | c << 24
+  long entries = 0;
+  for (uint32_t v = 0; v < 256; ++v) {
+    if (!walk_root(byte4(v)) || byte4(v) != v * 0x01010101u) {
+      std::printf("root entry wrong: v %u\n", v);
+      return 1;
+    }
+    for (uint32_t a = 0; a < 256; ++a) {
+      if (a == v) continue;
+      for (int rep = 0; rep < 16; ++rep) {
+        const uint64_t r = next64();
+        uint32_t b = (uint32_t)(r & 0xFFu), c = (uint32_t)((r >> 8) & 0xFFu);
+        if (rep == 0) b = c = 255;
+        if (rep == 1) b = c = 0;
+        if (rep == 2) b = c = v;
+        const uint32_t w = v | (a << 8) | (b << 16) | (c << 24);
+        if (walk_root(w)) {
+          std::printf("parallelogram entry taken for a root: %08x\n", w);
+          return 1;
+        }
+        ++entries;
+      }
+    }
+  }
+  std::printf("codewords %ld (fast path %ld, longer %ld), irlp %ld, quotients %ld, fifo ops %ld, "
+              "walk entries %ld\n",
+              checked, fast, longer, irl, quot, fifo_ops, entries);
   return 0;
 }
