@@ -920,18 +920,28 @@ STRIDE_DEV bool split_axis_g(const Src& words, uint64_t base,
   AxisChain<I32, Src, CodeT> ax;
   ax.init(words, base, e, P, d);
   // the walk slot is 16-byte aligned, with readable words past its end
+  // (quad (n_local >> 2) + 1 at most is read).  Entries 1..3, then whole
+  // quads with no per-entry guard (each guard costs a compare and three
+  // exec-mask instructions), then the last partial quad.
   Walk4 wq = walk[0];
   ax.first(wq.x, codes);
-  for (uint32_t i0 = 0; i0 < n_local; i0 += 4) {
-    const Walk4 nq = walk[(i0 >> 2) + 1];
-    const uint32_t wb[4] = {wq.x, wq.y, wq.z, wq.w};
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const uint32_t i = i0 + (uint32_t)j;
-      if (i >= 1 && i < n_local) ax.step(wb[j], codes);
-    }
+  Walk4 nq = walk[1];
+  if (1 < n_local) ax.step(wq.y, codes);
+  if (2 < n_local) ax.step(wq.z, codes);
+  if (3 < n_local) ax.step(wq.w, codes);
+  wq = nq;
+  uint32_t i0 = 4;
+  for (; i0 + 4 <= n_local; i0 += 4) {
+    nq = walk[(i0 >> 2) + 1];
+    ax.step(wq.x, codes);
+    ax.step(wq.y, codes);
+    ax.step(wq.z, codes);
+    ax.step(wq.w, codes);
     wq = nq;
   }
+  if (i0 < n_local) ax.step(wq.x, codes);
+  if (i0 + 1 < n_local) ax.step(wq.y, codes);
+  if (i0 + 2 < n_local) ax.step(wq.z, codes);
   return ax.good();
 }
 
