@@ -328,10 +328,12 @@ struct MeshletIds {
 };
 
 // The token lane as a stepper: one strip token per step().  A first
-// sighting hands its walk entry (local id + parallelogram refs, first-seen
-// order; v * 0x01010101 for strip roots, walk_root) to sink(index, entry); the triangles
-// go to `o` as ids(local).  The split, two-pass and PDLT token kernels run
-// steps back to back (split_tokens_c, pdlt_tokens).
+// sighting's walk entry (local id + parallelogram refs, first-seen order;
+// v * 0x01010101 for strip roots, walk_root) goes to sink(index, entry); a
+// sink with kEveryToken takes every token's entry at the next index, which
+// the next first sighting overwrites.  The triangles go to `o` as
+// ids(local).  The split, two-pass and PDLT token kernels run steps back to
+// back (split_tokens_c, pdlt_tokens).
 template <class Ids>
 struct TokenLane {
   uint32_t n_tok, it;      // tokens in the meshlet, tokens done
