@@ -245,9 +245,12 @@ struct SplitDequant {
         z = dequant((int32_t)cb[2 * cs + v], P.g_min[2], P.g_range[2], P.g_bits[2], P.scale,
                     P.center[2]);
       }
-      vo[3 * v] = x;
-      vo[3 * v + 1] = y;
-      vo[3 * v + 2] = z;
+      // one pointer for the vertex: the three stores merge into one
+      // 12-byte store (32-bit index sums 3v, 3v + 1, 3v + 2 do not)
+      float* p = vo + 3ull * v;
+      p[0] = x;
+      p[1] = y;
+      p[2] = z;
     }
   }
 };
