@@ -45,14 +45,6 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "decode Gtris/s + HBM GB/s (batched vs per-mesh); learned-encode speedup @≤1% bpv"
 HBM_PEAK_GBS = 8000.0
-TUNE_CANDIDATES = ([dict(mode=2, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
-                    for gs in (8, 16) for lds in (16384, 24576, 32768, 40960, 49152)]
-                   + [dict(mode=2, group_size=16, lds_bytes=24576, grid=g, use_queue=1)
-                      for g in (1536, 3072)]
-                   + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, grid=0, use_queue=0)
-                      for gs, lds, tg in ((16, 16384, 32), (16, 16384, 64), (8, 16384, 32))]
-                   + [dict(mode=1, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
-                      for gs in (32, 64) for lds in (16384, 40960)])
 
 
 def log(*a):
@@ -96,8 +88,8 @@ def time_calls(fn, steps, warmup):
 
 
 def autotune(dec, mode_kw=None):
-    from meshpress_amd.autotune import tune
-    best, res = tune(dec, steps=10, cands=TUNE_CANDIDATES, mode_kw=mode_kw)
+    from meshpress_amd.autotune import BENCH_CANDIDATES, tune
+    best, res = tune(dec, steps=10, cands=BENCH_CANDIDATES, mode_kw=mode_kw)
     return best, [[c["mode"], c["group_size"], c["lds_bytes"], c.get("tok_group", 0), c["grid"],
                    c["use_queue"], round(ms, 4)] for c, ms in res]
 

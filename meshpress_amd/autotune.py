@@ -47,6 +47,19 @@ def candidates(include_wave_mode: bool = True, grids=(0,)):
     return out
 
 
+# The compact search bench.py and the full-size tests run: split groups over
+# LDS budgets, the split kernel's persistent queue grids, the two-pass shapes
+# and lane-per-meshlet groups.
+BENCH_CANDIDATES = ([dict(mode=2, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
+                     for gs in (8, 16) for lds in (16384, 24576, 32768, 40960, 49152)]
+                    + [dict(mode=2, group_size=16, lds_bytes=24576, grid=g, use_queue=1)
+                       for g in (1536, 3072)]
+                    + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, grid=0, use_queue=0)
+                       for gs, lds, tg in ((16, 16384, 32), (16, 16384, 64), (8, 16384, 32))]
+                    + [dict(mode=1, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
+                       for gs in (32, 64) for lds in (16384, 40960)])
+
+
 def time_once(dec, steps: int, mode_kw=None) -> float:
     """Average ms per decode over `steps` launches (HIP events)."""
     import torch

@@ -108,6 +108,20 @@ class NativePrep:
             self._L.stride_encoder_destroy(h)
             self._h = None
 
+    def local_to_global(self) -> np.ndarray:
+        """Global vertex id of every decoded vertex (meshlet-major local
+        order, the order of the decoder's output), int64 (n_local_total,)."""
+        nm = self.n_meshlets
+        bufs = [np.zeros(nm, np.int32) for _ in range(4)]
+        l2g = np.zeros(self.n_local_total, np.int64)
+        tl = np.zeros((self.n_tris_total, 3), np.uint8)
+        slen = np.zeros(self.n_strips_total, np.int32)
+        stris = np.zeros(self.n_tris_total, np.int32)
+        walk = np.zeros((self.n_local_total, 5), np.uint8)
+        _lib.check(self._L.stride_encoder_export_plans(
+            self._h, *[_lib.ptr(a) for a in (*bufs, l2g, tl, slen, stris, walk)]))
+        return l2g
+
     def export_plans(self):
         """Reference-shaped plans: list of dicts with local_to_global,
         ml_tris_local, strips, n_bnd, n_int, n_tris_m, n_strips."""

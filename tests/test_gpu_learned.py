@@ -1,9 +1,11 @@
-"""The learned encoder proposal with the packaged weights, on held-out
-meshes (seeds the network never saw): bit-rate regret against the
+"""The learned encoder proposal with the packaged weights, on the 50
+held-out meshes (seeds the network never saw): bit-rate regret against the
 exhaustive search within 1%, every stream decoding to exactly what the
 exhaustive stream decodes to, and the gain of the trained network over
 the method without it (the closed-form coder choice with the canonical
 predictor, and the same network untrained)."""
+import sys
+
 import numpy as np
 import pytest
 
@@ -12,20 +14,21 @@ from tests.conftest import gpu_available
 pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
 
-# held-out seeds (training used seeds below 1,000,003 + 500)
-HELD_OUT = [("torus", 36594, 2000008), ("uv_sphere", 35727, 2000015),
-            ("heightfield", 45223, 2000019), ("heightfield", 30581, 2000016),
-            ("uv_sphere", 27467, 2000013)]
+
+
+def held_out():
+    """All 50 held-out meshes (seeds 2,000,006+; training used 1,000,003 +
+    i for i < 500), 20K-500K triangles."""
+    from meshpress_amd import workloads
+    return workloads.learned_specs(50, 1)
 
 
 def _preps():
     from meshpress_amd import workloads
     from meshpress_amd.encoder.prep import prepare
-    out = []
-    for fam, n, seed in HELD_OUT:
+    for fam, n, seed in held_out():
         V, T = workloads.mesh_with_tris(fam, n, seed)
-        out.append(prepare(V, T, precision_error=1.0 / 4094.5, precision_mode="bbox_frac"))
-    return out
+        yield prepare(V, T, precision_error=1.0 / 4094.5, precision_mode="bbox_frac")
 
 
 @pytest.fixture(scope="module")
@@ -43,7 +46,8 @@ def encodes():
                                    proposal=LearnedProposal(mlp=ProposalMLP(seed=0)), **kw),
     }
     res = {k: [] for k in encs}
-    for prep in _preps():
+    for i, prep in enumerate(_preps()):
+        print(f"[learned] held-out mesh {i}: {prep['n_t']:,} tris", file=sys.stderr, flush=True)
         for k, e in encs.items():
             res[k].append((e.encode_prepared(prep), dict(e.last_stats)))
     return res
