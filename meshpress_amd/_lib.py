@@ -30,7 +30,7 @@ P = C.POINTER
 class StrideTune(C.Structure):
     _fields_ = [("waves", i32), ("stage_words", i32), ("grid", i32), ("use_queue", i32),
                 ("mode", i32), ("group_size", i32), ("lds_bytes", i32), ("reserved", i32),
-                ("tok_group", i32)]
+                ("tok_group", i32), ("tok_chains", i32)]
 
     def as_dict(self):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}

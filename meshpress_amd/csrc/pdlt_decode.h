@@ -34,6 +34,9 @@ struct PdltSet {
   double* d_bnd = nullptr;              // boundary table, normalised f64 (n_b x 3)
   float* d_v_own = nullptr;
   uint32_t* d_t_own = nullptr;
+  // false while no launch layout is complete on the device (a failed
+  // set_layout leaves it false and decode() refuses to launch)
+  bool ready = false;
 
   ~PdltSet();
   // Parses, validates and uploads.  STRIDE_OK or a STRIDE_E_* status with

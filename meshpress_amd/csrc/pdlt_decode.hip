@@ -249,6 +249,29 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   int rc = pdlt_layout(tab, gs, budget, gr, lds, err, chain32 ? 3u : 6u);
   if (!rc) rc = pdlt_layout_tokens(tab, tok_budget, tok_group, tg, tlds, err);
   if (rc) return set_error(rc, err);
+  // the new device tables are built aside and take over only once every
+  // allocation and copy has succeeded; until then the set is not ready
+  ready = false;
+  PdltGroup* ng = nullptr;
+  PdltGroup* nt = nullptr;
+  hipError_t e = hipSuccess;
+  if (!gr.empty()) {
+    e = hipMalloc(&ng, gr.size() * sizeof(PdltGroup));
+    if (e == hipSuccess)
+      e = hipMemcpy(ng, gr.data(), gr.size() * sizeof(PdltGroup), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !tg.empty()) {
+    e = hipMalloc(&nt, tg.size() * sizeof(PdltGroup));
+    if (e == hipSuccess)
+      e = hipMemcpy(nt, tg.data(), tg.size() * sizeof(PdltGroup), hipMemcpyHostToDevice);
+  }
+  if (e == hipSuccess && !tab.empty())
+    e = hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(PdltEntry), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(ng);
+    (void)hipFree(nt);
+    return set_error(STRIDE_E_HIP, std::string("PDLT launch layout: ") + hipGetErrorString(e));
+  }
   H.tab.swap(tab);
   groups.swap(gr);
   tok_groups.swap(tg);
@@ -258,20 +281,10 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   lds_budget = budget;
   tok_lds_budget = tok_budget;
   (void)hipFree(d_groups);
-  d_groups = nullptr;
   (void)hipFree(d_tok_groups);
-  d_tok_groups = nullptr;
-  if (!H.tab.empty())
-    PDLT_HIP(hipMemcpy(d_tab, H.tab.data(), H.tab.size() * sizeof(PdltEntry), hipMemcpyHostToDevice));
-  if (!groups.empty()) {
-    PDLT_HIP(hipMalloc(&d_groups, groups.size() * sizeof(PdltGroup)));
-    PDLT_HIP(hipMemcpy(d_groups, groups.data(), groups.size() * sizeof(PdltGroup), hipMemcpyHostToDevice));
-  }
-  if (!tok_groups.empty()) {
-    PDLT_HIP(hipMalloc(&d_tok_groups, tok_groups.size() * sizeof(PdltGroup)));
-    PDLT_HIP(hipMemcpy(d_tok_groups, tok_groups.data(), tok_groups.size() * sizeof(PdltGroup),
-                       hipMemcpyHostToDevice));
-  }
+  d_groups = ng;
+  d_tok_groups = nt;
+  ready = true;
   return STRIDE_OK;
 }
 
@@ -283,6 +296,8 @@ int PdltSet::own_outputs() {
 
 int PdltSet::decode(float* V, uint32_t* T, int world, hipStream_t s) {
   if ((!V && H.n_v) || (!T && H.n_t)) return set_error(STRIDE_E_ARG, "null output pointer");
+  if (!ready)
+    return set_error(STRIDE_E_HIP, "the decoder has no launch layout (the last set_tune failed)");
   PdltParams p = H.params;
   p.world = world ? 1 : 0;
   p.chain32 = chain32;

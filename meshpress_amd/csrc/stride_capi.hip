@@ -153,8 +153,12 @@ struct DecodeSet {
   uint32_t group_lds = 0;                 // LDS bytes of the group launches (largest group)
   std::vector<uint32_t> tg_base;          // first token group of each mesh, n_meshes + 1
   uint32_t* d_walk = nullptr;
+  ChainEntry* d_chains = nullptr;         // token chains (tok_chains > 0)
+  uint32_t n_chains = 0;
   uint32_t n_legacy = 0;
   std::vector<uint32_t> l_base;           // first legacy entry of each mesh
+  bool ready = false;                     // the launch layout is complete on the device
+  bool narrow_codes = true;               // narrow(), fixed by layout()
   // captured graphs keyed by (verts, tris, tune)
   std::map<std::tuple<void*, void*, int, int, int, int, int, int, int>, hipGraphExec_t> graphs;
   hipStream_t cap_stream = nullptr;
@@ -172,6 +176,7 @@ struct DecodeSet {
     (void)hipFree(d_legacy);
     (void)hipFree(d_tgroups);
     (void)hipFree(d_walk);
+    (void)hipFree(d_chains);
   }
 
   void drop_graphs() {
@@ -305,13 +310,23 @@ struct DecodeSet {
       HIP_TRY(hipMemcpy(d_tab, tab.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&d_v_own, 12 * std::max<uint64_t>(n_v, 1)));
     HIP_TRY(hipMalloc(&d_t_own, 12 * std::max<uint64_t>(n_t, 1)));
+    // the default token pass for the set's size: with fewer meshlets than
+    // 16 per SIMD (1,024 SIMDs), 16 short chains per meshlet and 4 meshlets
+    // per wave keep more SIMDs busy; otherwise 4 chains and 16 meshlets
+    cfg.tok_chains = n_m < kFewMeshlets ? 16 : 4;
+    cfg.tok_group = n_m < kFewMeshlets ? 4 : 16;
     return layout();
   }
+  static constexpr uint32_t kFewMeshlets = 16384;
 
   // Partition the meshlets into groups for the current tune and run one
   // validation decode: groups whose codes leave the packed range move to
   // the wave-per-meshlet list, which must itself fit 32-bit codes.
   int layout() {
+    // not ready until every table of the new layout is on the device: a
+    // failure on the way leaves the set refusing to launch
+    ready = false;
+    narrow_codes = narrow();
     drop_graphs();
     (void)hipFree(d_groups);
     (void)hipFree(d_legacy);
@@ -327,7 +342,7 @@ struct DecodeSet {
         const uint64_t budget = (uint64_t)cfg.lds_bytes;
         const bool split = cfg.mode != 1;
         const bool two = cfg.mode == 4;
-        const bool nar = narrow();
+        const bool nar = narrow_codes;
         const uint32_t gmax = split ? std::min(cfg.group_size, 16) : cfg.group_size;
         // bytes of LDS: staged words + per-meshlet code storage ("codes"
         // counts vertices in mode 1, block words in mode 2, code words and
@@ -446,6 +461,7 @@ struct DecodeSet {
       if (flag)
         return fail(STRIDE_E_UNSUPPORTED, "decoded codes exceed 32 bits (unsupported by the GPU decoder)");
     }
+    ready = true;
     return STRIDE_OK;
   }
 
@@ -485,6 +501,13 @@ struct DecodeSet {
     // kTokenSeenStride
     for (const GroupEntry& G : tg)
       tl = std::max(tl, 4 * (G.n_words + 7 * kTokenSeenStride + (G.m1 - G.m0)));
+    (void)hipFree(d_chains);
+    d_chains = nullptr;
+    n_chains = 0;
+    if (cfg.tok_chains > 0) {
+      int rc = chain_layout(groups, tg, tl);
+      if (rc) return rc;
+    }
     (void)hipFree(d_tgroups);
     d_tgroups = nullptr;
     n_tgroups = (uint32_t)tg.size();
@@ -496,6 +519,84 @@ struct DecodeSet {
     // walks at 16-byte aligned slots (walk_slot): n_v + 4 words per meshlet,
     // + 8 for the 4-entry reads past the last one
     if (!d_walk) HIP_TRY(hipMalloc(&d_walk, 4 * (n_v + 4ull * n_m + 8)));
+    return STRIDE_OK;
+  }
+
+  // Chain token waves: consecutive meshlets of one mesh (those the split
+  // groups decode) cut into token chains (chain_tokens with k0 =
+  // tok_chains), up to 64 chains and tok_group meshlets per wave and
+  // kChainLds bytes of LDS (staged words, 16-bit token records, 16 bytes of
+  // final FIFO per chain, 256 first-sighting words).  Replaces tg and tl;
+  // the chain-start scan kernel fills d_chains.
+  static constexpr uint64_t kChainLds = 49152;
+  int chain_layout(const std::vector<GroupEntry>& groups, std::vector<GroupEntry>& tg,
+                   uint32_t& tl) {
+    const int n_meshes = (int)mp.size();
+    const uint32_t k0 = (uint32_t)cfg.tok_chains;
+    std::vector<uint32_t> cbase(n_m, 0xFFFFFFFFu);
+    tg.clear();
+    tl = 0;
+    uint32_t nc = 0;
+    for (int i = 0; i < n_meshes; ++i) {
+      tg_base[i] = (uint32_t)tg.size();
+      const uint64_t bb = mp[i].blob_byte * 8ull;
+      GroupEntry cur{};
+      uint64_t rec_words = 0;
+      bool open = false;
+      auto need = [](uint64_t words, uint64_t rw) { return 4 * (words + rw) + 16 * 64 + 4 * 256; };
+      auto close = [&]() {
+        if (open) {
+          tg.push_back(cur);
+          tl = std::max<uint32_t>(tl, (uint32_t)need(cur.n_words, rec_words));
+        }
+        open = false;
+      };
+      for (uint32_t gi = g_base[i]; gi < g_base[i + 1]; ++gi) {
+        for (uint32_t j = groups[gi].m0; j < groups[gi].m1; ++j) {
+          const MeshletEntry& e = tab[j];
+          const uint32_t nt = meshlet_tokens(e);
+          const uint32_t cnt = chain_count(nt, chain_tokens(nt, e.n_local, e.n_strips, k0));
+          const uint64_t rw = ((nt + 1) & ~1u) / 2;
+          const uint64_t lo = (bb + e.start_bit) >> 5;
+          const uint64_t hi = ((bb + e.end_bit + 31) >> 5) + 2;
+          if (open && j == cur.m1 && j - cur.m0 < (uint32_t)cfg.tok_group &&
+              cur.pad[1] - cur.pad[0] + cnt <= 64 &&
+              need(hi - cur.w_lo, rec_words + rw) <= kChainLds) {
+            cur.m1 = j + 1;
+            cur.n_words = (uint32_t)(hi - cur.w_lo);
+            cur.pad[1] += cnt;
+            rec_words += rw;
+          } else {
+            close();
+            if (need(hi - lo, rw) > kChainLds)
+              return fail(STRIDE_E_UNSUPPORTED, "meshlet too large for the chain token pass");
+            cur = GroupEntry{};
+            cur.m0 = j;
+            cur.m1 = j + 1;
+            cur.w_lo = (uint32_t)lo;
+            cur.n_words = (uint32_t)(hi - lo);
+            cur.pad[0] = nc;
+            cur.pad[1] = nc + cnt;
+            rec_words = rw;
+            open = true;
+          }
+          cbase[j] = nc;
+          nc += cnt;
+        }
+      }
+      close();
+    }
+    tg_base[n_meshes] = (uint32_t)tg.size();
+    n_chains = nc;
+    if (nc == 0) return STRIDE_OK;
+    uint32_t* d_cb = nullptr;
+    HIP_TRY(hipMalloc(&d_chains, sizeof(ChainEntry) * nc));
+    HIP_TRY(hipMalloc(&d_cb, 4ull * n_m));
+    hipError_t e = hipMemcpy(d_cb, cbase.data(), 4ull * n_m, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_chain_scan(d_buf, d_tab, d_mesh, d_cb, n_m, k0, d_chains, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    (void)hipFree(d_cb);
+    HIP_TRY(e);
     return STRIDE_OK;
   }
 
@@ -513,18 +614,31 @@ struct DecodeSet {
     g.lds_bytes = group_lds;   // the largest group's need (<= the lds_bytes budget)
     g.grid = (uint32_t)cfg.grid;
     g.split = cfg.mode == 2 || cfg.mode == 4;
-    g.narrow = narrow();
+    g.narrow = narrow_codes;
     g.queue = (cfg.mode == 2 && cfg.use_queue && cfg.grid > 0) ? d_queue : nullptr;
     const bool two = cfg.mode == 4;
     g.tgroups = two ? d_tgroups + tg0 : nullptr;
     g.n_tgroups = two ? tg1 - tg0 : 0;
     g.tlds_bytes = tlds_bytes;
     g.walk = two ? d_walk : nullptr;
+    g.chains = two ? d_chains : nullptr;
+    g.k0 = (uint32_t)cfg.tok_chains;
     if (two) g.grid = 0;
     return g;
   }
 
-  // every mesh has at most 16 bits per axis: the split kernel keeps uint16 codes
+  // side tables a decode reads: meshlet and mesh tables, the groups of both
+  // passes, the token chains and the wave-per-meshlet list
+  uint64_t side_bytes() const {
+    const bool two = cfg.mode == 4;
+    return (uint64_t)n_m * sizeof(MeshletEntry) + (uint64_t)mp.size() * sizeof(MeshParams) +
+           (uint64_t)n_groups * sizeof(GroupEntry) +
+           (two ? (uint64_t)n_tgroups * sizeof(GroupEntry) + (uint64_t)n_chains * sizeof(ChainEntry)
+                : 0ull) +
+           4ull * n_legacy;
+  }
+
+  // every mesh has at most 16 bits per axis: the split kernels keep uint16 codes
   bool narrow() const {
     for (const MeshParams& p : mp)
       for (int d = 0; d < 3; ++d)
@@ -553,6 +667,8 @@ struct DecodeSet {
 
   // Enqueue the decode of meshes [mesh0, mesh1).
   int enqueue(float* V, uint32_t* T, hipStream_t s, int mesh0, int mesh1) {
+    if (!ready)
+      return fail(STRIDE_E_HIP, "the decoder has no launch layout (the last set_tune failed)");
     if (cfg.mode == 1 || cfg.mode == 2 || cfg.mode == 4) {
       const uint32_t g0 = g_base[mesh0], g1 = g_base[mesh1];
       if (g1 > g0) {
@@ -605,7 +721,7 @@ struct DecodeSet {
     if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
     auto key = std::make_tuple((void*)V, (void*)T, cfg.mode, cfg.waves, cfg.stage_words,
                                cfg.grid, cfg.use_queue, cfg.group_size,
-                               cfg.lds_bytes * 128 + cfg.tok_group);
+                               (cfg.lds_bytes * 128 + cfg.tok_group) * 32 + cfg.tok_chains);
     auto it = graphs.find(key);
     if (it == graphs.end()) {
       if (!cap_stream) HIP_TRY(hipStreamCreateWithFlags(&cap_stream, hipStreamNonBlocking));
@@ -643,9 +759,14 @@ struct DecodeSet {
                   "use_queue needs a persistent grid");
     const int tok_group = t->tok_group ? t->tok_group : DecodeConfig{}.tok_group;
     if (tok_group < 1 || tok_group > 64) return fail(STRIDE_E_ARG, "tok_group must be in [1, 64]");
+    const int tok_chains = t->tok_chains < 0 ? cfg.tok_chains : t->tok_chains;
+    if (tok_chains > kMaxChains)
+      return fail(STRIDE_E_ARG, "tok_chains must be in [0, 16] (or -1 to keep)");
     const bool relayout = t->mode != cfg.mode || t->group_size != cfg.group_size ||
-                          t->lds_bytes != cfg.lds_bytes || tok_group != cfg.tok_group;
+                          t->lds_bytes != cfg.lds_bytes || tok_group != cfg.tok_group ||
+                          tok_chains != cfg.tok_chains;
     cfg.tok_group = tok_group;
+    cfg.tok_chains = tok_chains;
     cfg.waves = t->waves;
     cfg.stage_words = t->stage_words;
     cfg.grid = t->grid;
@@ -666,6 +787,7 @@ struct DecodeSet {
     t->lds_bytes = cfg.lds_bytes;
     t->reserved = (int32_t)n_legacy;
     t->tok_group = cfg.tok_group;
+    t->tok_chains = cfg.tok_chains;
   }
 };
 
@@ -901,9 +1023,7 @@ int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes, uint64
     return STRIDE_OK;
   }
   if (stream_bytes) *stream_bytes = h->set.stream_bytes;
-  if (side_bytes)
-    *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) + sizeof(MeshParams) +
-                  (uint64_t)h->set.n_groups * sizeof(GroupEntry) + 4ull * h->set.n_legacy;
+  if (side_bytes) *side_bytes = h->set.side_bytes();
   return STRIDE_OK;
 }
 
@@ -1005,10 +1125,7 @@ int stride_batch_offsets(const stride_batch* h, uint64_t* v_base, uint64_t* t_ba
 int stride_batch_bytes(const stride_batch* h, uint64_t* stream_bytes, uint64_t* side_bytes) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
   if (stream_bytes) *stream_bytes = h->set.stream_bytes;
-  if (side_bytes)
-    *side_bytes = (uint64_t)h->set.n_m * sizeof(MeshletEntry) +
-                  (uint64_t)h->set.hdr.size() * sizeof(MeshParams) +
-                  (uint64_t)h->set.n_groups * sizeof(GroupEntry) + 4ull * h->set.n_legacy;
+  if (side_bytes) *side_bytes = h->set.side_bytes();
   return STRIDE_OK;
 }
 int stride_batch_set_tune(stride_batch* h, const stride_tune* t) {

@@ -21,6 +21,8 @@ struct DecodeConfig {
   int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; modes 2, 4: 1..16
   int lds_bytes = 16384;  // modes 1, 2, 4: LDS per group (stage words + codes / walk blocks)
   int tok_group = 32;     // mode 4: meshlets per token-pass wave, 1..64
+  int tok_chains = 0;     // mode 4: token chains per meshlet (0 = one token lane per
+                          //     meshlet, else 1..16: the chain token pass)
 };
 
 struct DecodeLaunch {
@@ -56,12 +58,20 @@ struct GroupLaunch {
   uint32_t n_tgroups;
   uint32_t tlds_bytes;
   uint32_t* walk;         // one word per local vertex (mesh v_base + v_off)
+  // token chains: the token groups are chain waves (GroupEntry pad[0..1] =
+  // their chain range), k0 the chains per meshlet they were cut for; null =
+  // one token lane per meshlet
+  const ChainEntry* chains;
+  uint32_t k0;
 };
 
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream);
 hipError_t launch_scan(const uint8_t* buf, const MeshletEntry* seeds, const MeshParams* meshes,
                        uint32_t n_m, MeshletScan* out, hipStream_t stream);
 hipError_t launch_decode(const DecodeLaunch& d, const DecodeConfig& c, hipStream_t stream);
+hipError_t launch_chain_scan(const uint8_t* buf, const MeshletEntry* tab, const MeshParams* meshes,
+                             const uint32_t* cbase, uint32_t n_m, uint32_t k0, ChainEntry* out,
+                             hipStream_t stream);
 size_t decode_lds_bytes(const DecodeConfig& c);
 
 }  // namespace stride

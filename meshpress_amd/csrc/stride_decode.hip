@@ -536,6 +536,203 @@ __global__ __launch_bounds__(64) void stride_tokens_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Chain token pass (mode 4 with token chains).  The chain-start scan runs
+// when the launch layout is built: one lane per meshlet re-parses its tokens
+// and records the parser state at every chain cut (chain_starts).  A decode
+// then runs one wave per token wave of consecutive meshlets of one mesh (up
+// to 64 chains), whose bits are staged in LDS:
+//   phase 1: one lane per chain writes a 16-bit record per token (ChainLane);
+//   phase 2: the whole wave takes the meshlets one after another: it
+//     resolves the FIFO placeholders, finds first sightings (an LDS atomic
+//     min of the token index per local id), and forms every token's
+//     triangle and walk entry from the records (chain_entry); prefix counts
+//     over the wave (ballot + mbcnt) place the triangles, stored as one
+//     contiguous run per 64 tokens, and the walk entries.
+// LDS: staged words | records (each meshlet's padded to an even count) |
+// the final FIFO of each chain (16 bytes) | first sightings (256 words).
+// ---------------------------------------------------------------------------
+__global__ void stride_chain_scan_kernel(const uint8_t* __restrict__ buf,
+                                         const MeshletEntry* __restrict__ tab,
+                                         const MeshParams* __restrict__ meshes,
+                                         const uint32_t* __restrict__ cbase, uint32_t n_m,
+                                         uint32_t k0, ChainEntry* __restrict__ out) {
+  const uint32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= n_m) return;
+  const uint32_t cb = cbase[m];
+  if (cb == 0xFFFFFFFFu) return;
+  const MeshletEntry e = tab[m];
+  const MeshParams& P = meshes[e.mesh];
+  GlobalWords src{reinterpret_cast<const uint32_t*>(buf + P.blob_byte), (e.end_bit + 31u) / 32u + 2u};
+  chain_starts(src, e.start_bit, e, m, k0, out + cb);
+}
+
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Cross-lane moves by DPP (GFX9 data-parallel primitives: no LDS traffic).
+// update_dpp(old, src, ctrl, row_mask, bank_mask, bound_ctrl) returns src
+// from the lane that ctrl names, or `old` where there is none.
+//   0x111..0x118 row_shr:1..8, 0x142 row_bcast:15, 0x143 row_bcast:31,
+//   0x138 wave_shr:1 (lane i takes lane i - 1, lane 0 takes `old`).
+__device__ __forceinline__ int wave_shr1(int x, int lane0) {
+  return __builtin_amdgcn_update_dpp(lane0, x, 0x138, 0xF, 0xF, false);
+}
+// Inclusive max over lanes 0..lane, values >= -1.
+__device__ __forceinline__ int wave_incl_max(int x) {
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x111, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x112, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x114, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x118, 0xF, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xA, 0xF, false));
+  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xC, 0xF, false));
+  return x;
+}
+
+struct RecSink {
+  uint16_t* __restrict__ r;
+  __device__ __forceinline__ void operator()(uint32_t t, uint32_t rec) const { r[t] = (uint16_t)rec; }
+};
+
+template <bool kFind>
+__device__ __forceinline__ void chain_run(ChainLane<LdsCursor>& cl, LdsCursor& cur, RecSink& sink) {
+  while (!cl.done()) cl.template step<kFind>(cur, sink);
+}
+
+__global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
+    const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
+    const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
+    const ChainEntry* __restrict__ chains, uint32_t k0, uint32_t* __restrict__ walk_g,
+    uint32_t* __restrict__ T) {
+  extern __shared__ uint32_t lds32[];
+  const int lane = threadIdx.x;
+  const GroupEntry G = groups[blockIdx.x];
+  const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
+  const uint32_t nmesh = G.m1 - G.m0;
+  const uint32_t c0 = G.pad[0], nch = G.pad[1] - G.pad[0];
+  // meshlet lanes and chain lanes; their table reads overlap the staging
+  MeshletEntry e{};
+  if ((uint32_t)lane < nmesh) e = tab[G.m0 + (uint32_t)lane];
+  ChainEntry ce{};
+  if ((uint32_t)lane < nch) ce = chains[c0 + (uint32_t)lane];
+  const MeshParams& P = meshes[tab[G.m0].mesh];
+  stage_words(lds32, words, G.w_lo, G.n_words, lane, 64);
+  const uint32_t ntok = (uint32_t)lane < nmesh ? meshlet_tokens(e) : 0u;
+  uint32_t rtotal = 0;
+  const uint32_t roff = wave_excl_scan((ntok + 1u) & ~1u, rtotal);
+  uint16_t* recs = reinterpret_cast<uint16_t*>(lds32 + G.n_words);
+  uint32_t* fst = lds32 + G.n_words + rtotal / 2;   // 4 words per chain
+  uint32_t* first = fst + 4 * 64;
+  // the chain lane's meshlet fields, read from its meshlet lane
+  const uint32_t mi = (uint32_t)lane < nch ? ce.m - G.m0 : 0u;
+  const uint32_t c_roff = __shfl(roff, (int)mi, 64);
+  const uint32_t c_start = __shfl(e.start_bit, (int)mi, 64);
+  const uint32_t c_nl = __shfl(e.n_local, (int)mi, 64);
+  const uint32_t c_sf = __shfl(e.n_strips, (int)mi, 64);
+  __syncthreads();
+  if ((uint32_t)lane < nch) {
+    const uint64_t abs_bit = P.blob_byte * 8ull + c_start;
+    LdsCursor cur{lds32, (uint32_t)(abs_bit - 32ull * G.w_lo) + (ce.pos & 0xFFFFu)};
+    ChainLane<LdsCursor> cl;
+    cl.init(cur, ce, c_nl, c_sf);
+    RecSink sink{recs + c_roff};
+    if (wave_any(cl.raw_in_fifo)) chain_run<true>(cl, cur, sink);
+    else chain_run<false>(cl, cur, sink);
+    // the final FIFO, ranks 0..15 (newest first) as bytes
+    fst[4 * lane + 0] = (uint32_t)cl.fifo.lo;
+    fst[4 * lane + 1] = (uint32_t)(cl.fifo.lo >> 32);
+    fst[4 * lane + 2] = (uint32_t)cl.fifo.hi;
+    fst[4 * lane + 3] = (uint32_t)(cl.fifo.hi >> 32);
+  }
+  __syncthreads();
+  uint8_t* fb = reinterpret_cast<uint8_t*>(fst);
+  // first sightings: first[v] = min over the meshlet's tokens of
+  // (gen << 16 | t); gen falls from meshlet to meshlet, so a meshlet's
+  // values undercut every stale one and the array is cleared only once
+  for (uint32_t i = (uint32_t)lane; i < 256; i += 64) first[i] = 0xFFFFFFFFu;
+  // phase 2 is one wave: its LDS hand-offs need ordering, not a workgroup
+  // barrier (which would also wait for the global stores in flight)
+  wave_sync();
+  uint32_t cb = 0;   // the meshlet's first chain in the wave
+  for (uint32_t j = 0; j < nmesh; ++j) {
+    const uint32_t nl = (uint32_t)__builtin_amdgcn_readlane((int)e.n_local, (int)j);
+    const uint32_t sf = (uint32_t)__builtin_amdgcn_readlane((int)e.n_strips, (int)j);
+    const uint32_t nt = (uint32_t)__builtin_amdgcn_readlane((int)ntok, (int)j);
+    const uint32_t rb = (uint32_t)__builtin_amdgcn_readlane((int)roff, (int)j);
+    const uint32_t voff = (uint32_t)__builtin_amdgcn_readlane((int)e.v_off, (int)j);
+    const uint32_t toff = (uint32_t)__builtin_amdgcn_readlane((int)e.t_off, (int)j);
+    const uint32_t gen = (63u - j) << 16;
+    const uint32_t mc = chain_count(nt, chain_tokens(nt, nl, sf, k0));
+    uint16_t* rj = recs + rb;
+    // the true FIFO at the start of chain c + 1 replaces chain c's final
+    // FIFO: its placeholders stand for entries of the true FIFO at the
+    // start of chain c (chain 0's final FIFO holds local ids only)
+    for (uint32_t c = 1; c + 1 < mc; ++c) {
+      if (lane < 16) {
+        const uint32_t x = fb[16 * (cb + c) + lane];
+        fb[16 * (cb + c) + lane] = x >= nl ? fb[16 * (cb + c - 1) + ((x - nl) & 15u)] : (uint8_t)x;
+      }
+      wave_sync();
+    }
+    // placeholders -> local ids; first sighting of every local id
+    for (uint32_t t = lane; t < nt; t += 64) {
+      const uint32_t r = rj[t];
+      uint32_t v = r & 0xFFu;
+      if (v >= nl) {   // only in a chain c >= 1 (record bits 12..15)
+        const uint32_t ci = r >> 12;
+        v = fb[16 * (cb + (ci ? ci - 1 : 0u)) + ((v - nl) & 15u)];
+        rj[t] = (uint16_t)((r & 0xFF00u) | v);
+      }
+      atomicMin(first + (v & 0xFFu), gen | t);
+    }
+    wave_sync();
+    uint32_t* wout = walk_g + (((P.v_base + voff + 3) & ~3ull) + 4ull * (G.m0 + j));
+    uint32_t* tout = T + 3ull * (P.t_base + toff);
+    uint32_t wbase = 0, tbase = 0;
+    // carried from the chunk before: the ids of its last two tokens, and
+    // (t << 8 | v_{t-2}) of its last edge-bit-0 token t
+    int cv1 = 0, cv2 = 0, carry = -1;
+    for (uint32_t t0 = 0; t0 < nt; t0 += 64) {
+      const uint32_t t = t0 + (uint32_t)lane;
+      const bool act = t < nt;
+      const uint32_t r = act ? rj[t] : kRecRoot;
+      const uint32_t v = r & 0xFFu;
+      const bool fresh = act && first[v] == (gen | t);
+      // v_{t-1}, v_{t-2} by wave shifts; p0 = v_{s-2} of the last edge-bit-0
+      // token s before t by a max scan over (s << 8 | v_{s-2})
+      const int iv1 = wave_shr1((int)v, cv1);
+      const int iv2 = wave_shr1(iv1, cv2);
+      const int key = (act && (r & kRecEc) == 0) ? (int)((t << 8) | (uint32_t)iv2) : -1;
+      const int inc = wave_incl_max(key);
+      const int s = max(wave_shr1(inc, -1), carry);
+      cv1 = __builtin_amdgcn_readlane((int)v, 63);
+      cv2 = __builtin_amdgcn_readlane(iv1, 63);
+      carry = max(carry, __builtin_amdgcn_readlane(inc, 63));
+      const uint32_t v1 = (uint32_t)iv1 & 0xFFu;
+      const uint32_t v2 = (uint32_t)iv2 & 0xFFu;
+      const uint32_t p0 = (uint32_t)s & 0xFFu;
+      uint32_t a, b;
+      const uint32_t entry = chain_entry(r, v1, v2, p0, a, b);
+      const bool emit = act && chain_emits(r);
+      const uint64_t bf = __ballot(fresh), be = __ballot(emit);
+      const uint32_t widx = wbase + lanes_below(bf), tidx = tbase + lanes_below(be);
+      wbase += (uint32_t)__popcll(bf);
+      tbase += (uint32_t)__popcll(be);
+      if (fresh) wout[widx] = entry;
+      if (emit) {
+        uint32_t* o = tout + 3ull * tidx;
+        o[0] = voff + a;
+        o[1] = voff + b;
+        o[2] = voff + v;
+      }
+    }
+    cb += mc;
+    wave_sync();
+  }
+}
+
 template <bool NARROW>
 __global__ __launch_bounds__(64) void stride_axes_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -629,6 +826,16 @@ hipError_t launch_scan(const uint8_t* buf, const MeshletEntry* seeds, const Mesh
   return hipGetLastError();
 }
 
+hipError_t launch_chain_scan(const uint8_t* buf, const MeshletEntry* tab, const MeshParams* meshes,
+                             const uint32_t* cbase, uint32_t n_m, uint32_t k0, ChainEntry* out,
+                             hipStream_t stream) {
+  if (n_m == 0) return hipSuccess;
+  const uint32_t tpb = 64;
+  hipLaunchKernelGGL(stride_chain_scan_kernel, dim3((n_m + tpb - 1) / tpb), dim3(tpb), 0, stream,
+                     buf, tab, meshes, cbase, n_m, k0, out);
+  return hipGetLastError();
+}
+
 template <int WAVES, int SW>
 static hipError_t launch_decode_t(const DecodeLaunch& d, hipStream_t stream) {
   const uint32_t grid = d.grid ? d.grid : (d.n_m + WAVES - 1) / WAVES;
@@ -641,7 +848,12 @@ static hipError_t launch_decode_t(const DecodeLaunch& d, hipStream_t stream) {
 hipError_t launch_group(const GroupLaunch& d, hipStream_t stream) {
   if (d.n_groups == 0) return hipSuccess;
   if (d.walk) {   // mode 4: token pass, then the axis pass
-    if (d.n_tgroups) {
+    if (d.n_tgroups && d.chains) {
+      hipLaunchKernelGGL(stride_chain_tokens_kernel, dim3(d.n_tgroups), dim3(64), d.tlds_bytes,
+                         stream, d.buf, d.tab, d.meshes, d.tgroups, d.chains, d.k0, d.walk, d.T);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return e;
+    } else if (d.n_tgroups) {
       hipLaunchKernelGGL(stride_tokens_kernel, dim3(d.n_tgroups), dim3(64), d.tlds_bytes, stream,
                          d.buf, d.tab, d.meshes, d.tgroups, d.walk, d.T);
       hipError_t e = hipGetLastError();

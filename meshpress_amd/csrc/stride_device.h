@@ -78,6 +78,22 @@ struct GroupEntry {
 };
 static_assert(sizeof(GroupEntry) == 32, "GroupEntry is 32 bytes");
 
+// A token chain: a run of consecutive strip tokens of one meshlet that one
+// lane parses on its own (the chain token pass).  A meshlet's tokens are cut
+// into up to kMaxChains chains; the chain-start scan records the parser state
+// at each cut.  A chain after the first starts with placeholder FIFO entries
+// (n_local + slot), resolved once the chain before it has finished.
+struct ChainEntry {
+  uint32_t m;       // global meshlet index
+  uint32_t pos;     // bit offset from the meshlet header (low 16) | first token << 16
+  uint32_t state;   // strip token index j (9 bits) | strip tokens lp2 << 9 (10 bits) |
+                    // FIFO fill << 19 (5 bits) | chain index in the meshlet << 24 (4 bits)
+  uint32_t tok1;    // one past the chain's last token
+};
+static_assert(sizeof(ChainEntry) == 16, "ChainEntry is 16 bytes");
+constexpr int kMaxChains = 16;     // chains per meshlet (4-bit chain index)
+constexpr int kChainMaxLocal = 240;  // placeholders n_local + 0..15 must fit a byte
+
 constexpr int kMaxLocal = 256;   // vertices per meshlet (max_verts cap)
 constexpr int kMaxTris = 256;    // triangles per meshlet (max_tris cap)
 

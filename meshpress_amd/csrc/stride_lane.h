@@ -316,6 +316,19 @@ struct LdsCursor {            // staged words in LDS
   uint32_t pc;
   STRIDE_DEV uint32_t peek32() const { return win32(st, pc); }
   STRIDE_DEV void advance(uint32_t n) { pc += n; }
+  // the next 32 bits as the two words holding them: read now, combined
+  // (window()) where the bits are needed, so the reads' latency passes
+  // under the work in between
+  struct Raw {
+    uint32_t a, b, sh;
+  };
+  STRIDE_DEV Raw peek_raw() const {
+    const uint32_t w = pc >> 5;
+    return Raw{st[w], st[w + 1], pc & 31u};
+  }
+  STRIDE_DEV static uint32_t window(const Raw& r) {
+    return (r.a << r.sh) | ((r.b >> 1) >> (31u - r.sh));
+  }
 };
 
 // Token lane: strip tokens -> triangles (tout) and the walk.  `cur` starts
@@ -474,6 +487,181 @@ STRIDE_DEV void tokens_run(Lane& t, Cursor& cur, Seen& seen, Sink& sink) {
   } else {
     for (uint32_t it = 0; it < t.n_tok; ++it) t.template step<false>(cur, seen, sink);
   }
+}
+
+// ---------------------------------------------------------------------------
+// Chain token pass.  A meshlet's strip tokens are cut into chains of about
+// equal length (chain_tokens); the chain-start scan (chain_starts) records the
+// parser state at every cut.  Phase 1: one lane per chain parses its tokens
+// (strip headers, edge bits, FIFO hits and raw ids) and stores one 16-bit
+// record per token.  A chain after the meshlet's first one starts with a FIFO
+// of placeholders: rank r holds n_local + r, so a hit on an entry the chain
+// did not see itself records that placeholder, and the lane's final FIFO (its
+// newest-first ranks) tells the next chain what its placeholders stand for.
+// Phase 2 (a whole wave per meshlet) resolves the placeholders, finds first
+// sightings, and forms triangles and walk entries from the records alone:
+//
+//   body token t, edge bit ec:  triangle (a, b, v_t), walk refs (a, b, c) with
+//     b = v_{t-1},  ec 0: a = v_{t-2}, c = p0;  ec 1: a = p0, c = v_{t-2},
+//     p0 = v_{s-2}, s the last token before t with edge bit 0 (strip roots
+//     count as edge bit 0; a strip's third root precedes its body tokens).
+//   third root: triangle (v_{t-2}, v_{t-1}, v_t), the ec-0 form.
+//
+// This is decode_dup's strip state (encoder/paradelta_v5_dup.py:512-542):
+// prev_tri = (p0, p1, p2) with p2 = v_{t-1} and p1 = v_{t-2} at every body
+// token, and p0 replaced by p1 exactly on edge bit 0.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kRecEc = 1u << 8;       // record: edge bit
+constexpr uint32_t kRecRoot = 1u << 9;     // strip root token
+constexpr uint32_t kRecRoot2 = 1u << 10;   // third strip root (emits the first triangle)
+constexpr uint32_t kMinChainTokens = 16;   // no chain shorter than this is cut off
+
+// Tokens per chain of a meshlet with n_tok tokens cut into about k0 chains
+// (k0 <= kMaxChains).  Meshlets with raw ids already in the FIFO, or too
+// many locals for the placeholders, stay one chain.
+STRIDE_DEV uint32_t chain_tokens(uint32_t n_tok, uint32_t n_local, uint32_t strips_flags,
+                                 uint32_t k0) {
+  const bool splittable = k0 > 1 && n_local <= (uint32_t)kChainMaxLocal &&
+                          (strips_flags & kFlagRawInFifo) == 0;
+  if (!splittable || n_tok <= kMinChainTokens) return n_tok > 0 ? n_tok : 1u;
+  const uint32_t lm = (n_tok + k0 - 1) / k0;
+  return lm < kMinChainTokens ? kMinChainTokens : lm;
+}
+STRIDE_DEV uint32_t chain_count(uint32_t n_tok, uint32_t lm) {
+  return n_tok > 0 ? (n_tok + lm - 1) / lm : 1u;
+}
+STRIDE_DEV uint32_t meshlet_tokens(const MeshletEntry& e) {
+  return e.n_tris + 2 * (e.n_strips & kStripMask);
+}
+
+// The parser state at every chain cut of meshlet e (its header at bit
+// `start` of src), written to out[0 .. chain_count).  The stream was
+// validated by the index pre-pass.
+template <class Src>
+STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry& e, uint32_t m,
+                             uint32_t k0, ChainEntry* __restrict__ out) {
+  const uint32_t n_tok = meshlet_tokens(e);
+  const uint32_t lm = chain_tokens(n_tok, e.n_local, e.n_strips, k0);
+  BitReader<Src> r;
+  r.init(src, start + 48);
+  int ib = 1;
+  while ((1u << ib) < e.n_local + 1) ++ib;
+  Fifo fifo;
+  fifo.clear();
+  uint32_t j = 0, lp2 = 0, c = 0, next = 0;
+  for (uint32_t t = 0; t < n_tok; ++t) {
+    if (t == next) {
+      ChainEntry ce;
+      ce.m = m;
+      ce.pos = (r.pos - start) | (t << 16);
+      ce.state = j | (lp2 << 9) | (fifo.n << 19) | (c << 24);
+      next = t + lm < n_tok ? t + lm : n_tok;
+      ce.tok1 = next;
+      out[c++] = ce;
+    }
+    if (j == lp2) {
+      lp2 = r.bits(16) + 2u;
+      j = 0;
+    }
+    if (j >= 3) r.bits(1);
+    if (r.bits(1) == 0) {
+      fifo.hit(r.bits(5));
+    } else {
+      fifo.raw(r.bits(ib));
+    }
+    ++j;
+  }
+  if (n_tok == 0) {   // not produced by a valid stream; one empty chain
+    ChainEntry ce{m, 0u, 0u, 0u};
+    out[0] = ce;
+  }
+}
+
+// Phase 1: the lane of one chain, over a Cursor (LdsCursor in the kernel).
+template <class Cursor>
+struct ChainLane {
+  uint32_t t, t1;       // next token, one past the chain's last
+  uint32_t j, lp2;      // token index in the strip, tokens in the strip (length + 2)
+  uint32_t ib;
+  uint32_t tag;         // chain index in the meshlet << 12
+  typename Cursor::Raw Wn;   // the next token's window, read one token ahead
+  bool raw_in_fifo;
+  MtfLite fifo;
+
+  // `cur` is at the chain's first token: the meshlet header + (c.pos & 0xFFFF).
+  STRIDE_DEV void init(Cursor& cur, const ChainEntry& c, uint32_t n_local, uint32_t strips_flags) {
+    t = c.pos >> 16;
+    t1 = c.tok1;
+    j = c.state & 511u;
+    lp2 = (c.state >> 9) & 1023u;
+    const uint32_t n = (c.state >> 19) & 31u;
+    const uint32_t ci = (c.state >> 24) & 15u;
+    tag = ci << 12;
+    raw_in_fifo = (strips_flags & kFlagRawInFifo) != 0;
+    ib = 1;
+    while ((1u << ib) < n_local + 1) ++ib;
+    fifo.clear();
+    fifo.n = n;
+    // placeholders: rank r holds n_local + r (chains after the first only;
+    // the first chain starts with an empty FIFO)
+    uint64_t lo = 0, hi = 0;
+    for (uint32_t r = 0; r < 16; ++r) {
+      const uint64_t b = (uint64_t)((n_local + r) & 0xFFu);
+      if (r < 8) lo |= b << (8 * r);
+      else hi |= b << (8 * (r - 8));
+    }
+    fifo.lo = ci ? lo : 0ull;
+    fifo.hi = ci ? hi : 0ull;
+    Wn = cur.peek_raw();
+  }
+  STRIDE_DEV bool done() const { return t >= t1; }
+
+  template <bool kFind = true, class Sink>
+  STRIDE_DEV void step(Cursor& cur, Sink& sink) {
+    uint32_t W = Cursor::window(Wn);
+    const bool start = j == lp2;
+    lp2 = start ? (W >> 16) + 2u : lp2;
+    W = start ? (W << 16) : W;
+    j = start ? 0u : j;
+    const bool root = j < 3;
+    const uint32_t ec = root ? 0u : (W >> 31);
+    W = root ? W : (W << 1);
+    const uint32_t flag = W >> 31;
+    const uint32_t val = flag ? ((W << 1) >> (32 - ib)) : ((W >> 26) & 31u);
+    cur.advance((start ? 16u : 0u) + (root ? 1u : 2u) + (flag ? ib : 5u));
+    Wn = cur.peek_raw();
+    const bool hit = flag == 0;
+    uint32_t jf = hit ? ((fifo.n - 1 - val) & 15u) : fifo.n;
+    const uint32_t v = hit ? fifo.rank(jf) : (val & 0xFFu);
+    if constexpr (kFind) {
+      if (raw_in_fifo && !hit) {
+        const int pf = fifo.find(v);
+        jf = pf >= 0 ? (uint32_t)pf : jf;
+      }
+    }
+    fifo.n += (!hit && jf == fifo.n && fifo.n < 16) ? 1u : 0u;
+    fifo.to_front(jf, v);
+    const uint32_t rec = v | (ec << 8) | (root ? kRecRoot : 0u) | (j == 2 ? kRecRoot2 : 0u) | tag;
+    sink(t, rec);
+    ++j;
+    ++t;
+  }
+};
+
+// Phase 2 of one token from its record and the ids of tokens t-1, t-2 and
+// s-2 (see above): the triangle's first two ids (a, b) and the walk entry
+// (v * 0x01010101 for a strip root, v | a << 8 | b << 16 | c << 24 else).
+STRIDE_DEV uint32_t chain_entry(uint32_t rec, uint32_t v1, uint32_t v2, uint32_t p0, uint32_t& a,
+                                uint32_t& b) {
+  const uint32_t v = rec & 0xFFu;
+  const bool ec = (rec & kRecEc) != 0;
+  a = ec ? p0 : v2;
+  b = v1;
+  const uint32_t c = ec ? v2 : p0;
+  return (rec & kRecRoot) ? v * 0x01010101u : (v | (a << 8) | (b << 16) | (c << 24));
+}
+STRIDE_DEV bool chain_emits(uint32_t rec) {
+  return (rec & kRecRoot) == 0 || (rec & kRecRoot2) != 0;
 }
 
 // Walk sinks: the whole walk in one array.  WalkArray stores first
@@ -647,6 +835,9 @@ struct BitStream {
   // token-lane cursor interface
   STRIDE_DEV uint32_t peek32() const { return (uint32_t)(win >> 32); }
   STRIDE_DEV void advance(uint32_t n) { drop(n); }
+  using Raw = uint32_t;
+  STRIDE_DEV Raw peek_raw() const { return peek32(); }
+  STRIDE_DEV static uint32_t window(Raw r) { return r; }
 };
 
 // The stream over the little-endian device buffer.

@@ -3,13 +3,18 @@
 // the lane decoder (decode_lane) run here on the CPU, under the sanitizers
 // when built with them, and their output is compared with the oracle.
 //
-//   lane_check <stream.dpr4> <verts.f32 out> <tris.u32 out> [split|split16|twopass|twopass16]
+//   lane_check <stream.dpr4> <verts.f32 out> <tris.u32 out>
+//              [split|split16|twopass|twopass16|chains<K>|chains<K>n16]
 //
 // With "split" the split decoder runs instead: every meshlet's token lane
 // (split_tokens), then its three axis lanes (split_axis), as the kernel's
 // phases do.  "twopass" / "twopass16" run the two-pass decoder: all token
 // lanes first with register seen sets, then the axis lanes on blocks that
-// hold only the copied walk.
+// hold only the copied walk.  "chains<K>" runs the chain token pass instead
+// of the token lanes (chain-start scan with K chains per meshlet, one
+// ChainLane per chain, then the per-meshlet record resolution the kernel's
+// wave does, here one token after another), then the two-pass axis lanes
+// ("chains<K>n16": with uint16 codes).
 // Exit status: 0 ok, 2 stream rejected by the pre-pass, 3 code overflow
 // (the kernels route such groups to the wave-per-meshlet decoder), 4 the
 // 32-bit and 64-bit axis loops disagree.
@@ -89,10 +94,13 @@ int main(int argc, char** argv) {
     nv += s.n_local;
     nt += s.n_tris;
   }
-  const bool two = argc > 4 && std::strncmp(argv[4], "twopass", 7) == 0;
+  const bool chains = argc > 4 && std::strncmp(argv[4], "chains", 6) == 0;
+  const uint32_t k0 = chains ? (uint32_t)std::atoi(argv[4] + 6) : 0u;
+  const bool two = chains || (argc > 4 && std::strncmp(argv[4], "twopass", 7) == 0);
   const bool split = two || (argc > 4 && std::strncmp(argv[4], "split", 5) == 0);
   const bool narrow = argc > 4 && (std::strcmp(argv[4], "split16") == 0 ||
-                                   std::strcmp(argv[4], "twopass16") == 0);
+                                   std::strcmp(argv[4], "twopass16") == 0 ||
+                                   (chains && std::strstr(argv[4], "n16") != nullptr));
   std::vector<uint64_t> codes(nv, kUnseen);
   std::vector<uint32_t> tris(3 * nt, 0xFFFFFFFFu);
   std::vector<int32_t> codes3(3 * nv, 0);
@@ -103,7 +111,73 @@ int main(int argc, char** argv) {
   // the walk buffer of the device: 16-byte aligned slots (walk_slot)
   std::vector<Walk4> walk4(two ? (nv + 4ull * n_m + 8) / 4 + 1 : 0);
   uint32_t* walk_g = reinterpret_cast<uint32_t*>(walk4.data());
-  if (two)
+  if (chains)
+    for (uint32_t m = 0; m < n_m; ++m) {
+      const MeshletEntry& e = tab[m];
+      const uint32_t nl = e.n_local, nt = meshlet_tokens(e);
+      const uint32_t cnt = chain_count(nt, chain_tokens(nt, nl, e.n_strips, k0));
+      std::vector<ChainEntry> ch(cnt);
+      GlobalWords src{le.data(), (uint32_t)n_words};
+      chain_starts(src, e.start_bit, e, m, k0, ch.data());
+      // phase 1: every chain on its own, exactly sized record array
+      std::vector<uint16_t> rec(nt);
+      std::vector<uint8_t> fin(16 * cnt);
+      struct Sink {
+        std::vector<uint16_t>* r;
+        void operator()(uint32_t t, uint32_t x) const { r->at(t) = (uint16_t)x; }
+      } sink{&rec};
+      for (uint32_t c = 0; c < cnt; ++c) {
+        if (ch[c].m != m || ((ch[c].state >> 24) & 15u) != c) return 5;
+        StreamBits cur;
+        cur.init(le.data(), e.start_bit + (ch[c].pos & 0xFFFFu));
+        ChainLane<StreamBits> cl;
+        cl.init(cur, ch[c], nl, e.n_strips);
+        while (!cl.done()) cl.step<true>(cur, sink);
+        if (cl.t != (c + 1 < cnt ? (ch[c + 1].pos >> 16) : nt)) return 5;
+        for (int i = 0; i < 16; ++i)
+          fin[16 * c + i] = (uint8_t)((i < 8 ? cl.fifo.lo >> (8 * i) : cl.fifo.hi >> (8 * (i - 8))) & 0xFFu);
+      }
+      // phase 2: placeholders, first sightings, triangles and walk entries
+      for (uint32_t c = 1; c + 1 < cnt; ++c)
+        for (int i = 0; i < 16; ++i) {
+          const uint32_t x = fin[16 * c + i];
+          if (x >= nl) fin[16 * c + i] = fin[16 * (c - 1) + ((x - nl) & 15u)];
+        }
+      std::vector<uint32_t> first(nl, 0xFFFFFFFFu);
+      for (uint32_t t = 0; t < nt; ++t) {
+        uint32_t v = rec[t] & 0xFFu;
+        if (v >= nl) {
+          const uint32_t c = rec[t] >> 12;
+          if (c == 0) return 5;
+          v = fin[16 * (c - 1) + ((v - nl) & 15u)];
+          rec[t] = (uint16_t)((rec[t] & 0xFF00u) | v);
+        }
+        if (v >= nl) return 5;
+        first[v] = std::min(first[v], t);
+      }
+      uint32_t* wout = walk_g + walk_slot(P, e, m);
+      uint32_t* tout = tris.data() + 3ull * e.t_off;
+      int last = -1;
+      uint32_t wi = 0, ti = 0;
+      for (uint32_t t = 0; t < nt; ++t) {
+        const uint32_t r = rec[t], v = r & 0xFFu;
+        const int s = last;
+        const uint32_t v1 = rec[t >= 1 ? t - 1 : 0] & 0xFFu, v2 = rec[t >= 2 ? t - 2 : 0] & 0xFFu;
+        const uint32_t p0 = rec[s >= 2 ? (uint32_t)s - 2 : 0] & 0xFFu;
+        uint32_t a, b;
+        const uint32_t entry = chain_entry(r, v1, v2, p0, a, b);
+        if (first[v] == t) wout[wi++] = entry;
+        if (chain_emits(r)) {
+          tout[3 * ti] = e.v_off + a;
+          tout[3 * ti + 1] = e.v_off + b;
+          tout[3 * ti + 2] = e.v_off + v;
+          ++ti;
+        }
+        if ((r & kRecEc) == 0) last = (int)t;
+      }
+      if (wi != nl || ti != e.n_tris) return 5;
+    }
+  else if (two)
     for (uint32_t m = 0; m < n_m; ++m) {
       const MeshletEntry& e = tab[m];
       uint32_t sw[8];

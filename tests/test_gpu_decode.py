@@ -52,11 +52,14 @@ def stream_fixtures():
     return out
 
 
-# the default launch shape (two passes) and the split kernel
-SHAPES = [dict(), dict(mode=2, group_size=16, lds_bytes=24576)]
+# the default launch shape (two passes), the split kernel, and the two-pass
+# decode with the chain token pass (2 and 16 chains per meshlet)
+SHAPES = [dict(), dict(mode=2, group_size=16, lds_bytes=24576), dict(mode=4, tok_chains=2),
+          dict(mode=4, tok_chains=16, tok_group=8), dict(mode=4, tok_chains=0)]
+SHAPE_IDS = ["default", "split", "chains2", "chains16", "token_lanes"]
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=["default", "split"])
+@pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
 @pytest.mark.parametrize("path", stream_fixtures(), ids=lambda p: p.stem)
 def test_golden_streams(path, shape):
     if shape and path.stem.startswith("pdlt_"):
@@ -67,7 +70,7 @@ def test_golden_streams(path, shape):
     assert sha(T.astype(np.int64)) == z["T_sha"].tobytes().hex()
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=["default", "split"])
+@pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
 @pytest.mark.parametrize("pred", ["canonical", "fit"])
 def test_small_meshes(oracle, pred, shape):
     for name, V, T in small_meshes():
@@ -97,6 +100,15 @@ TUNES = [dict(mode=0, waves=w, stage_words=s) for w in (1, 2, 4) for s in (256, 
          dict(mode=4, group_size=5, lds_bytes=16384),
          dict(mode=4, group_size=1, lds_bytes=8192),
          dict(mode=4, group_size=16, lds_bytes=8192),
+         # the chain token pass
+         dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=1),
+         dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=3, tok_group=64),
+         dict(mode=4, group_size=5, lds_bytes=16384, tok_chains=7, tok_group=5),
+         dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=16, tok_group=1),
+         dict(mode=4, group_size=3, lds_bytes=8192, tok_chains=5, tok_group=7),
+         dict(mode=4, group_size=16, lds_bytes=65536, tok_chains=16, tok_group=64),
+         # one token lane per meshlet
+         dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=0),
          # too small for most meshlets: exercises the wave-per-meshlet fallback list
          dict(mode=1, group_size=32, lds_bytes=8192, stage_words=256)]
 
@@ -193,7 +205,9 @@ def test_batch_decode_matches_single(coracle):
     for tune in (dict(), dict(mode=0, waves=4, stage_words=512, grid=128, use_queue=1),
                  dict(mode=1, group_size=16, lds_bytes=16384, grid=0, use_queue=0),
                  dict(mode=2, group_size=16, lds_bytes=24576, grid=96, use_queue=1),
-                 dict(mode=4, group_size=16, lds_bytes=24576, grid=0, use_queue=0)):
+                 dict(mode=4, group_size=16, lds_bytes=24576, grid=0, use_queue=0),
+                 dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=4, tok_group=16),
+                 dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=0)):
         if tune:
             bd.set_tune(**tune)
         for mode in ("launch", "per_mesh", "graph", "graph", "graph"):
@@ -224,6 +238,9 @@ def test_icosphere_l8_full_size(coracle, bits):
     assert_same(Vg, Tg, Vo, To)
     V4, T4 = gpu_decode(data, mode=4, group_size=16, lds_bytes=24576)   # two passes
     assert_same(V4, T4, Vo, To)
+    for k in (0, 2, 8, 16):   # one token lane per meshlet, or token chains
+        V4, T4 = gpu_decode(data, mode=4, group_size=16, lds_bytes=16384, tok_chains=k)
+        assert_same(V4, T4, Vo, To)
     plans = prep["native"].export_plans()
     l2g = np.concatenate([p["local_to_global"] for p in plans])
     assert np.array_equal(codes, prep["global_codes"][l2g])
@@ -257,6 +274,8 @@ def test_wide_codes(coracle, bits):
     Vo, To = coracle.decode(data)
     for tune in (dict(mode=2, group_size=16, lds_bytes=40960), dict(mode=2, group_size=3, lds_bytes=8192),
                  dict(mode=4, group_size=16, lds_bytes=24576), dict(mode=4, group_size=3, lds_bytes=8192),
+                 dict(mode=4, group_size=16, lds_bytes=24576, tok_chains=5),
+                 dict(mode=4, group_size=16, lds_bytes=24576, tok_chains=0),
                  dict(mode=1, group_size=64, lds_bytes=49152), dict(mode=0, waves=1, stage_words=512)):
         Vg, Tg = gpu_decode(data, **tune)
         assert_same(Vg, Tg, Vo, To)

@@ -79,7 +79,7 @@ def test_heightfield_1900_codes_roundtrip(bits):
     assert np.array_equal(codes, gcodes[l2g])
 
 
-@pytest.mark.parametrize("shape", ["default", "autotuned"])
+@pytest.mark.parametrize("shape", ["default", "autotuned", "token_lanes", "chains2", "chains16"])
 @pytest.mark.parametrize("bits", [12, 16])
 def test_heightfield_1900_gpu_equals_oracle(bits, shape):
     from meshpress_amd.autotune import BENCH_CANDIDATES, tune
@@ -90,6 +90,11 @@ def test_heightfield_1900_gpu_equals_oracle(bits, shape):
     if shape == "autotuned":
         best, _ = tune(dec, steps=5, cands=BENCH_CANDIDATES)
         say(f"heightfield {bits}-bit: tuned {best}")
+    elif shape == "token_lanes":
+        dec.set_tune(mode=4, tok_chains=0, tok_group=32)
+    elif shape.startswith("chains"):
+        k = int(shape[6:])
+        dec.set_tune(mode=4, tok_chains=k, tok_group=64 // k)
     say(f"heightfield {bits}-bit {shape}: decoding with {dec.get_tune()}")
     dec.d_verts.fill_(float("nan"))
     dec.d_tris.fill_(-1)
@@ -135,7 +140,8 @@ def test_batch_2048_graph_and_per_mesh_equal_oracle(batch2048):
     default = bd.get_tune()
     best, _ = tune(bd, steps=3, cands=[c for c in BENCH_CANDIDATES if c["mode"] == 4],
                    mode_kw={"mode": "graph"})
-    for name, shape in (("default", default), ("autotuned", best)):
+    lanes = {**default, "tok_chains": 0, "tok_group": 32}
+    for name, shape in (("default", default), ("autotuned", best), ("token_lanes", lanes)):
         bd.set_tune(**shape)
         for mode in ("graph", "graph", "per_mesh"):
             say(f"batch {name} {mode}: {bd.get_tune()}")

@@ -233,7 +233,8 @@ def test_codeword_fast_path_and_irlp24(tmp_path):
 def run_lane_check(exe, data, tmp_path, split=False):
     """split: False (one lane per meshlet), True or "split" (int32 codes),
     "split16" (uint16 codes), "twopass" / "twopass16" (token pass, then the
-    axis lanes)."""
+    axis lanes), "chains<K>" / "chains<K>n16" (the chain token pass with K
+    chains per meshlet, then the axis lanes)."""
     import subprocess
     src = tmp_path / "s.dpr4"
     src.write_bytes(data)
@@ -244,13 +245,17 @@ def run_lane_check(exe, data, tmp_path, split=False):
         np.fromfile(ft, np.uint32).reshape(-1, 3) if r.returncode in (0, 3) else None
 
 
-@pytest.mark.parametrize("split", [False, "split", "split16", "twopass", "twopass16"],
-                         ids=["lane", "split", "split16", "twopass", "twopass16"])
+LANE_MODES = [False, "split", "split16", "twopass", "twopass16", "chains1", "chains2",
+              "chains3n16", "chains16"]
+
+
+@pytest.mark.parametrize("split", LANE_MODES, ids=["lane"] + LANE_MODES[1:])
 def test_lane_decoder_matches_oracle(lane_check, coracle, tmp_path, split):
     """The gfx950 kernels' pre-pass and per-lane decoders (one lane per
-    meshlet, and the split token/axis lanes), compiled for the host,
-    reproduce the oracle bit-for-bit on every golden stream, the hand-built
-    streams and freshly encoded small meshes."""
+    meshlet, the split token/axis lanes, and the chain token pass: chain
+    starts, placeholder FIFOs and the per-meshlet record resolution),
+    compiled for the host, reproduce the oracle bit-for-bit on every golden
+    stream, the hand-built streams and freshly encoded small meshes."""
     from tests.test_oracle import hand_stream_fifo, hand_stream_small
     streams = [np.load(p)["data"].tobytes() for p in sorted((ROOT / "tests" / "golden").glob("*_*.npz"))
                if "data" in np.load(p) and not p.stem.startswith("pdlt_")]
