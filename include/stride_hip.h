@@ -110,6 +110,17 @@ int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
 int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
 /* Synchronous decode into caller-owned host buffers. */
 int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris);
+/* stride_decode for callers that cannot allocate device memory (code under
+ * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * No counterpart in the reference.  DPR4 streams only; the decoder's device
+ * must be current.  Same kernels and output as stride_decode, but their
+ * scratch (the two-pass walk buffer and the work-queue counter) is the
+ * library's static 32 MiB device array instead of the decoder's own buffers,
+ * and the call allocates nothing.  A decode needing more scratch (64 words
+ * + n_v + 4 n_meshlets + 8 in mode 4, 64 otherwise) fails with
+ * STRIDE_E_UNSUPPORTED.  Every call on a GPU shares that GPU's one array:
+ * calls that overlap on one GPU (other streams, other threads) race. */
+int stride_decode_noalloc(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
 /* PDLT v5 only: normalised positions (float32 of the float64 reconstruction,
  * before scale and center), as the reference's decode() returns them
  * (utils/paradelta_v5_cuda.py:686-712); STRIDE_E_ARG for DPR4. */

@@ -63,6 +63,7 @@ SIGNATURES = {
     "stride_decoder_sizes": (C.c_int, [p, P(u64), P(u64), P(u32)]),
     "stride_decode": (C.c_int, [p, p, p, p]),
     "stride_decode_to_host": (C.c_int, [p, p, p]),
+    "stride_decode_noalloc": (C.c_int, [p, p, p, p]),
     "stride_decoder_bytes": (C.c_int, [p, P(u64), P(u64)]),
     "stride_decoder_header": (C.c_int, [p, p, p]),
     "stride_decoder_set_tune": (C.c_int, [p, P(StrideTune)]),

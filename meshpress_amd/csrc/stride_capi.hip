@@ -121,6 +121,34 @@ __global__ void fill_u32_kernel(uint32_t* p, uint32_t v) {
   if (threadIdx.x == 0 && blockIdx.x == 0) *p = v;
 }
 
+// Scratch of stride_decode_noalloc: the work-queue counter (words 0..63)
+// and the two-pass walk slots, 32 MiB in all.  Each GPU the library runs on
+// holds its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, not the device array: kernels and copies get the
+// address hipGetSymbolAddress returns for the current device (scratch_ptr).
+constexpr uint64_t kScratchWords = 8ull << 20;
+__device__ __align__(256) uint32_t scratch[kScratchWords];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_dev[kMaxDevices] = {};
+
+// The device address of `scratch` on `device`, which must be the current
+// device; looked up once per device.
+int scratch_ptr(int device, uint32_t** out) {
+  if (device < 0 || device >= kMaxDevices) return fail(STRIDE_E_ARG, "device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_dev[device]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(STRIDE_E_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_scratch_dev[device] = static_cast<uint32_t*>(p);
+  }
+  *out = g_scratch_dev[device];
+  return STRIDE_OK;
+}
+
 }  // namespace
 
 // One device-resident set of DPR4 streams with its side table.
@@ -600,8 +628,23 @@ struct DecodeSet {
     return STRIDE_OK;
   }
 
+  // the scratch a decode's kernels use: the token pass's walk buffer and the
+  // work-queue counter (the decoder's own buffers, or the library's static
+  // scratch for stride_decode_noalloc)
+  struct Scratch {
+    uint32_t* walk;
+    uint32_t* queue;
+  };
+  Scratch own_scratch() const { return Scratch{d_walk, d_queue}; }
+  // words of scratch a decode takes: 64 for the queue counter, then the
+  // walk slots of the two-pass decode (walk_slot: n_v + 4 per meshlet + 8)
+  uint64_t scratch_words() const {
+    return 64ull + (cfg.mode == 4 ? n_v + 4ull * n_m + 8ull : 0ull);
+  }
+
   GroupLaunch group_args(float* V, uint32_t* T, uint32_t g0, uint32_t g1, uint32_t* flags,
-                         uint32_t tg0 = 0, uint32_t tg1 = 0) const {
+                         uint32_t tg0 = 0, uint32_t tg1 = 0, const Scratch* sc = nullptr) const {
+    const Scratch S = sc ? *sc : own_scratch();
     GroupLaunch g;
     g.buf = d_buf;
     g.tab = d_tab;
@@ -615,12 +658,12 @@ struct DecodeSet {
     g.grid = (uint32_t)cfg.grid;
     g.split = cfg.mode == 2 || cfg.mode == 4;
     g.narrow = narrow_codes;
-    g.queue = (cfg.mode == 2 && cfg.use_queue && cfg.grid > 0) ? d_queue : nullptr;
+    g.queue = (cfg.mode == 2 && cfg.use_queue && cfg.grid > 0) ? S.queue : nullptr;
     const bool two = cfg.mode == 4;
     g.tgroups = two ? d_tgroups + tg0 : nullptr;
     g.n_tgroups = two ? tg1 - tg0 : 0;
     g.tlds_bytes = tlds_bytes;
-    g.walk = two ? d_walk : nullptr;
+    g.walk = two ? S.walk : nullptr;
     g.chains = two ? d_chains : nullptr;
     g.k0 = (uint32_t)cfg.tok_chains;
     if (two) g.grid = 0;
@@ -646,7 +689,9 @@ struct DecodeSet {
     return true;
   }
 
-  DecodeLaunch legacy_args(float* V, uint32_t* T, uint32_t i0, uint32_t i1) const {
+  DecodeLaunch legacy_args(float* V, uint32_t* T, uint32_t i0, uint32_t i1,
+                           const Scratch* sc = nullptr) const {
+    const Scratch S = sc ? *sc : own_scratch();
     DecodeLaunch d;
     d.buf = d_buf;
     d.tab = d_tab;
@@ -656,7 +701,7 @@ struct DecodeSet {
     d.n_m = i1 - i0;
     d.V = V;
     d.T = T;
-    d.queue = d_queue;
+    d.queue = S.queue;
     d.use_queue = cfg.use_queue;
     d.grid = (uint32_t)cfg.grid;
     if (cfg.mode == 0) {  // index range is a meshlet range
@@ -665,18 +710,21 @@ struct DecodeSet {
     return d;
   }
 
-  // Enqueue the decode of meshes [mesh0, mesh1).
-  int enqueue(float* V, uint32_t* T, hipStream_t s, int mesh0, int mesh1) {
+  // Enqueue the decode of meshes [mesh0, mesh1) (scratch: the decoder's own
+  // buffers unless sc is given).
+  int enqueue(float* V, uint32_t* T, hipStream_t s, int mesh0, int mesh1,
+              const Scratch* sc = nullptr) {
     if (!ready)
       return fail(STRIDE_E_HIP, "the decoder has no launch layout (the last set_tune failed)");
+    const Scratch S = sc ? *sc : own_scratch();
     if (cfg.mode == 1 || cfg.mode == 2 || cfg.mode == 4) {
       const uint32_t g0 = g_base[mesh0], g1 = g_base[mesh1];
       if (g1 > g0) {
         const bool two = cfg.mode == 4;
         GroupLaunch g = group_args(V, T, g0, g1, nullptr, two ? tg_base[mesh0] : 0,
-                                   two ? tg_base[mesh1] : 0);
+                                   two ? tg_base[mesh1] : 0, &S);
         if (g.queue) {   // a fill kernel, not a memset, so hipGraph capture stays clean
-          hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_queue, 0u);
+          hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, S.queue, 0u);
           HIP_TRY(hipGetLastError());
         }
         HIP_TRY(launch_group(g, s));
@@ -686,7 +734,7 @@ struct DecodeSet {
         DecodeConfig c = cfg;
         c.use_queue = 0;
         c.grid = 0;
-        DecodeLaunch d = legacy_args(V, T, l0, l1);
+        DecodeLaunch d = legacy_args(V, T, l0, l1, &S);
         d.use_queue = 0;
         d.grid = 0;
         HIP_TRY(launch_decode(d, c, s));
@@ -696,10 +744,10 @@ struct DecodeSet {
     const uint32_t m0 = m_base[mesh0], m1 = m_base[mesh1];
     if (m1 == m0) return STRIDE_OK;
     if (cfg.use_queue) {
-      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, d_queue, 0u);
+      hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, S.queue, 0u);
       HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(launch_decode(legacy_args(V, T, m0, m1), cfg, s));
+    HIP_TRY(launch_decode(legacy_args(V, T, m0, m1, &S), cfg, s));
     return STRIDE_OK;
   }
 
@@ -826,7 +874,8 @@ int stride_last_error(char* buf, size_t cap) {
 
 int stride_init(int device) {
   HIP_TRY(hipSetDevice(device));
-  return STRIDE_OK;
+  uint32_t* sp = nullptr;   // the device's static scratch address, looked up once
+  return scratch_ptr(device, &sp);
 }
 
 int stride_version(void) { return 1; }
@@ -956,6 +1005,31 @@ int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* str
   if (!h) return fail(STRIDE_E_ARG, "null handle");
   if (h->v5) return h->v5->decode(d_verts, d_tris, 1, (hipStream_t)stream);
   return h->set.decode(d_verts, d_tris, (hipStream_t)stream);
+}
+
+int stride_decode_noalloc(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream) {
+  if (!h) return fail(STRIDE_E_ARG, "null handle");
+  if (h->v5) return fail(STRIDE_E_ARG, "stride_decode_noalloc decodes DPR4 streams only");
+  if (!d_verts || !d_tris) return fail(STRIDE_E_ARG, "null output pointer");
+  DecodeSet& S = h->set;
+  int dev = -1;
+  HIP_TRY(hipGetDevice(&dev));
+  if (dev != S.device)
+    return fail(STRIDE_E_ARG, "the decoder's device is not the current device");
+  const uint64_t need = S.scratch_words();
+  if (need > kScratchWords) {
+    char b[200];
+    std::snprintf(b, sizeof b,
+                  "stride_decode_noalloc: this decode needs %llu words of scratch, the static "
+                  "scratch holds %llu (32 MiB)",
+                  (unsigned long long)need, (unsigned long long)kScratchWords);
+    return fail(STRIDE_E_UNSUPPORTED, b);
+  }
+  uint32_t* sp = nullptr;
+  int rc = scratch_ptr(dev, &sp);
+  if (rc) return rc;
+  const DecodeSet::Scratch sc{sp + 64, sp};
+  return S.enqueue(d_verts, d_tris, (hipStream_t)stream, 0, (int)S.mp.size(), &sc);
 }
 
 int stride_decode_normalized(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream) {
