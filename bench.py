@@ -90,8 +90,8 @@ def time_calls(fn, steps, warmup):
 def autotune(dec, mode_kw=None):
     from meshpress_amd.autotune import BENCH_CANDIDATES, tune
     best, res = tune(dec, steps=10, cands=BENCH_CANDIDATES, mode_kw=mode_kw)
-    return best, [[c["mode"], c["group_size"], c["lds_bytes"], c.get("tok_group", 0), c["grid"],
-                   c["use_queue"], round(ms, 4)] for c, ms in res]
+    return best, [[c["mode"], c["group_size"], c["lds_bytes"], c.get("tok_group", 0),
+                   c.get("tok_chains", 0), c["grid"], c["use_queue"], round(ms, 4)] for c, ms in res]
 
 
 def cpu_baseline(data, n_t, gpu_out, budget_s=10.0, n_threads=1):
@@ -294,7 +294,8 @@ def batched(n_meshes, steps, warmup, predictor):
     # the per-mesh loop a caller of the single-mesh surface writes
     decs = [ParaDeltaV5GpuDecoder(b) for b in blobs]
     for d in decs:
-        d.set_tune(**{k: bd.get_tune()[k] for k in ("mode", "group_size", "lds_bytes")})
+        d.set_tune(**{k: bd.get_tune()[k] for k in ("mode", "group_size", "lds_bytes", "tok_group",
+                                                   "tok_chains")})
 
     def loop():
         for d in decs:

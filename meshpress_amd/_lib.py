@@ -119,6 +119,14 @@ def lib():
                 raise RuntimeError(
                     f"{LIB_PATH} is missing: build it with `python -m meshpress_amd.build`; "
                     "the STRIDE codec has no CPU fallback")
+            # torch carries its own HIP runtime: load it first, so that the
+            # library binds to the runtime torch's tensors and streams live
+            # in (loaded the other way round, the library's first device call
+            # finds no device)
+            try:
+                import torch  # noqa: F401
+            except ImportError:
+                pass
             L = C.CDLL(str(LIB_PATH))
             for name, (res, args) in SIGNATURES.items():
                 fn = getattr(L, name)

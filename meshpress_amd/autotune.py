@@ -6,8 +6,9 @@ launch stream and keeps the fastest.  The search space:
 
   mode 2 (split groups, 4 lanes per meshlet): group_size x lds_bytes, and
                                      persistent grids with a device work queue
-  mode 4 (two passes: token lanes over tok_group meshlets per wave, then the split
-         groups' axis lanes):       group_size x lds_bytes x tok_group
+  mode 4 (two passes: token lanes or token chains over tok_group meshlets per wave,
+         then the split groups' axis lanes): group_size x lds_bytes x tok_group x
+                                    tok_chains
   mode 1 (lane-per-meshlet groups):  group_size x lds_bytes x grid
   mode 0 (wave per meshlet):         waves x stage_words (x persistent queue)
 
@@ -36,9 +37,15 @@ def candidates(include_wave_mode: bool = True, grids=(0,)):
     # persistent grids pulling groups from the device work queue
     for g in QUEUE_GRIDS:
         out.append(dict(mode=2, group_size=16, lds_bytes=24576, grid=g, use_queue=1))
-    # two passes: token lanes over tok_group meshlets per wave, then the axis lanes
+    # two passes: token lanes over tok_group meshlets per wave, or token
+    # chains (tok_chains per meshlet), then the axis lanes
     for gs, lds, tg in itertools.product((8, 16), (16384, 24576), (32, 64)):
-        out.append(dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, grid=0, use_queue=0))
+        out.append(dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, tok_chains=0, grid=0,
+                        use_queue=0))
+    for (kc, tg), (gs, lds) in itertools.product(((4, 16), (8, 8), (16, 4)),
+                                                 ((16, 16384), (21, 24576))):
+        out.append(dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, tok_chains=kc,
+                        grid=0, use_queue=0))
     for gs, lds, grid in itertools.product(GROUP_SIZES, LDS_BUDGETS, grids):
         out.append(dict(mode=1, group_size=gs, lds_bytes=lds, grid=grid, use_queue=0))
     if include_wave_mode:
@@ -54,8 +61,14 @@ BENCH_CANDIDATES = ([dict(mode=2, group_size=gs, lds_bytes=lds, grid=0, use_queu
                      for gs in (8, 16) for lds in (16384, 24576, 32768, 40960, 49152)]
                     + [dict(mode=2, group_size=16, lds_bytes=24576, grid=g, use_queue=1)
                        for g in (1536, 3072)]
-                    + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, grid=0, use_queue=0)
-                       for gs, lds, tg in ((16, 16384, 32), (16, 16384, 64), (8, 16384, 32))]
+                    + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=32, tok_chains=0, grid=0,
+                            use_queue=0) for gs, lds in ((16, 16384), (8, 16384))]
+                    # the chain token pass: chains per meshlet x meshlets per token wave,
+                    # axis groups of 16 or 21 meshlets
+                    + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_chains=kc, tok_group=tg,
+                            grid=0, use_queue=0)
+                       for kc, tg in ((4, 16), (8, 8), (16, 4))
+                       for gs, lds in ((16, 16384), (21, 24576))]
                     + [dict(mode=1, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
                        for gs in (32, 64) for lds in (16384, 40960)])
 

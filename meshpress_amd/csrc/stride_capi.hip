@@ -371,7 +371,8 @@ struct DecodeSet {
         const bool split = cfg.mode != 1;
         const bool two = cfg.mode == 4;
         const bool nar = narrow_codes;
-        const uint32_t gmax = split ? std::min(cfg.group_size, 16) : cfg.group_size;
+        const uint32_t gmax =
+            split ? std::min(cfg.group_size, two ? kAxisSlots : 16) : cfg.group_size;
         // bytes of LDS: staged words + per-meshlet code storage ("codes"
         // counts vertices in mode 1, block words in mode 2, code words and
         // staged axis words in mode 4, whose axis pass stages nothing else)
@@ -797,12 +798,14 @@ struct DecodeSet {
     bool ok_w = t->waves == 1 || t->waves == 2 || t->waves == 4;
     bool ok_s = t->stage_words == 256 || t->stage_words == 512 || t->stage_words == 1024;
     bool ok_m = t->mode == 0 || t->mode == 1 || t->mode == 2 || t->mode == 4;
-    bool ok_g = t->group_size >= 1 && t->group_size <= (t->mode == 2 || t->mode == 4 ? 16 : 64);
+    bool ok_g = t->group_size >= 1 &&
+                t->group_size <= (t->mode == 2 ? 16 : (t->mode == 4 ? kAxisSlots : 64));
     bool ok_l = t->lds_bytes >= 8192 && t->lds_bytes <= 65536 && t->lds_bytes % 8 == 0;
     if (!ok_w || !ok_s || !ok_m || !ok_g || !ok_l || t->grid < 0 || (t->use_queue && t->grid == 0))
       return fail(STRIDE_E_ARG,
                   "tune: mode in {0,1,2,4}, waves in {1,2,4}, stage_words in {256,512,1024}, "
-                  "group_size in [1,64] ([1,16] for modes 2 and 4), lds_bytes in [8192,65536] "
+                  "group_size in [1,64] ([1,16] for mode 2, [1,21] for mode 4), lds_bytes in "
+                  "[8192,65536] "
                   "(multiple of 8), grid >= 0, "
                   "use_queue needs a persistent grid");
     const int tok_group = t->tok_group ? t->tok_group : DecodeConfig{}.tok_group;

@@ -18,7 +18,8 @@ struct DecodeConfig {
   int stage_words = 512;  // mode 0: LDS words staged per meshlet (256, 512, 1024)
   int grid = 0;           // 0 = one workgroup per unit of work, else persistent grid
   int use_queue = 0;      // mode 0: persistent grid pulls meshlets from an atomic counter
-  int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; modes 2, 4: 1..16
+  int group_size = 16;    // mode 1: meshlets per group (lanes), 1..64; mode 2: 1..16;
+                          // mode 4: 1..21 (kAxisSlots)
   int lds_bytes = 16384;  // modes 1, 2, 4: LDS per group (stage words + codes / walk blocks)
   int tok_group = 32;     // mode 4: meshlets per token-pass wave, 1..64
   int tok_chains = 0;     // mode 4: token chains per meshlet (0 = one token lane per

@@ -733,6 +733,9 @@ __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
   }
 }
 
+// The axis pass: one wave per split group of up to kAxisSlots meshlets,
+// three lanes per meshlet (lane 3s + d runs axis d of slot s; lane 63 only
+// stages and dequantises).
 template <bool NARROW>
 __global__ __launch_bounds__(64) void stride_axes_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -741,53 +744,55 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
   using CodeT = typename std::conditional<NARROW, uint16_t, int32_t>::type;
   extern __shared__ uint32_t lds32[];
   const int lane = threadIdx.x;
-  const int slot = lane & 15, role = lane >> 4;
+  const int slot = lane / 3, d = lane - 3 * slot;
   const uint32_t g = blockIdx.x;
   const uint32_t* words = reinterpret_cast<const uint32_t*>(buf);
   const GroupEntry G = groups[g];
   const uint32_t m = G.m0 + (uint32_t)slot;
-  const bool active = m < G.m1;
+  const bool active = slot < kAxisSlots && m < G.m1;
   MeshletEntry e;
   if (active) e = tab[m];
   const uint32_t nl = active ? e.n_local : 0u;
   const MeshParams* Pm = meshes + (active ? e.mesh : 0u);
   const uint64_t bb = active ? Pm->blob_byte * 8ull : 0ull;
   // LDS per meshlet: its three code arrays, then its axis words (the walk
-  // stays in HBM)
+  // stays in HBM); the slot's values are read from its axis-0 lane 3s
   const uint64_t alo = active ? axis_stage_lo(bb, e) : 0ull;
   const uint32_t an = active ? axis_stage_words(bb, e) : 0u;
   const uint32_t cw = split_code_words(nl, NARROW);
   uint32_t total = 0;
-  const uint32_t pre0 = wave_excl_scan(role == 0 ? cw + an : 0u, total);
-  const uint32_t pre = __shfl(pre0, slot, 64);
-  uint32_t amax = an;   // the same in every role: the max over the 16 slots
-  for (int o = 8; o >= 1; o >>= 1) amax = max(amax, (uint32_t)__shfl_xor((int)amax, o, 64));
+  const uint32_t pre0 = wave_excl_scan(d == 0 ? cw + an : 0u, total);
+  const uint32_t pre = __shfl(pre0, 3 * (lane < 63 ? slot : 0), 64);
+  const uint32_t nslots = G.m1 - G.m0;
+  uint32_t amax = 0;
+  for (uint32_t j = 0; j < nslots; ++j)
+    amax = max(amax, (uint32_t)__builtin_amdgcn_readlane((int)an, (int)(3 * j)));
   // stage the axis words of every meshlet of the group: all the loads of a
   // 64-word row first, then the stores (slots past the group load word 0)
   for (uint32_t b0 = 0; b0 < amax; b0 += 64) {
     const uint32_t i = b0 + (uint32_t)lane;
-    uint32_t t[16];
+    uint32_t t[kAxisSlots];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const uint32_t nj = __shfl(an, j, 64);
-      const uint64_t lj = ((uint64_t)(uint32_t)__shfl((int)(alo >> 32), j, 64) << 32) |
-                          (uint32_t)__shfl((int)(uint32_t)alo, j, 64);
+    for (int j = 0; j < kAxisSlots; ++j) {
+      const uint32_t nj = (uint32_t)__builtin_amdgcn_readlane((int)an, 3 * j);
+      const uint64_t lj =
+          ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(alo >> 32), 3 * j) << 32) |
+          (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)alo, 3 * j);
       t[j] = words[i < nj ? lj + i : lj];
     }
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-      const uint32_t nj = __shfl(an, j, 64);
-      const uint32_t oj = __shfl(pre0, j, 64) + __shfl(cw, j, 64);
+    for (int j = 0; j < kAxisSlots; ++j) {
+      const uint32_t nj = (uint32_t)__builtin_amdgcn_readlane((int)an, 3 * j);
+      const uint32_t oj = (uint32_t)__builtin_amdgcn_readlane((int)(pre0 + cw), 3 * j);
       if (i < nj) lds32[oj + i] = __builtin_bswap32(t[j]);
     }
   }
   __syncthreads();
   CodeT* codes0 = reinterpret_cast<CodeT*>(lds32 + pre);
   const uint32_t cstride = split_code_stride(nl, NARROW);
-  if (active && role > 0) {
+  if (active) {
     const MeshParams& P = *Pm;
-    CodeT* codes = codes0 + cstride * (uint32_t)(role - 1);
-    const int d = role - 1;
+    CodeT* codes = codes0 + cstride * (uint32_t)d;
     const int32_t n0 = pick3(d, P.pred_n[0], P.pred_n[3], P.pred_n[6]);
     const int32_t n1 = pick3(d, P.pred_n[1], P.pred_n[4], P.pred_n[7]);
     const int32_t n2 = pick3(d, P.pred_n[2], P.pred_n[5], P.pred_n[8]);
@@ -802,15 +807,17 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
   }
   __syncthreads();
   // a group lies in one mesh; each meshlet's size and output offset come
-  // from its slot lane, not from memory
+  // from its slot lanes, not from memory
   SplitDequant dq;
-  const uint32_t gmesh = (uint32_t)__shfl((int)(active ? e.mesh : 0u), 0, 64);
+  const uint32_t gmesh = (uint32_t)__builtin_amdgcn_readlane((int)(active ? e.mesh : 0u), 0);
   const MeshParams& Pg = meshes[gmesh];
   const uint32_t voff = active ? e.v_off : 0u;
-  for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
-    const uint32_t pj = __shfl(pre0, (int)j, 64);
-    dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + pj), __shfl(nl, (int)j, 64),
-                       __shfl(voff, (int)j, 64), gmesh, Pg, V, lane);
+  for (uint32_t j = 0; j < nslots; ++j) {
+    const uint32_t pj = (uint32_t)__builtin_amdgcn_readlane((int)pre0, (int)(3 * j));
+    dq.meshlet<NARROW>(reinterpret_cast<const CodeT*>(lds32 + pj),
+                       (uint32_t)__builtin_amdgcn_readlane((int)nl, (int)(3 * j)),
+                       (uint32_t)__builtin_amdgcn_readlane((int)voff, (int)(3 * j)), gmesh, Pg, V,
+                       lane);
   }
 }
 

@@ -94,6 +94,8 @@ static_assert(sizeof(ChainEntry) == 16, "ChainEntry is 16 bytes");
 constexpr int kMaxChains = 16;     // chains per meshlet (4-bit chain index)
 constexpr int kChainMaxLocal = 240;  // placeholders n_local + 0..15 must fit a byte
 
+constexpr int kAxisSlots = 21;   // meshlets per wave of the two-pass axis pass (3 lanes each)
+
 constexpr int kMaxLocal = 256;   // vertices per meshlet (max_verts cap)
 constexpr int kMaxTris = 256;    // triangles per meshlet (max_tris cap)
 
