@@ -113,6 +113,33 @@ def cpu_baseline(data, n_t, gpu_out, budget_s=10.0, n_threads=1):
     return n_t / (t_total / reps) / 1e9, reps, t_total, same
 
 
+def python_c1_baseline(budget_s=3.0):
+    """The oracle's pure-Python decode_dup restatement (the speed class of the
+    reference's pure-Python decoder) on the 65,536-triangle UV sphere (lat-long
+    128 x 256, displaced, 12 bits in world mode: STRIDEEncoder(max_verts=256,
+    precision_error=0.0005)), checked against the GPU decode of the same
+    stream."""
+    from meshpress_amd import workloads
+    from meshpress_amd.encoder import STRIDEEncoder
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    from meshpress_amd.utils.types import Model
+    from oracle import stride_oracle as O
+    V, T = workloads.uv_sphere_c1()
+    data = STRIDEEncoder(max_verts=256, precision_error=0.0005).encode(Model(V, T)).data
+    Vg, Tg = ParaDeltaV5GpuDecoder(data).decode_to_host()
+    reps, t_total, out = 0, 0.0, None
+    while t_total < budget_s and reps < 10:
+        t0 = time.perf_counter()
+        out = O.decode_dup(data)
+        t_total += time.perf_counter() - t0
+        reps += 1
+    same = bool(np.array_equal(Tg.astype(np.int64), out[1])
+                and np.array_equal(Vg.view(np.uint32), out[0].view(np.uint32)))
+    return {"value": len(T) / (t_total / reps) / 1e9, "unit": "Gtris/s", "cores": 1,
+            "kind": "port", "sample": f"UV sphere 128 x 256 (65,536 tris), x{reps} ({t_total:.1f} s)",
+            "gpu_output_matches": same}
+
+
 def host_threads():
     """The host threads a run may use: OMP_NUM_THREADS when set (the GPU
     machine's share of cores), else the visible cores."""
@@ -311,6 +338,37 @@ def batched(n_meshes, steps, warmup, predictor):
     return r
 
 
+def cpu_exhaustive(prep, n_threads):
+    """The exhaustive predictor and coder search on n_threads host threads:
+    the oracle's restatement of fit_predictor's control flow
+    (encoder/_irlp_fit.py:364-394) with the C restatement of
+    _cost_seeds_njit (:120-183) as the candidate cost, then _pick_best_k
+    (encoder/paradelta_v5_dup.py:159-171) per stream in C.  The residual pass
+    between the two is timed apart, as the encoder's own search_s does."""
+    from oracle import stride_oracle as O
+    from tests.oracle_c import load
+    co = load()
+    nat = prep["native"]
+    A, B, C, V, off = (np.ascontiguousarray(x, dtype=np.int64) for x in nat.para_samples())
+
+    def factory(a, b, c, v, o):
+        return lambda cands, K: co.irlp_cost_mt(a, b, c, v, o, cands, K, n_threads)
+
+    def full(a, b, c, v, o, n3, K):
+        return int(co.irlp_cost_mt(a, b, c, v, o, [n3], K, n_threads)[0])
+
+    t0 = time.perf_counter()
+    pn, pk = O.fit_predictor(A, B, C, V, off, cost_fn_factory=factory, full_cost_fn=full)
+    t1 = time.perf_counter()
+    vals, soff, _ = nat.residuals(pn, pk)
+    t2 = time.perf_counter()
+    tagk, _ = co.pick_streams(vals, soff, n_threads)
+    t3 = time.perf_counter()
+    return {"predictor_s": t1 - t0, "coder_s": t3 - t2, "search_s": (t1 - t0) + (t3 - t2),
+            "pred_n": np.asarray(pn).tolist(), "pred_K": np.asarray(pk).tolist(), "tagk": tagk,
+            "vals": vals, "soff": soff}
+
+
 def learned_encode(head_prep, n_heldout):
     """Search time and bytes of the learned proposal vs the exhaustive search."""
     from meshpress_amd import workloads
@@ -321,6 +379,7 @@ def learned_encode(head_prep, n_heldout):
     ex = STRIDEEncoder(**kw)
     le = STRIDEEncoder(selector="learned", **kw)
     specs = workloads.learned_specs(50, 1)[:n_heldout]
+    n_cpu = host_threads()
     preps = []
     for fam, n, seed in specs:
         V, T = workloads.mesh_with_tris(fam, n, seed)
@@ -336,26 +395,45 @@ def learned_encode(head_prep, n_heldout):
             data = e.encode_prepared(prep)
             st = e.last_stats
             r[tag] = {"bytes": len(data), "search_s": st["search_s"],
-                      "predictor_s": st["predictor_s"], "coder_s": st["coder_s"]}
+                      "predictor_s": st["predictor_s"], "coder_s": st["coder_s"],
+                      "pred_n": [int(x) for x in np.asarray(st["pred_n"]).reshape(-1)],
+                      "pred_K": [int(x) for x in np.asarray(st["pred_K"]).reshape(-1)]}
             V, T = ParaDeltaV5GpuDecoder(data).decode_to_host()
             outs.append((V.view(np.uint32).copy(), T))
         r["decode_equal"] = bool(np.array_equal(outs[0][0], outs[1][0])
                                  and np.array_equal(outs[0][1], outs[1][1]))
+        # the same exhaustive search on the host's threads (encode-side CPU
+        # baseline); its choices equal the GPU search's
+        cpu = cpu_exhaustive(prep, n_cpu)
+        from meshpress_amd.encoder import search
+        gpu_tagk, _ = search.select_codes(cpu.pop("vals").astype(np.int32), cpu.pop("soff"))
+        cpu["same_choice"] = bool(cpu["pred_n"] == r["exhaustive"]["pred_n"]
+                                  and cpu["pred_K"] == r["exhaustive"]["pred_K"]
+                                  and np.array_equal(cpu.pop("tagk"), gpu_tagk))
+        cpu.pop("pred_n")
+        cpu.pop("pred_K")
+        r["exhaustive_cpu"] = cpu
         rows.append(r)
         ex_, le_ = r["exhaustive"], r["learned"]
         log(f"learned {name}: search {ex_['search_s'] * 1e3:.1f} ms (predictor "
             f"{ex_['predictor_s'] * 1e3:.1f} + coders {ex_['coder_s'] * 1e3:.1f}) -> "
             f"{le_['search_s'] * 1e3:.1f} ms ({le_['predictor_s'] * 1e3:.1f} + "
-            f"{le_['coder_s'] * 1e3:.1f}), bytes {ex_['bytes']} -> {le_['bytes']}")
+            f"{le_['coder_s'] * 1e3:.1f}), bytes {ex_['bytes']} -> {le_['bytes']}; "
+            f"CPU exhaustive {r['exhaustive_cpu']['search_s'] * 1e3:.1f} ms on {n_cpu} threads "
+            f"(same choice: {r['exhaustive_cpu']['same_choice']})")
 
     def agg(rs):
         be = sum(r["exhaustive"]["bytes"] for r in rs)
         bl = sum(r["learned"]["bytes"] for r in rs)
         te = sum(r["exhaustive"]["search_s"] for r in rs)
         tl = sum(r["learned"]["search_s"] for r in rs)
+        tc = sum(r["exhaustive_cpu"]["search_s"] for r in rs)
         return {"meshes": len(rs), "exhaustive_search_s": te, "learned_search_s": tl,
                 "speedup": te / tl, "regret": (bl - be) / be,
-                "decode_equal": all(r["decode_equal"] for r in rs)}
+                "decode_equal": all(r["decode_equal"] for r in rs),
+                "cpu_exhaustive_search_s": tc, "cpu_threads": n_cpu,
+                "speedup_vs_cpu_exhaustive": tc / tl,
+                "cpu_same_choice": all(r["exhaustive_cpu"]["same_choice"] for r in rs)}
 
     return {"headline_mesh": agg(rows[:1]), "heldout": agg(rows[1:]), "all": agg(rows),
             "per_mesh": rows}
@@ -369,7 +447,7 @@ def main():
     ap.add_argument("--workload", default="heightfield_1900_q12")
     ap.add_argument("--batch", type=int, default=2048, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--batch-predictor", default="generalized", choices=("generalized", "canonical"))
-    ap.add_argument("--learned-heldout", type=int, default=12,
+    ap.add_argument("--learned-heldout", type=int, default=50,
                     help="held-out meshes in the learned-encode comparison (0: skip)")
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-autotune", action="store_true",
@@ -431,6 +509,8 @@ def main():
         log(f"cpu baseline, {nt} threads: {vm:.4f} Gtris/s")
         if not same_m:
             sys.exit("GPU decode differs from the threaded oracle on the bench workload")
+        cpu["python_c1"] = python_c1_baseline()
+        log(f"cpu baseline, pure Python on the UV sphere: {cpu['python_c1']['value']:.6f} Gtris/s")
     del dec
     if not a.quick:
         details[f"{name}_pdlt_v5"] = pdlt_v5(head_prep, name, a.steps, a.warmup)

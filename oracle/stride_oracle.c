@@ -13,6 +13,8 @@
  *   oracle_stream_costs utils/residual_entropy.py:39-54 (_rice_bits,
  *                       _exp_golomb_bits) over Rice k 0..11, EG k 0..7
  *   oracle_irlp_cost    encoder/_irlp_fit.py:120-183 (_cost_seeds_njit)
+ *   oracle_pick         encoder/paradelta_v5_dup.py:159-171 (_pick_best_k)
+ *   oracle_irlp_cost_mt, oracle_pick_streams: the same over host threads
  *
  * n_threads == 1 reads the stream sequentially as decode_dup does; with more
  * threads, meshlets are decoded independently from the offset table.
@@ -430,4 +432,101 @@ void oracle_irlp_cost(const int64_t* a, const int64_t* b, const int64_t* c, cons
     out[s] = total;
   }
   free(tmp);
+}
+
+/* ------------------------------------------------------------------------
+ * Threaded forms of the encoder search cost model: the CPU side of the
+ * exhaustive-search baseline (bench.py's encode-side cpu_baseline).  Same
+ * arithmetic as oracle_irlp_cost and oracle_pick; the work is split over
+ * n_threads host threads by meshlet (costs) or by stream (picks). */
+typedef struct {
+  const int64_t *a, *b, *c, *v, *off, *cand;
+  int64_t m0, m1;
+  int S, K;
+  int64_t* part;   /* S partial sums of this thread */
+} irlp_job_t;
+
+static void* irlp_range(void* arg) {
+  irlp_job_t* j = (irlp_job_t*)arg;
+  int64_t* tmp = (int64_t*)malloc(sizeof(int64_t) * 65536);
+  for (int s = 0; s < j->S; ++s) {
+    const int64_t* n3 = j->cand + 3 * s;
+    int64_t total = 0;
+    for (int64_t m = j->m0; m < j->m1; ++m) {
+      int64_t cnt = j->off[m + 1] - j->off[m];
+      if (cnt <= 0) continue;
+      if (cnt > 65536) cnt = 65536;
+      for (int64_t i = 0; i < cnt; ++i) {
+        int64_t q = j->off[m] + i;
+        tmp[i] = j->v[q] - irlp(j->a[q], j->b[q], j->c[q], n3, j->K);
+      }
+      int64_t cs[20];
+      oracle_stream_costs(tmp, cnt, cs);
+      int64_t best = cs[0];
+      for (int t = 1; t < 20; ++t) if (cs[t] < best) best = cs[t];
+      total += best;
+    }
+    j->part[s] = total;
+  }
+  free(tmp);
+  return NULL;
+}
+
+/* oracle_irlp_cost over n_threads threads (meshlets split evenly). */
+void oracle_irlp_cost_mt(const int64_t* a, const int64_t* b, const int64_t* c, const int64_t* v,
+                         const int64_t* off, int64_t n_m, const int64_t* cand, int S, int K,
+                         int64_t* out, int n_threads) {
+  if (n_threads < 1) n_threads = 1;
+  if (n_threads > 256) n_threads = 256;
+  pthread_t th[256];
+  irlp_job_t jobs[256];
+  int64_t* part = (int64_t*)calloc((size_t)n_threads * (size_t)(S > 0 ? S : 1), sizeof(int64_t));
+  for (int t = 0; t < n_threads; ++t) {
+    jobs[t] = (irlp_job_t){a, b, c, v, off, cand, n_m * t / n_threads, n_m * (t + 1) / n_threads,
+                           S, K, part + (size_t)t * (size_t)S};
+    pthread_create(&th[t], NULL, irlp_range, &jobs[t]);
+  }
+  for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
+  for (int s = 0; s < S; ++s) {
+    int64_t total = 0;
+    for (int t = 0; t < n_threads; ++t) total += part[(size_t)t * (size_t)S + s];
+    out[s] = total;
+  }
+  free(part);
+}
+
+typedef struct {
+  const int64_t *vals, *off;
+  int64_t s0, s1;
+  uint8_t* tagk;
+  int64_t* bits;
+} pick_job_t;
+
+static void* pick_range(void* arg) {
+  pick_job_t* j = (pick_job_t*)arg;
+  for (int64_t s = j->s0; s < j->s1; ++s) {
+    int tag, k;
+    int64_t bits;
+    oracle_pick(j->vals + j->off[s], j->off[s + 1] - j->off[s], &tag, &k, &bits);
+    j->tagk[2 * s] = (uint8_t)tag;
+    j->tagk[2 * s + 1] = (uint8_t)k;
+    j->bits[s] = bits;
+  }
+  return NULL;
+}
+
+/* oracle_pick for streams [off[s], off[s+1]) of vals, s < n_streams, over
+ * n_threads threads: (tag, k) bytes and body bits per stream. */
+void oracle_pick_streams(const int64_t* vals, const int64_t* off, int64_t n_streams,
+                         uint8_t* tagk, int64_t* bits, int n_threads) {
+  if (n_threads < 1) n_threads = 1;
+  if (n_threads > 256) n_threads = 256;
+  pthread_t th[256];
+  pick_job_t jobs[256];
+  for (int t = 0; t < n_threads; ++t) {
+    jobs[t] = (pick_job_t){vals, off, n_streams * t / n_threads, n_streams * (t + 1) / n_threads,
+                           tagk, bits};
+    pthread_create(&th[t], NULL, pick_range, &jobs[t]);
+  }
+  for (int t = 0; t < n_threads; ++t) pthread_join(th[t], NULL);
 }

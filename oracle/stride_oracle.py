@@ -580,8 +580,12 @@ def search_axis(a, b, c, v, off, w_axis, cost_fn=None):
     return best
 
 
-def fit_predictor(A, B, C, V, off, cost_fn_factory=None, subsample=4000):
-    """(n 3x3, K 3) per-axis IRLP fit with canonical fallback on all samples."""
+def fit_predictor(A, B, C, V, off, cost_fn_factory=None, subsample=4000, full_cost_fn=None):
+    """(n 3x3, K 3) per-axis IRLP fit with canonical fallback on all samples.
+    cost_fn_factory(a, b, c, v, off) -> cost_fn(cands, K) replaces the search's
+    candidate costs, full_cost_fn(a, b, c, v, off, n3, K) -> bits the fallback
+    comparison's; both default to irlp_cost (same values, other code)."""
+    full_cost_fn = full_cost_fn or irlp_cost
     A, B, C, V = (np.asarray(x, dtype=np.int64) for x in (A, B, C, V))
     off = np.asarray(off, dtype=np.int64)
     n_m = len(off) - 1
@@ -604,8 +608,8 @@ def fit_predictor(A, B, C, V, off, cost_fn_factory=None, subsample=4000):
         n_fit[d] = ns
         K_fit[d] = K
     for d in range(3):
-        b_fit = irlp_cost(A[:, d], B[:, d], C[:, d], V[:, d], off, n_fit[d], int(K_fit[d]))
-        b_can = irlp_cost(A[:, d], B[:, d], C[:, d], V[:, d], off, CANON_N[d], 0)
+        b_fit = full_cost_fn(A[:, d], B[:, d], C[:, d], V[:, d], off, n_fit[d], int(K_fit[d]))
+        b_can = full_cost_fn(A[:, d], B[:, d], C[:, d], V[:, d], off, CANON_N[d], 0)
         if b_can < b_fit:
             n_fit[d] = CANON_N[d]
             K_fit[d] = 0

@@ -59,6 +59,30 @@ class COracle:
         return out
 
 
+    def irlp_cost_mt(self, a, b, c, v, off, cands, K, n_threads) -> np.ndarray:
+        """irlp_cost over n_threads host threads (int64 inputs, no copies
+        when they are already contiguous int64)."""
+        a, b, c, v = (np.ascontiguousarray(x, dtype=np.int64) for x in (a, b, c, v))
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        cands = np.ascontiguousarray(np.asarray(cands).reshape(-1, 3), dtype=np.int64)
+        out = np.zeros(len(cands), np.int64)
+        self.L.oracle_irlp_cost_mt(a.ctypes.data, b.ctypes.data, c.ctypes.data, v.ctypes.data,
+                                   off.ctypes.data, len(off) - 1, cands.ctypes.data, len(cands),
+                                   int(K), out.ctypes.data, int(n_threads))
+        return out
+
+    def pick_streams(self, vals, off, n_threads):
+        """(tag, k) per stream (n_streams x 2 uint8) and body bits."""
+        v = np.ascontiguousarray(vals, dtype=np.int64)
+        off = np.ascontiguousarray(off, dtype=np.int64)
+        n = len(off) - 1
+        tagk = np.zeros((n, 2), np.uint8)
+        bits = np.zeros(n, np.int64)
+        self.L.oracle_pick_streams(v.ctypes.data, off.ctypes.data, n, tagk.ctypes.data,
+                                   bits.ctypes.data, int(n_threads))
+        return tagk, bits
+
+
 def build():
     subprocess.run(["make", "-s", "-C", str(ROOT / "oracle")], check=True)
 
@@ -78,5 +102,10 @@ def load() -> COracle:
         L.oracle_pick.restype = None
         L.oracle_irlp_cost.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp, C.c_int, C.c_int, vp]
         L.oracle_irlp_cost.restype = None
+        L.oracle_irlp_cost_mt.argtypes = [vp, vp, vp, vp, vp, C.c_int64, vp, C.c_int, C.c_int, vp,
+                                          C.c_int]
+        L.oracle_irlp_cost_mt.restype = None
+        L.oracle_pick_streams.argtypes = [vp, vp, C.c_int64, vp, vp, C.c_int]
+        L.oracle_pick_streams.restype = None
         _lib = L
     return COracle(_lib)

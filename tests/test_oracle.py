@@ -370,3 +370,27 @@ def test_bitcodec_roundtrip_hypothesis(ops):
     for kind, u, k in ops:
         got = r.get(max(1, u.bit_length())) if kind == 0 else (r.rice(k) if kind == 1 else r.eg(k))
         assert got == u
+
+
+def test_threaded_search_costs_equal_serial(coracle):
+    """The C oracle's threaded candidate cost and stream picks (the host side
+    of the exhaustive-search baseline) equal the serial forms, for every
+    thread count, including empty meshlets and streams."""
+    rng = np.random.default_rng(3)
+    cnt = rng.integers(0, 90, 37)
+    cnt[[0, 5]] = 0
+    off = np.concatenate(([0], np.cumsum(cnt))).astype(np.int64)
+    n = int(off[-1])
+    a, b, c = (rng.integers(0, 4096, n) for _ in range(3))
+    v = a + b - c + rng.integers(-40, 40, n)
+    cands = [[32, 32, -32], [33, 31, -32], [1, 1, -1], [0, 0, 0]]
+    for K in (0, 5, 9):
+        ser = coracle.irlp_cost(a, b, c, v, off, cands, K)
+        for t in (1, 3, 8, 64):
+            assert np.array_equal(coracle.irlp_cost_mt(a, b, c, v, off, cands, K, t), ser)
+    vals = rng.integers(-300, 300, n)
+    for t in (1, 4, 40):
+        tagk, bits = coracle.pick_streams(vals, off, t)
+        for s in range(len(off) - 1):
+            assert (int(tagk[s, 0]), int(tagk[s, 1]), int(bits[s])) == \
+                coracle.pick(vals[off[s]:off[s + 1]])
