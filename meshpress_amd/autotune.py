@@ -43,7 +43,7 @@ def candidates(include_wave_mode: bool = True, grids=(0,)):
         out.append(dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, tok_chains=0, grid=0,
                         use_queue=0))
     for (kc, tg), (gs, lds) in itertools.product(((4, 16), (8, 8), (16, 4)),
-                                                 ((16, 16384), (21, 24576))):
+                                                 ((8, 24576), (16, 16384), (21, 24576))):
         out.append(dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=tg, tok_chains=kc,
                         grid=0, use_queue=0))
     for gs, lds, grid in itertools.product(GROUP_SIZES, LDS_BUDGETS, grids):
@@ -68,7 +68,7 @@ BENCH_CANDIDATES = ([dict(mode=2, group_size=gs, lds_bytes=lds, grid=0, use_queu
                     + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_chains=kc, tok_group=tg,
                             grid=0, use_queue=0)
                        for kc, tg in ((4, 16), (8, 8), (16, 4))
-                       for gs, lds in ((16, 16384), (21, 24576))]
+                       for gs, lds in ((8, 24576), (16, 16384), (21, 24576))]
                     + [dict(mode=1, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
                        for gs in (32, 64) for lds in (16384, 40960)])
 

@@ -338,14 +338,15 @@ struct DecodeSet {
       HIP_TRY(hipMemcpy(d_tab, tab.data(), sizeof(MeshletEntry) * n_m, hipMemcpyHostToDevice));
     HIP_TRY(hipMalloc(&d_v_own, 12 * std::max<uint64_t>(n_v, 1)));
     HIP_TRY(hipMalloc(&d_t_own, 12 * std::max<uint64_t>(n_t, 1)));
-    // the default token pass for the set's size: with fewer meshlets than
-    // 16 per SIMD (1,024 SIMDs), 16 short chains per meshlet and 4 meshlets
-    // per wave keep more SIMDs busy; otherwise 4 chains and 16 meshlets
-    cfg.tok_chains = n_m < kFewMeshlets ? 16 : 4;
-    cfg.tok_group = n_m < kFewMeshlets ? 4 : 16;
+    // default launch shape (the fastest measured on the 1.3M- and
+    // 7.2M-triangle meshes and the 2,048-mesh batch): 16 token chains per
+    // meshlet, 4 meshlets per token wave, axis groups of 21 meshlets
+    cfg.tok_chains = 16;
+    cfg.tok_group = 4;
+    cfg.group_size = kAxisSlots;
+    cfg.lds_bytes = 24576;
     return layout();
   }
-  static constexpr uint32_t kFewMeshlets = 16384;
 
   // Partition the meshlets into groups for the current tune and run one
   // validation decode: groups whose codes leave the packed range move to

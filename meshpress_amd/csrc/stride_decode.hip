@@ -580,14 +580,15 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
 __device__ __forceinline__ int wave_shr1(int x, int lane0) {
   return __builtin_amdgcn_update_dpp(lane0, x, 0x138, 0xF, 0xF, false);
 }
-// Inclusive max over lanes 0..lane, values >= -1.
-__device__ __forceinline__ int wave_incl_max(int x) {
-  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x111, 0xF, 0xF, false));
-  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x112, 0xF, 0xF, false));
-  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x114, 0xF, 0xF, false));
-  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x118, 0xF, 0xF, false));
-  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x142, 0xA, 0xF, false));
-  x = max(x, __builtin_amdgcn_update_dpp(-1, x, 0x143, 0xC, 0xF, false));
+// Inclusive max over lanes 0..lane of values >= 0 (0 = none): lanes with no
+// source read 0 (bound_ctrl), so each step is one max with a DPP operand.
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t x) {
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
+  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
   return x;
 }
 
@@ -620,6 +621,8 @@ __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
   const MeshParams& P = meshes[tab[G.m0].mesh];
   stage_words(lds32, words, G.w_lo, G.n_words, lane, 64);
   const uint32_t ntok = (uint32_t)lane < nmesh ? meshlet_tokens(e) : 0u;
+  // the meshlet's chain count, once per lane (phase 2 reads it per meshlet)
+  const uint32_t nchain = chain_count(ntok, chain_tokens(ntok, e.n_local, e.n_strips, k0));
   uint32_t rtotal = 0;
   const uint32_t roff = wave_excl_scan((ntok + 1u) & ~1u, rtotal);
   uint16_t* recs = reinterpret_cast<uint16_t*>(lds32 + G.n_words);
@@ -652,67 +655,84 @@ __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
   // (gen << 16 | t); gen falls from meshlet to meshlet, so a meshlet's
   // values undercut every stale one and the array is cleared only once
   for (uint32_t i = (uint32_t)lane; i < 256; i += 64) first[i] = 0xFFFFFFFFu;
+  // each meshlet's first chain in the wave
+  uint32_t ctotal = 0;
+  const uint32_t cfirst = wave_excl_scan((uint32_t)lane < nmesh ? nchain : 0u, ctotal);
   // phase 2 is one wave: its LDS hand-offs need ordering, not a workgroup
   // barrier (which would also wait for the global stores in flight)
   wave_sync();
-  uint32_t cb = 0;   // the meshlet's first chain in the wave
+  // FIFO placeholders, four meshlets at a time (16 lanes each, one per
+  // rank): the true FIFO at the start of chain c + 1 replaces chain c's
+  // final FIFO; its placeholders stand for entries of the true FIFO at the
+  // start of chain c (chain 0's final FIFO holds local ids only)
+  for (uint32_t j0 = 0; j0 < nmesh; j0 += 4) {
+    const uint32_t jm = j0 + ((uint32_t)lane >> 4);
+    const uint32_t js = jm < nmesh ? jm : j0;
+    const uint32_t mnl = __shfl(e.n_local, (int)js, 64);
+    const uint32_t mcb = __shfl(cfirst, (int)js, 64);
+    const uint32_t mmc = jm < nmesh ? __shfl(nchain, (int)js, 64) : 0u;
+    uint32_t cmax = 0;
+    for (uint32_t q = j0; q < j0 + 4 && q < nmesh; ++q)
+      cmax = max(cmax, (uint32_t)__builtin_amdgcn_readlane((int)nchain, (int)q));
+    const uint32_t rk = (uint32_t)lane & 15u;
+    for (uint32_t c = 1; c + 1 < cmax; ++c) {
+      if (c + 1 < mmc) {
+        const uint32_t x = fb[16 * (mcb + c) + rk];
+        fb[16 * (mcb + c) + rk] =
+            x >= mnl ? fb[16 * (mcb + c - 1) + ((x - mnl) & 15u)] : (uint8_t)x;
+      }
+      wave_sync();
+    }
+  }
   for (uint32_t j = 0; j < nmesh; ++j) {
     const uint32_t nl = (uint32_t)__builtin_amdgcn_readlane((int)e.n_local, (int)j);
-    const uint32_t sf = (uint32_t)__builtin_amdgcn_readlane((int)e.n_strips, (int)j);
     const uint32_t nt = (uint32_t)__builtin_amdgcn_readlane((int)ntok, (int)j);
     const uint32_t rb = (uint32_t)__builtin_amdgcn_readlane((int)roff, (int)j);
     const uint32_t voff = (uint32_t)__builtin_amdgcn_readlane((int)e.v_off, (int)j);
     const uint32_t toff = (uint32_t)__builtin_amdgcn_readlane((int)e.t_off, (int)j);
+    const uint32_t cb = (uint32_t)__builtin_amdgcn_readlane((int)cfirst, (int)j);
     const uint32_t gen = (63u - j) << 16;
-    const uint32_t mc = chain_count(nt, chain_tokens(nt, nl, sf, k0));
     uint16_t* rj = recs + rb;
-    // the true FIFO at the start of chain c + 1 replaces chain c's final
-    // FIFO: its placeholders stand for entries of the true FIFO at the
-    // start of chain c (chain 0's final FIFO holds local ids only)
-    for (uint32_t c = 1; c + 1 < mc; ++c) {
-      if (lane < 16) {
-        const uint32_t x = fb[16 * (cb + c) + lane];
-        fb[16 * (cb + c) + lane] = x >= nl ? fb[16 * (cb + c - 1) + ((x - nl) & 15u)] : (uint8_t)x;
-      }
-      wave_sync();
-    }
-    // placeholders -> local ids; first sighting of every local id
+    // placeholders -> local ids (every record rewritten, no branch); first
+    // sighting of every local id
     for (uint32_t t = lane; t < nt; t += 64) {
       const uint32_t r = rj[t];
-      uint32_t v = r & 0xFFu;
-      if (v >= nl) {   // only in a chain c >= 1 (record bits 12..15)
-        const uint32_t ci = r >> 12;
-        v = fb[16 * (cb + (ci ? ci - 1 : 0u)) + ((v - nl) & 15u)];
-        rj[t] = (uint16_t)((r & 0xFF00u) | v);
-      }
-      atomicMin(first + (v & 0xFFu), gen | t);
+      const uint32_t v = r & 0xFFu;
+      const uint32_t ci = r >> 12;
+      const uint32_t pv = fb[16 * (cb + (ci ? ci - 1 : 0u)) + ((v - nl) & 15u)];
+      const uint32_t rv = v >= nl ? pv : v;   // placeholders only in chains c >= 1
+      rj[t] = (uint16_t)((r & 0xFF00u) | rv);
+      atomicMin(first + rv, gen | t);
     }
     wave_sync();
     uint32_t* wout = walk_g + (((P.v_base + voff + 3) & ~3ull) + 4ull * (G.m0 + j));
     uint32_t* tout = T + 3ull * (P.t_base + toff);
     uint32_t wbase = 0, tbase = 0;
     // carried from the chunk before: the ids of its last two tokens, and
-    // (t << 8 | v_{t-2}) of its last edge-bit-0 token t
-    int cv1 = 0, cv2 = 0, carry = -1;
+    // 1 + (t << 8 | v_{t-2}) of its last edge-bit-0 token t (0: none)
+    int cv1 = 0, cv2 = 0;
+    uint32_t carry = 0;
     for (uint32_t t0 = 0; t0 < nt; t0 += 64) {
       const uint32_t t = t0 + (uint32_t)lane;
       const bool act = t < nt;
-      const uint32_t r = act ? rj[t] : kRecRoot;
+      const uint32_t rr = rj[act ? t : nt - 1];
+      const uint32_t r = act ? rr : kRecRoot;
       const uint32_t v = r & 0xFFu;
       const bool fresh = act && first[v] == (gen | t);
       // v_{t-1}, v_{t-2} by wave shifts; p0 = v_{s-2} of the last edge-bit-0
-      // token s before t by a max scan over (s << 8 | v_{s-2})
+      // token s before t by a max scan over 1 + (s << 8 | v_{s-2})
       const int iv1 = wave_shr1((int)v, cv1);
       const int iv2 = wave_shr1(iv1, cv2);
-      const int key = (act && (r & kRecEc) == 0) ? (int)((t << 8) | (uint32_t)iv2) : -1;
-      const int inc = wave_incl_max(key);
-      const int s = max(wave_shr1(inc, -1), carry);
+      const uint32_t key =
+          (act && (r & kRecEc) == 0) ? ((t << 8) | ((uint32_t)iv2 & 0xFFu)) + 1u : 0u;
+      const uint32_t inc = wave_incl_max(key);
+      const uint32_t s = max((uint32_t)wave_shr1((int)inc, 0), carry);
       cv1 = __builtin_amdgcn_readlane((int)v, 63);
       cv2 = __builtin_amdgcn_readlane(iv1, 63);
-      carry = max(carry, __builtin_amdgcn_readlane(inc, 63));
+      carry = max(carry, (uint32_t)__builtin_amdgcn_readlane((int)inc, 63));
       const uint32_t v1 = (uint32_t)iv1 & 0xFFu;
       const uint32_t v2 = (uint32_t)iv2 & 0xFFu;
-      const uint32_t p0 = (uint32_t)s & 0xFFu;
+      const uint32_t p0 = (s - 1u) & 0xFFu;
       uint32_t a, b;
       const uint32_t entry = chain_entry(r, v1, v2, p0, a, b);
       const bool emit = act && chain_emits(r);
@@ -722,20 +742,16 @@ __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
       tbase += (uint32_t)__popcll(be);
       if (fresh) wout[widx] = entry;
       if (emit) {
-        uint32_t* o = tout + 3ull * tidx;
+        uint32_t* o = tout + (size_t)(3u * tidx);
         o[0] = voff + a;
         o[1] = voff + b;
         o[2] = voff + v;
       }
     }
-    cb += mc;
     wave_sync();
   }
 }
 
-// The axis pass: one wave per split group of up to kAxisSlots meshlets,
-// three lanes per meshlet (lane 3s + d runs axis d of slot s; lane 63 only
-// stages and dequantises).
 template <bool NARROW>
 __global__ __launch_bounds__(64) void stride_axes_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -799,10 +815,15 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
     const int32_t K = pick3(d, P.pred_K[0], P.pred_K[1], P.pred_K[2]);
     const Walk4* wv = reinterpret_cast<const Walk4*>(walk_g + walk_slot(P, e, m));
     const LdsStaged src{lds32 + pre + cw, alo};
-    // one loop per wave: 32-bit IRLP when it is exact for every lane
-    const bool ok = __all(NARROW && irlp32_exact(n0, n1, n2, K))
-                        ? split_axis_g<true>(src, bb, e, P, d, wv, codes)
-                        : split_axis_g<false>(src, bb, e, P, d, wv, codes);
+    // one loop per wave: 32-bit IRLP when it is exact for every lane, the
+    // Rice-only codeword path when every lane's substreams are Rice coded
+    const bool i32 = __all(NARROW && irlp32_exact(n0, n1, n2, K));
+    const bool rice = __all(axis_rice(src, bb, e, P, d));
+    bool ok;
+    if (i32 && rice) ok = split_axis_g<true, true>(src, bb, e, P, d, wv, codes);
+    else if (i32) ok = split_axis_g<true, false>(src, bb, e, P, d, wv, codes);
+    else if (rice) ok = split_axis_g<false, true>(src, bb, e, P, d, wv, codes);
+    else ok = split_axis_g<false, false>(src, bb, e, P, d, wv, codes);
     if (!ok && overflow) overflow[g] = 1u;
   }
   __syncthreads();

@@ -632,7 +632,10 @@ struct ChainLane {
     Wn = cur.peek_raw();
     const bool hit = flag == 0;
     uint32_t jf = hit ? ((fifo.n - 1 - val) & 15u) : fifo.n;
-    const uint32_t v = hit ? fifo.rank(jf) : (val & 0xFFu);
+    // the hit's entry, read on every token and blended in: a select here
+    // compiles to a branch around the read
+    const uint32_t hm = 0u - (uint32_t)hit;
+    const uint32_t v = (fifo.rank(jf) & hm) | (val & 0xFFu & ~hm);
     if constexpr (kFind) {
       if (raw_in_fifo && !hit) {
         const int pf = fifo.find(v);
@@ -658,7 +661,10 @@ STRIDE_DEV uint32_t chain_entry(uint32_t rec, uint32_t v1, uint32_t v2, uint32_t
   a = ec ? p0 : v2;
   b = v1;
   const uint32_t c = ec ? v2 : p0;
-  return (rec & kRecRoot) ? v * 0x01010101u : (v | (a << 8) | (b << 16) | (c << 24));
+  // a root's entry blended in by mask (a select of the two entries compiles
+  // to a branch)
+  const uint32_t rm = 0u - ((rec >> 9) & 1u);
+  return (byte4(v) & rm) | ((v | (a << 8) | (b << 16) | (c << 24)) & ~rm);
 }
 STRIDE_DEV bool chain_emits(uint32_t rec) {
   return (rec & kRecRoot) == 0 || (rec & kRecRoot2) != 0;
@@ -915,6 +921,17 @@ STRIDE_DEV uint32_t codeword_win(uint64_t W, uint32_t egm, int k, uint32_t& u) {
   u = (shifted << k) | low;
   return len <= 32 ? len : 0u;
 }
+// codeword_win for Rice(k) alone: q = the zero count, the k low bits right
+// after the terminating one.  Shift counts are masked to 5 bits: exact
+// whenever len <= 32, and the value is dropped otherwise.
+STRIDE_DEV uint32_t codeword_rice(uint64_t W, int k, uint32_t& u) {
+  const uint32_t T = (uint32_t)(W >> 32);
+  const uint32_t q = T ? (uint32_t)__builtin_clz(T) : 32u;
+  const uint32_t len = q + 1 + (uint32_t)k;
+  const uint32_t low = (T >> ((32u - len) & 31u)) & ((1u << (k & 31)) - 1u);
+  u = (q << (k & 31)) | low;
+  return len <= 32 ? len : 0u;
+}
 STRIDE_DEV int64_t unzigzag32(uint32_t u) {
   return (int64_t)(int32_t)((u >> 1) ^ (0u - (u & 1u)));
 }
@@ -934,6 +951,17 @@ STRIDE_DEV bool irlp32_exact(int32_t n0, int32_t n1, int32_t n2, int32_t K) {
   const bool small = n0 == (int16_t)n0 && n1 == (int16_t)n1 && n2 == (int16_t)n2;
   const int64_t top = (mag(n0) + mag(n1) + mag(n2)) * 65535 + (K > 0 ? (1ll << (K - 1)) : 0);
   return small && K >= 0 && K <= 30 && top <= 0x7FFFFFFFll;
+}
+
+// True when both substreams of axis d of meshlet e are Rice coded (tag 1).
+template <class Src>
+STRIDE_DEV bool axis_rice(const Src& words, uint64_t base, const MeshletEntry& e,
+                          const MeshParams& P, int d) {
+  const uint64_t tb = base + e.resid_bit + (uint32_t)P.g_bits[0] + (uint32_t)P.g_bits[1] +
+                      (uint32_t)P.g_bits[2];
+  const uint32_t TD = bits_at_src(words, tb + 16u * (uint32_t)d, 8);
+  const uint32_t TP = bits_at_src(words, tb + 16u * (uint32_t)(3 + d), 8);
+  return TD == 1u && TP == 1u;
 }
 
 // The words an axis lane pass reads of meshlet e: from the word holding its
@@ -1009,7 +1037,9 @@ STRIDE_DEV uint32_t split_code_words(uint32_t n_local, bool narrow) {
 // offset of the mesh's blob.  I32: IRLP in 32-bit arithmetic (the caller
 // checks irlp32_exact; uint16 codes only).  ok turns false when a code
 // leaves the range of CodeT.
-template <bool I32, class Src, class CodeT>
+// RICE: both substreams of the lane are Rice coded (the kernels choose it
+// per wave when every lane's are), so codewords take codeword_rice.
+template <bool I32, class Src, class CodeT, bool RICE = false>
 struct AxisChain {
   BitStream<Src> sd, sp;
   bool deg, peg, ok;
@@ -1061,8 +1091,9 @@ struct AxisChain {
     // parallelogram stream; only the chosen one advances
     uint32_t u;
     const uint32_t kp = root ? dkp : pkp;
-    const uint32_t len = codeword_win(root ? sd.win : sp.win, (uint32_t)((int32_t)kp >> 5),
-                                      (int)(kp & 31u), u);
+    const uint32_t len = RICE ? codeword_rice(root ? sd.win : sp.win, (int)(kp & 31u), u)
+                              : codeword_win(root ? sd.win : sp.win,
+                                             (uint32_t)((int32_t)kp >> 5), (int)(kp & 31u), u);
     int64_t r;
     if (len) {
       sd.drop(root ? len : 0u);
@@ -1105,12 +1136,12 @@ struct AxisChain {
 
 // The two-pass axis lane: the chain along the meshlet's walk slot in HBM
 // (read four entries ahead).  Returns false when a code leaves CodeT.
-template <bool I32, class Src, class CodeT>
+template <bool I32, bool RICE = false, class Src, class CodeT>
 STRIDE_DEV bool split_axis_g(const Src& words, uint64_t base,
                              const MeshletEntry& e, const MeshParams& P, int d,
                              const Walk4* __restrict__ walk, CodeT* __restrict__ codes) {
   const uint32_t n_local = e.n_local;
-  AxisChain<I32, Src, CodeT> ax;
+  AxisChain<I32, Src, CodeT, RICE> ax;
   ax.init(words, base, e, P, d);
   // the walk slot is 16-byte aligned, with readable words past its end
   // (quad (n_local >> 2) + 1 at most is read).  Entries 1..3, then whole

@@ -1,5 +1,6 @@
 // Host build of the axis lanes' codeword fast path: codeword_win (32-bit,
-// codewords of at most 32 bits) and irlp24 against the general stream
+// codewords of at most 32 bits) and its Rice-only form codeword_rice, and
+// irlp24 against the general stream
 // decoder codeword_s and irlp64 (meshpress_amd/csrc/stride_lane.h), over
 // every unary length q, coder parameter k and coder, with random bodies and
 // trailing bits, and over the edge windows (all zeros, all ones); the longer
@@ -52,6 +53,15 @@ int main() {
           s.init(le.data(), 0);
           uint32_t u = 0;
           const uint32_t len = codeword_win(s.win, eg ? ~0u : 0u, k, u);
+          if (!eg) {   // the Rice-only form gives the same length and value
+            uint32_t ur = 0;
+            const uint32_t lr = codeword_rice(s.win, k, ur);
+            if (lr != len || (len && ur != u)) {
+              std::printf("codeword_rice mismatch: k %d W %016llx (%u, %u) vs (%u, %u)\n", k,
+                          (unsigned long long)W, lr, ur, len, u);
+              return 1;
+            }
+          }
           const uint32_t nb = (eg ? (W ? (uint32_t)__builtin_clzll(W) : 64u) : 0u) + (uint32_t)k;
           const uint32_t qq = W ? (uint32_t)__builtin_clzll(W) : 64u;
           if (qq + 1 + nb > 32) {

@@ -213,6 +213,13 @@ int main(int argc, char** argv) {
           const bool ok2 = i32 ? split_axis_g<true>(gw, 0, e, P, d, wv, c2.data())
                                : split_axis_g<false>(gw, 0, e, P, d, wv, c2.data());
           if (ok2 != okd || (okd && !std::equal(c2.begin(), c2.end(), c))) rc = 4;
+          // the Rice-only codeword path, where both substreams are Rice coded
+          if (axis_rice(sw, 0, e, P, d)) {
+            std::vector<uint16_t> c3(e.n_local, 0x5A5Au);
+            const bool ok3 = i32 ? split_axis_g<true, true>(sw, 0, e, P, d, wv, c3.data())
+                                 : split_axis_g<false, true>(sw, 0, e, P, d, wv, c3.data());
+            if (ok3 != okd || (okd && !std::equal(c3.begin(), c3.end(), c))) rc = 4;
+          }
           for (uint32_t v = 0; v < e.n_local; ++v) codes3[3 * ((size_t)e.v_off + v) + d] = c[v];
         } else {
           int32_t* c = reinterpret_cast<int32_t*>(blk.data()) + cs * d;
