@@ -144,19 +144,19 @@ int stride_decoder_bytes(const stride_decoder* h, uint64_t* stream_bytes,
 int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
 
 /* Launch configuration (autotuned).
- *   mode 4 (default; group_size 16, lds_bytes 16384): two passes.  First
+ *   mode 4 (default; tok_chains 16, tok_group 4, group_size 21, lds_bytes
+ *     24576): two passes.  First
  *     the token pass, writing the walks to a device buffer and the
  *     triangles to the output: with tok_chains > 0 (the default) each
  *     meshlet's strip tokens are cut into about tok_chains chains, one lane
  *     per chain parses its tokens into 16-bit records and the wave then
  *     forms the triangles and walk entries of its tok_group meshlets; with
  *     tok_chains = 0, one token lane per meshlet and tok_group (<= 64)
- *     consecutive meshlets per wave, with an 8-word seen set per lane.  A
- *     new decoder takes tok_chains 16 and tok_group 4 below 16,384
- *     meshlets, else 4 and 16.  Then the axis lanes of the split groups
- *     (group_size, lds_bytes) stream their substreams from device memory
- *     and keep only the codes in LDS.  The launch takes the LDS its largest
- *     group needs, at most lds_bytes.
+ *     consecutive meshlets per wave, with an 8-word seen set per lane.  Then
+ *     the axis pass: split groups of up to group_size (<= 21) meshlets,
+ *     three lanes per meshlet, staging their residual sections in LDS
+ *     (lds_bytes per group) and keeping only the codes there.  The launch
+ *     takes the LDS its largest group needs, at most lds_bytes.
  *   mode 2 (group_size 16, lds_bytes 24576 typical): split groups of up to
  *     16 consecutive meshlets per wave, four lanes per meshlet (one decodes
  *     the strip tokens into a walk list in LDS, then one per axis decodes
