@@ -407,11 +407,11 @@ def learned_encode(head_prep, n_heldout):
         cpu = cpu_exhaustive(prep, n_cpu)
         from meshpress_amd.encoder import search
         gpu_tagk, _ = search.select_codes(cpu.pop("vals").astype(np.int32), cpu.pop("soff"))
-        cpu["same_choice"] = bool(cpu["pred_n"] == r["exhaustive"]["pred_n"]
-                                  and cpu["pred_K"] == r["exhaustive"]["pred_K"]
-                                  and np.array_equal(cpu.pop("tagk"), gpu_tagk))
-        cpu.pop("pred_n")
-        cpu.pop("pred_K")
+        cpu_n, cpu_k, cpu_tagk = (cpu.pop(k) for k in ("pred_n", "pred_K", "tagk"))
+        flat = lambda x: [int(v) for v in np.asarray(x).reshape(-1)]  # noqa: E731
+        cpu["same_choice"] = bool(flat(cpu_n) == r["exhaustive"]["pred_n"]
+                                  and flat(cpu_k) == r["exhaustive"]["pred_K"]
+                                  and np.array_equal(cpu_tagk, gpu_tagk))
         r["exhaustive_cpu"] = cpu
         rows.append(r)
         ex_, le_ = r["exhaustive"], r["learned"]

@@ -86,3 +86,32 @@ def test_trained_network_gains_over_method_without_it(encodes):
     ev_l = sum(s["irlp_evals"] for _, s in encodes["learned"])
     ev_e = sum(s["irlp_evals"] for _, s in encodes["exhaustive"])
     assert ev_l * 5 <= ev_e, (ev_l, ev_e)
+
+
+def test_cpu_exhaustive_search_chooses_like_gpu(encodes):
+    """The host-thread exhaustive search that bench.py times as the
+    encode-side CPU baseline (the oracle's fit_predictor with the threaded
+    C cost, then the threaded C coder pick) makes the GPU search's choices:
+    the same predictor and K per axis, the same code per substream."""
+    from itertools import islice
+
+    from meshpress_amd.encoder import search
+    from oracle import stride_oracle as O
+    from tests.oracle_c import load
+    co = load()
+    for i, prep in enumerate(islice(_preps(), 3)):
+        nat = prep["native"]
+        A, B, C, V, off = (np.ascontiguousarray(x, dtype=np.int64) for x in nat.para_samples())
+        pn, pk = O.fit_predictor(
+            A, B, C, V, off,
+            cost_fn_factory=lambda a, b, c, v, o: (
+                lambda cands, K: co.irlp_cost_mt(a, b, c, v, o, cands, K, 4)),
+            full_cost_fn=lambda a, b, c, v, o, n3, K: int(
+                co.irlp_cost_mt(a, b, c, v, o, [n3], K, 4)[0]))
+        st = encodes["exhaustive"][i][1]
+        assert np.array_equal(np.asarray(pn).reshape(-1), np.asarray(st["pred_n"]).reshape(-1)), i
+        assert np.array_equal(np.asarray(pk).reshape(-1), np.asarray(st["pred_K"]).reshape(-1)), i
+        vals, soff, _ = nat.residuals(pn, pk)
+        tagk_cpu, _ = co.pick_streams(vals, soff, 4)
+        tagk_gpu, _ = search.select_codes(vals.astype(np.int32), soff)
+        assert np.array_equal(tagk_cpu, tagk_gpu), i
