@@ -1,10 +1,11 @@
 """STRIDE decode benchmark on one MI355X.
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME]
-                    [--batch N] [--quick] [--no-cpu]
+                    [--batch N] [--batch-cache NPZ] [--quick] [--no-cpu]
 
-A step is one decode of the whole workload (one launch of the decode: the
-token pass and the axis pass for a single mesh by default).  The headline workload is the 1900 x 1900 heightfield (7,212,402
+A step is one decode of the whole workload: the token pass and then the axis
+pass of one mesh, or for the batch one hipGraph replay of both passes over
+every mesh.  The headline workload is the 1900 x 1900 heightfield (7,212,402
 triangles) with 12-bit bbox quantisation, encoded by this package's
 STRIDEEncoder; inputs are resident in HBM before timing starts.  The same
 run also decodes
@@ -274,12 +275,20 @@ def pdlt_v5(prep, name, steps, warmup):
     return r
 
 
-def batched(n_meshes, steps, warmup, predictor):
-    import torch
+def batch_blobs(n_meshes, predictor, cache=None):
+    """The batch's DPR4 streams: encoded here, or read from (and then kept
+    in) the .npz file `cache`, so that a profiled run need not encode."""
     from meshpress_amd import workloads
-    from meshpress_amd.batch import BatchDecoder
     from meshpress_amd.encoder.many import encode_many
-    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    key = f"{n_meshes}_{predictor}"
+    if cache and os.path.exists(cache):
+        with np.load(cache) as z:
+            if str(z["key"]) == key:
+                n = z["sizes"]
+                flat = z["blobs"]
+                ends = np.cumsum(n)
+                log(f"batch: {n_meshes} streams read from {cache}")
+                return [flat[e - k:e].tobytes() for e, k in zip(ends, n)], 0.0
     specs = workloads.batch_specs(n_meshes, 1000, 200000, 0)
     sources = [functools.partial(workloads.mesh_with_tris, f, n, s) for f, n, s in specs]
     t0 = time.perf_counter()
@@ -288,27 +297,46 @@ def batched(n_meshes, steps, warmup, predictor):
         if i % 256 == 0 or i == n:
             log(f"batch: encoded {i}/{n} meshes ({time.perf_counter() - t0:.0f} s)")
 
-    blobs, st = encode_many(sources, progress=progress, precision_error=bbox_error(12),
-                            precision_mode="bbox_frac", predictor=predictor)
+    blobs, _ = encode_many(sources, progress=progress, precision_error=bbox_error(12),
+                           precision_mode="bbox_frac", predictor=predictor)
     t_enc = time.perf_counter() - t0
+    if cache:
+        np.savez(cache, key=np.array(key), sizes=np.array([len(b) for b in blobs], np.int64),
+                 blobs=np.frombuffer(b"".join(blobs), np.uint8))
+    return blobs, t_enc
+
+
+def batched(n_meshes, steps, warmup, predictor, tune=True, modes=("graph", "launch", "per_mesh"),
+            cache=None):
+    import torch
+    from meshpress_amd.batch import BatchDecoder
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    blobs, t_enc = batch_blobs(n_meshes, predictor, cache)
     bd = BatchDecoder(blobs)
     n_t = bd.n_t_total
     log(f"batch: {n_meshes} meshes, {n_t:,} tris, {bd.n_meshlets:,} meshlets, "
         f"{bd.stream_bytes:,} stream bytes, encode {t_enc:.0f} s")
-    best, sweep = autotune(bd, mode_kw={"mode": "graph"})
-    log(f"batch: tuned {best}")
+    sweep = None
+    if tune:
+        best, sweep = autotune(bd, mode_kw={"mode": "graph"})
+        log(f"batch: tuned {best}")
     r = {"workload": f"batch{n_meshes}_log_uniform_1k_200k_q12", "n_meshes": n_meshes,
          "n_tris": n_t, "n_verts_out": bd.n_v_total, "meshlets": bd.n_meshlets,
          "stream_bytes": bd.stream_bytes, "side_bytes": bd.side_bytes, "predictor": predictor,
-         "encode_s": t_enc, "tune": bd.get_tune(), "tune_sweep_ms": sweep}
+         "encode_s": t_enc, "tune": bd.get_tune()}
+    if sweep is not None:
+        r["tune_sweep_ms"] = sweep
     bytes_alg = bd.bytes_read + bd.bytes_written
-    for mode in ("graph", "launch", "per_mesh"):
+    r["algorithmic_bytes"] = bytes_alg
+    for mode in modes:
         # the per-mesh loop is ~100x slower: fewer repetitions keep the run short
         k = steps if mode != "per_mesh" else max(3, steps // 10)
         w, g = time_calls(functools.partial(bd.decode, mode=mode), k, min(warmup, k))
         r[mode] = {"ms_wall": w * 1e3, "ms_gpu": g * 1e3, "gtris_s": n_t / w / 1e9,
                    "hbm_gb_s": bytes_alg / g / 1e9}
         log(f"batch {mode}: {w * 1e3:.3f} ms wall, {n_t / w / 1e9:.3f} Gtris/s")
+    if "per_mesh" not in modes:
+        return r
     # equal output: graph replay vs the per-mesh loop, bit for bit
     bd.decode(mode="graph")
     v1, t1 = bd.d_verts.clone(), bd.d_tris.clone()
@@ -444,8 +472,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="heightfield_1900_q12")
+    ap.add_argument("--workload", default="heightfield_1900_q12",
+                    help="heightfield_1900_q12 | heightfield_1900_q16 | icosphere_l8_q12 | "
+                         "batch (the batched workload alone, as the headline)")
     ap.add_argument("--batch", type=int, default=2048, help="meshes in the batched workload (0: skip)")
+    ap.add_argument("--batch-cache", default=None, metavar="NPZ",
+                    help="read the batch's streams from this file, or encode and write them there")
+    ap.add_argument("--batch-modes", default="graph,launch,per_mesh",
+                    help="batched decode modes to time (graph, launch, per_mesh)")
     ap.add_argument("--batch-predictor", default="generalized", choices=("generalized", "canonical"))
     ap.add_argument("--learned-heldout", type=int, default=50,
                     help="held-out meshes in the learned-encode comparison (0: skip)")
@@ -477,6 +511,19 @@ def main():
     from meshpress_amd import workloads
 
     name = a.workload
+    if name == "batch":
+        r = batched(a.batch, a.steps, a.warmup, a.batch_predictor, tune=not a.no_autotune,
+                    modes=tuple(a.batch_modes.split(",")), cache=a.batch_cache)
+        mode = "graph" if "graph" in r else tuple(k for k in ("launch", "per_mesh") if k in r)[0]
+        print(json.dumps({
+            "metric": METRIC, "value": r[mode]["gtris_s"], "unit": "Gtris/s", "n_gpus": 1,
+            "steps": a.steps, "warmup": a.warmup, "ms_per_step": r[mode]["ms_wall"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "int32+f64", "data": "synthetic",
+            "config": {"workload": r["workload"], "mode": mode, "n_tris": r["n_tris"],
+                       "meshlets": r["meshlets"], "tune": r["tune"]},
+            "details": {"batched": r}}), flush=True)
+        return
     if name.startswith("heightfield_1900"):
         bits = 16 if name.endswith("q16") else 12
         log("generating 1900x1900 heightfield")
@@ -531,7 +578,9 @@ def main():
             details["learned_encode"] = learned_encode(head_prep, a.learned_heldout)
         del head_prep
         if a.batch > 0:
-            details["batched"] = batched(a.batch, a.steps, a.warmup, a.batch_predictor)
+            details["batched"] = batched(a.batch, a.steps, a.warmup, a.batch_predictor,
+                                         modes=tuple(a.batch_modes.split(",")),
+                                         cache=a.batch_cache)
     out = {
         "metric": METRIC, "value": head["gtris_s"], "unit": "Gtris/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": head["ms_wall"], "higher_is_better": True,
