@@ -1087,6 +1087,10 @@ struct AxisChain {
   }
   STRIDE_DEV void step(uint32_t w, CodeT* __restrict__ codes) {
     const bool root = walk_root(w);
+    const uint32_t ia = (w >> 8) & 0xFFu, ib = (w >> 16) & 0xFFu, ic = w >> 24;
+    // the reference codes are read before the codeword, so that the LDS
+    // reads overlap its decode (the previous step's store is already issued)
+    const CodeT ca = codes[ia], cb = codes[ib], cc = codes[ic];
     // the code comes from the delta stream (strip roots) or the
     // parallelogram stream; only the chosen one advances
     uint32_t u;
@@ -1105,7 +1109,6 @@ struct AxisChain {
         if (r != (int64_t)(int32_t)r) bad |= 0x80000000u;   // no 16-bit code then
       }
     }
-    const uint32_t ia = (w >> 8) & 0xFFu, ib = (w >> 16) & 0xFFu, ic = w >> 24;
     if constexpr (I32 && sizeof(CodeT) == 2) {
       // 32-bit chain.  The stored codes are uint16 and irlp32_exact holds,
       // so pred and prev lie in int32; c = base + r wraps only on a signed
@@ -1113,8 +1116,7 @@ struct AxisChain {
       // code outside [0, 65535] has c's top 16 bits set.  bad collects both
       // (and a general-path residual outside int32); the chain is exact
       // while bad's top 16 bits stay clear.
-      const int32_t pred = irlp24((int32_t)codes[ia], (int32_t)codes[ib], (int32_t)codes[ic], n0,
-                                  n1, n2, K, rnd);
+      const int32_t pred = irlp24((int32_t)ca, (int32_t)cb, (int32_t)cc, n0, n1, n2, K, rnd);
       const uint32_t r32 = (uint32_t)r;
       const uint32_t base = (uint32_t)(root ? prev32 : pred);
       const uint32_t c = base + r32;
@@ -1123,7 +1125,7 @@ struct AxisChain {
       prev32 = (int32_t)c;
     } else {
       const int64_t pred =
-          irlp64((int64_t)codes[ia], (int64_t)codes[ib], (int64_t)codes[ic], n0, n1, n2, K);
+          irlp64((int64_t)ca, (int64_t)cb, (int64_t)cc, n0, n1, n2, K);
       const int64_t c = (root ? prev : pred) + r;
       ok &= code_fits<CodeT>(c);
       codes[w & 0xFFu] = (CodeT)c;
