@@ -283,12 +283,13 @@ def batch_blobs(n_meshes, predictor, cache=None):
     key = f"{n_meshes}_{predictor}"
     if cache and os.path.exists(cache):
         with np.load(cache) as z:
-            if str(z["key"]) == key:
+            if str(z["key"]) == key and "n_v" in z.files:
                 n = z["sizes"]
                 flat = z["blobs"]
                 ends = np.cumsum(n)
                 log(f"batch: {n_meshes} streams read from {cache}")
-                return [flat[e - k:e].tobytes() for e, k in zip(ends, n)], 0.0
+                return ([flat[e - k:e].tobytes() for e, k in zip(ends, n)],
+                        [int(x) for x in z["n_v"]], 0.0)
     specs = workloads.batch_specs(n_meshes, 1000, 200000, 0)
     sources = [functools.partial(workloads.mesh_with_tris, f, n, s) for f, n, s in specs]
     t0 = time.perf_counter()
@@ -297,13 +298,14 @@ def batch_blobs(n_meshes, predictor, cache=None):
         if i % 256 == 0 or i == n:
             log(f"batch: encoded {i}/{n} meshes ({time.perf_counter() - t0:.0f} s)")
 
-    blobs, _ = encode_many(sources, progress=progress, precision_error=bbox_error(12),
-                           precision_mode="bbox_frac", predictor=predictor)
+    blobs, st = encode_many(sources, progress=progress, precision_error=bbox_error(12),
+                            precision_mode="bbox_frac", predictor=predictor)
     t_enc = time.perf_counter() - t0
+    n_v = [int(s["n_v"]) for s in st]
     if cache:
         np.savez(cache, key=np.array(key), sizes=np.array([len(b) for b in blobs], np.int64),
-                 blobs=np.frombuffer(b"".join(blobs), np.uint8))
-    return blobs, t_enc
+                 n_v=np.array(n_v, np.int64), blobs=np.frombuffer(b"".join(blobs), np.uint8))
+    return blobs, n_v, t_enc
 
 
 def batched(n_meshes, steps, warmup, predictor, tune=True, modes=("graph", "launch", "per_mesh"),
@@ -311,7 +313,7 @@ def batched(n_meshes, steps, warmup, predictor, tune=True, modes=("graph", "laun
     import torch
     from meshpress_amd.batch import BatchDecoder
     from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
-    blobs, t_enc = batch_blobs(n_meshes, predictor, cache)
+    blobs, n_v, t_enc = batch_blobs(n_meshes, predictor, cache)
     bd = BatchDecoder(blobs)
     n_t = bd.n_t_total
     log(f"batch: {n_meshes} meshes, {n_t:,} tris, {bd.n_meshlets:,} meshlets, "
@@ -328,6 +330,7 @@ def batched(n_meshes, steps, warmup, predictor, tune=True, modes=("graph", "laun
         r["tune_sweep_ms"] = sweep
     bytes_alg = bd.bytes_read + bd.bytes_written
     r["algorithmic_bytes"] = bytes_alg
+    r["bpv_mean"] = float(np.mean([8 * len(b) / nv for b, nv in zip(blobs, n_v)]))
     for mode in modes:
         # the per-mesh loop is ~100x slower: fewer repetitions keep the run short
         k = steps if mode != "per_mesh" else max(3, steps // 10)
@@ -335,7 +338,7 @@ def batched(n_meshes, steps, warmup, predictor, tune=True, modes=("graph", "laun
         r[mode] = {"ms_wall": w * 1e3, "ms_gpu": g * 1e3, "gtris_s": n_t / w / 1e9,
                    "hbm_gb_s": bytes_alg / g / 1e9}
         log(f"batch {mode}: {w * 1e3:.3f} ms wall, {n_t / w / 1e9:.3f} Gtris/s")
-    if "per_mesh" not in modes:
+    if not {"graph", "per_mesh"} <= set(modes):
         return r
     # equal output: graph replay vs the per-mesh loop, bit for bit
     bd.decode(mode="graph")
@@ -362,7 +365,6 @@ def batched(n_meshes, steps, warmup, predictor, tune=True, modes=("graph", "laun
     del decs
     r["speedup_vs_per_mesh"] = r["per_mesh"]["ms_wall"] / r["graph"]["ms_wall"]
     r["speedup_vs_per_mesh_python"] = r["per_mesh_python"]["ms_wall"] / r["graph"]["ms_wall"]
-    r["bpv_mean"] = float(np.mean([8 * s["bytes"] / s["n_v"] for s in st]))
     return r
 
 
