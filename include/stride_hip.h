@@ -6,7 +6,10 @@
  * stride_last_error() copies the message of the last failure on the calling
  * thread.  Device pointers are HIP device memory; `stream` is a hipStream_t
  * (NULL = the default stream).  Handles are not reentrant: one thread at a
- * time per handle; creating handles is thread safe.
+ * time per handle; creating handles is thread safe.  The decodes of one
+ * handle share its device scratch (the two-pass walk buffer), so they must
+ * not overlap on the GPU: issue them on one stream, or order the streams
+ * with events.
  *
  * Reference interfaces replaced (file:line in maletsden/meshpress):
  *   decoder  utils/paradelta_v5_cuda.py:580-718  ParaDeltaV5GpuDecoder

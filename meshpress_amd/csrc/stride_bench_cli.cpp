@@ -40,13 +40,14 @@ static int die(const char* what, int rc) {
 int main(int argc, char** argv) {
   if (argc < 2) {
     std::fprintf(stderr, "usage: %s FILE [--iters N] [--warmup W] [--mode M] [--group G] "
-                         "[--lds BYTES] [--grid R] [--queue 0|1]\n", argv[0]);
+                         "[--lds BYTES] [--grid R] [--queue 0|1] [--tok T] [--chains K] [--stage W] "
+                         "[--chain 64|32]\n", argv[0]);
     return 2;
   }
   int iters = 100, warmup = 10, chain = 64;
   stride_tune want{};
   bool set_mode = false, set_group = false, set_lds = false, set_grid = false, set_queue = false,
-       set_stage = false, set_tok = false;
+       set_stage = false, set_tok = false, set_chains = false;
   for (int i = 2; i + 1 < argc; i += 2) {
     const std::string k = argv[i];
     const int v = std::atoi(argv[i + 1]);
@@ -60,6 +61,7 @@ int main(int argc, char** argv) {
     else if (k == "--chain") chain = v;
     else if (k == "--stage") { want.stage_words = v; set_stage = true; }
     else if (k == "--tok") { want.tok_group = v; set_tok = true; }
+    else if (k == "--chains") { want.tok_chains = v; set_chains = true; }
     else { std::fprintf(stderr, "stride_bench: unknown option %s\n", k.c_str()); return 2; }
   }
   if (iters < 1 || warmup < 0) { std::fprintf(stderr, "stride_bench: bad iteration counts\n"); return 2; }
@@ -92,6 +94,7 @@ int main(int argc, char** argv) {
   if (set_queue) t.use_queue = want.use_queue;
   if (set_stage) t.stage_words = want.stage_words;
   if (set_tok) t.tok_group = want.tok_group;
+  if (set_chains) t.tok_chains = want.tok_chains;
   if ((rc = stride_decoder_set_tune(h, &t))) return die("set_tune", rc);
   uint64_t n_v = 0, n_t = 0, sb = 0, side = 0;
   uint32_t n_m = 0;
@@ -121,10 +124,12 @@ int main(int argc, char** argv) {
   std::printf("{\"file\": \"%s\", \"format\": \"%s\", \"n_tris\": %llu, \"n_verts\": %llu, "
               "\"meshlets\": %u, \"iters\": %d, \"ms\": %.6f, \"gtris_s\": %.4f, \"hbm_gb_s\": %.2f, "
               "\"chain\": %d, \"tune\": {\"mode\": %d, \"group_size\": %d, \"lds_bytes\": %d, "
-              "\"grid\": %d, \"use_queue\": %d, \"stage_words\": %d, \"tok_group\": %d}}\n",
+              "\"grid\": %d, \"use_queue\": %d, \"stage_words\": %d, \"tok_group\": %d, "
+              "\"tok_chains\": %d}}\n",
               argv[1], fmt, (unsigned long long)n_t, (unsigned long long)n_v,
               n_m, iters, ms, (double)n_t / (ms * 1e-3) / 1e9, bytes / (ms * 1e-3) / 1e9, chain, t.mode,
-              t.group_size, t.lds_bytes, t.grid, t.use_queue, t.stage_words, t.tok_group);
+              t.group_size, t.lds_bytes, t.grid, t.use_queue, t.stage_words, t.tok_group,
+              t.tok_chains);
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   (void)hipStreamDestroy(s);

@@ -564,7 +564,7 @@ __global__ void stride_chain_scan_kernel(const uint8_t* __restrict__ buf,
   const MeshletEntry e = tab[m];
   const MeshParams& P = meshes[e.mesh];
   GlobalWords src{reinterpret_cast<const uint32_t*>(buf + P.blob_byte), (e.end_bit + 31u) / 32u + 2u};
-  chain_starts(src, e.start_bit, e, m, k0, out + cb);
+  chain_starts(src, e.start_bit, e, k0, out + cb);
 }
 
 __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
@@ -628,18 +628,24 @@ __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
   uint16_t* recs = reinterpret_cast<uint16_t*>(lds32 + G.n_words);
   uint32_t* fst = lds32 + G.n_words + rtotal / 2;   // 4 words per chain
   uint32_t* first = fst + 4 * 64;
-  // the chain lane's meshlet fields, read from its meshlet lane
-  const uint32_t mi = (uint32_t)lane < nch ? ce.m - G.m0 : 0u;
+  // the chain lane's meshlet: the wave's chains run meshlet by meshlet, each
+  // meshlet's from chain index 0, so it is the count of index-0 chains up to
+  // the lane, less one; its fields come from its meshlet lane
+  const bool chain_lane = (uint32_t)lane < nch;
+  const uint64_t starts = __ballot(chain_lane && ((ce.state >> 24) & 15u) == 0u);
+  const uint32_t mi =
+      chain_lane ? lanes_below(starts) - (((ce.state >> 24) & 15u) != 0u ? 1u : 0u) : 0u;
   const uint32_t c_roff = __shfl(roff, (int)mi, 64);
   const uint32_t c_start = __shfl(e.start_bit, (int)mi, 64);
   const uint32_t c_nl = __shfl(e.n_local, (int)mi, 64);
   const uint32_t c_sf = __shfl(e.n_strips, (int)mi, 64);
+  const uint32_t c_ntok = __shfl(ntok, (int)mi, 64);
   __syncthreads();
-  if ((uint32_t)lane < nch) {
+  if (chain_lane) {
     const uint64_t abs_bit = P.blob_byte * 8ull + c_start;
     LdsCursor cur{lds32, (uint32_t)(abs_bit - 32ull * G.w_lo) + (ce.pos & 0xFFFFu)};
     ChainLane<LdsCursor> cl;
-    cl.init(cur, ce, c_nl, c_sf);
+    cl.init(cur, ce, chain_end(ce, c_ntok, chain_tokens(c_ntok, c_nl, c_sf, k0)), c_nl, c_sf);
     RecSink sink{recs + c_roff};
     if (wave_any(cl.raw_in_fifo)) chain_run<true>(cl, cur, sink);
     else chain_run<false>(cl, cur, sink);

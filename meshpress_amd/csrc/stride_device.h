@@ -83,14 +83,15 @@ static_assert(sizeof(GroupEntry) == 32, "GroupEntry is 32 bytes");
 // into up to kMaxChains chains; the chain-start scan records the parser state
 // at each cut.  A chain after the first starts with placeholder FIFO entries
 // (n_local + slot), resolved once the chain before it has finished.
+// The entry holds only what the cut's position does not fix: a token wave
+// finds a chain's meshlet from the chain-index-0 entries before it, and its
+// last token from the meshlet's chain length (chain_end).
 struct ChainEntry {
-  uint32_t m;       // global meshlet index
   uint32_t pos;     // bit offset from the meshlet header (low 16) | first token << 16
   uint32_t state;   // strip token index j (9 bits) | strip tokens lp2 << 9 (10 bits) |
                     // FIFO fill << 19 (5 bits) | chain index in the meshlet << 24 (4 bits)
-  uint32_t tok1;    // one past the chain's last token
 };
-static_assert(sizeof(ChainEntry) == 16, "ChainEntry is 16 bytes");
+static_assert(sizeof(ChainEntry) == 8, "ChainEntry is 8 bytes");
 constexpr int kMaxChains = 16;     // chains per meshlet (4-bit chain index)
 constexpr int kChainMaxLocal = 240;  // placeholders n_local + 0..15 must fit a byte
 

@@ -533,12 +533,18 @@ STRIDE_DEV uint32_t chain_count(uint32_t n_tok, uint32_t lm) {
 STRIDE_DEV uint32_t meshlet_tokens(const MeshletEntry& e) {
   return e.n_tris + 2 * (e.n_strips & kStripMask);
 }
+// One past the last token of chain c of a meshlet of n_tok tokens cut every
+// lm tokens.
+STRIDE_DEV uint32_t chain_end(const ChainEntry& c, uint32_t n_tok, uint32_t lm) {
+  const uint32_t t0 = c.pos >> 16;
+  return t0 + lm < n_tok ? t0 + lm : n_tok;
+}
 
 // The parser state at every chain cut of meshlet e (its header at bit
 // `start` of src), written to out[0 .. chain_count).  The stream was
 // validated by the index pre-pass.
 template <class Src>
-STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry& e, uint32_t m,
+STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry& e,
                              uint32_t k0, ChainEntry* __restrict__ out) {
   const uint32_t n_tok = meshlet_tokens(e);
   const uint32_t lm = chain_tokens(n_tok, e.n_local, e.n_strips, k0);
@@ -552,11 +558,9 @@ STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry&
   for (uint32_t t = 0; t < n_tok; ++t) {
     if (t == next) {
       ChainEntry ce;
-      ce.m = m;
       ce.pos = (r.pos - start) | (t << 16);
       ce.state = j | (lp2 << 9) | (fifo.n << 19) | (c << 24);
-      next = t + lm < n_tok ? t + lm : n_tok;
-      ce.tok1 = next;
+      next = chain_end(ce, n_tok, lm);
       out[c++] = ce;
     }
     if (j == lp2) {
@@ -572,8 +576,7 @@ STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry&
     ++j;
   }
   if (n_tok == 0) {   // not produced by a valid stream; one empty chain
-    ChainEntry ce{m, 0u, 0u, 0u};
-    out[0] = ce;
+    out[0] = ChainEntry{0u, 0u};
   }
 }
 
@@ -588,10 +591,12 @@ struct ChainLane {
   bool raw_in_fifo;
   MtfLite fifo;
 
-  // `cur` is at the chain's first token: the meshlet header + (c.pos & 0xFFFF).
-  STRIDE_DEV void init(Cursor& cur, const ChainEntry& c, uint32_t n_local, uint32_t strips_flags) {
+  // `cur` is at the chain's first token: the meshlet header + (c.pos & 0xFFFF);
+  // `end` is one past its last token (chain_end).
+  STRIDE_DEV void init(Cursor& cur, const ChainEntry& c, uint32_t end, uint32_t n_local,
+                       uint32_t strips_flags) {
     t = c.pos >> 16;
-    t1 = c.tok1;
+    t1 = end;
     j = c.state & 511u;
     lp2 = (c.state >> 9) & 1023u;
     const uint32_t n = (c.state >> 19) & 31u;

@@ -124,7 +124,9 @@ class ParaDeltaV5GpuDecoder:
     def decode(self, stream=None):
         """Launch on `stream` (default: torch's current stream); returns the
         decoder-owned (verts f32 (n_v,3), tris int32 (n_t,3)) device tensors:
-        world positions for DPR4, normalised ones for PDLT v5."""
+        world positions for DPR4, normalised ones for PDLT v5.  Decodes of
+        one decoder share its outputs and device scratch: on several streams
+        they must be ordered (events), never run at once."""
         if self.version == 5:
             _lib.check(self._L.stride_decode_normalized(
                 self._h, _lib.ptr(self.d_verts_norm), _lib.ptr(self.d_tris), _lib.stream_ptr(stream)))

@@ -115,10 +115,11 @@ int main(int argc, char** argv) {
     for (uint32_t m = 0; m < n_m; ++m) {
       const MeshletEntry& e = tab[m];
       const uint32_t nl = e.n_local, nt = meshlet_tokens(e);
-      const uint32_t cnt = chain_count(nt, chain_tokens(nt, nl, e.n_strips, k0));
+      const uint32_t lm = chain_tokens(nt, nl, e.n_strips, k0);
+      const uint32_t cnt = chain_count(nt, lm);
       std::vector<ChainEntry> ch(cnt);
       GlobalWords src{le.data(), (uint32_t)n_words};
-      chain_starts(src, e.start_bit, e, m, k0, ch.data());
+      chain_starts(src, e.start_bit, e, k0, ch.data());
       // phase 1: every chain on its own, exactly sized record array
       std::vector<uint16_t> rec(nt);
       std::vector<uint8_t> fin(16 * cnt);
@@ -127,11 +128,11 @@ int main(int argc, char** argv) {
         void operator()(uint32_t t, uint32_t x) const { r->at(t) = (uint16_t)x; }
       } sink{&rec};
       for (uint32_t c = 0; c < cnt; ++c) {
-        if (ch[c].m != m || ((ch[c].state >> 24) & 15u) != c) return 5;
+        if (((ch[c].state >> 24) & 15u) != c) return 5;
         StreamBits cur;
         cur.init(le.data(), e.start_bit + (ch[c].pos & 0xFFFFu));
         ChainLane<StreamBits> cl;
-        cl.init(cur, ch[c], nl, e.n_strips);
+        cl.init(cur, ch[c], chain_end(ch[c], nt, lm), nl, e.n_strips);
         while (!cl.done()) cl.step<true>(cur, sink);
         if (cl.t != (c + 1 < cnt ? (ch[c + 1].pos >> 16) : nt)) return 5;
         for (int i = 0; i < 16; ++i)

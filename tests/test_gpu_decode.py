@@ -316,6 +316,11 @@ def test_native_bench_cli(tmp_path):
         assert r.returncode == 0, r.stderr
         out = json.loads(r.stdout)
         assert out["format"] == fmt and out["n_tris"] == 65536 and out["ms"] > 0
-    r = subprocess.run([str(build.BENCH), str(tmp_path / "c1.dpr4"), "--mode", "7"],
-                       capture_output=True, text=True, timeout=120)
-    assert r.returncode == 1 and "set_tune" in r.stderr
+    r = subprocess.run([str(build.BENCH), str(tmp_path / "c1.dpr4"), "--iters", "3", "--chains",
+                        "4", "--tok", "16"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    assert json.loads(r.stdout)["tune"]["tok_chains"] == 4
+    for bad in (["--mode", "7"], ["--chains", "17"]):
+        r = subprocess.run([str(build.BENCH), str(tmp_path / "c1.dpr4"), *bad],
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 1 and "set_tune" in r.stderr, bad
