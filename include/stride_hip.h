@@ -189,7 +189,8 @@ typedef struct stride_tune {
   int32_t lds_bytes;
   int32_t reserved;
   int32_t tok_group;   /* mode 4: meshlets per token-pass wave, 1..64 (0: the
-                          default, 32 for DPR4 streams, 64 for PDLT v5) */
+                          default: 4 for DPR4 chain token waves, 32 for one
+                          token lane per meshlet, 64 for PDLT v5) */
   int32_t tok_chains;  /* DPR4 mode 4: 0 = one token lane per meshlet; 1..16 =
                           the chain token pass, each meshlet's strip tokens cut
                           into about tok_chains chains of >= 16 tokens parsed

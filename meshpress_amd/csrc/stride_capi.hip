@@ -809,11 +809,11 @@ struct DecodeSet {
                   "[8192,65536] "
                   "(multiple of 8), grid >= 0, "
                   "use_queue needs a persistent grid");
-    const int tok_group = t->tok_group ? t->tok_group : DecodeConfig{}.tok_group;
-    if (tok_group < 1 || tok_group > 64) return fail(STRIDE_E_ARG, "tok_group must be in [1, 64]");
     const int tok_chains = t->tok_chains < 0 ? cfg.tok_chains : t->tok_chains;
     if (tok_chains > kMaxChains)
       return fail(STRIDE_E_ARG, "tok_chains must be in [0, 16] (or -1 to keep)");
+    const int tok_group = t->tok_group ? t->tok_group : (tok_chains > 0 ? 4 : 32);
+    if (tok_group < 1 || tok_group > 64) return fail(STRIDE_E_ARG, "tok_group must be in [1, 64]");
     const bool relayout = t->mode != cfg.mode || t->group_size != cfg.group_size ||
                           t->lds_bytes != cfg.lds_bytes || tok_group != cfg.tok_group ||
                           tok_chains != cfg.tok_chains;
