@@ -587,8 +587,11 @@ struct DecodeSet {
           const uint32_t nt = meshlet_tokens(e);
           const uint32_t cnt = chain_count(nt, chain_tokens(nt, e.n_local, e.n_strips, k0));
           const uint64_t rw = ((nt + 1) & ~1u) / 2;
+          // the token section only (the header and strips, up to the
+          // residual section), + 2 words for the lanes' one-token
+          // look-ahead window
           const uint64_t lo = (bb + e.start_bit) >> 5;
-          const uint64_t hi = ((bb + e.end_bit + 31) >> 5) + 2;
+          const uint64_t hi = ((bb + e.resid_bit + 31) >> 5) + 2;
           if (open && j == cur.m1 && j - cur.m0 < (uint32_t)cfg.tok_group &&
               cur.pad[1] - cur.pad[0] + cnt <= 64 &&
               need(hi - cur.w_lo, rec_words + rw) <= kChainLds) {
