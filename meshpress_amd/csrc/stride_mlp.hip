@@ -1,6 +1,6 @@
 // Learned coder/predictor proposal: a 32 -> 128 -> 128 -> 176 MLP with
 // BF16 weights and activations and FP32 accumulation on the gfx950 matrix
-// cores (v_mfma_f32_16x16x16_bf16), plus its backward pass, a grouped
+// cores (v_mfma_f32_16x16x32_bf16), plus its backward pass, a grouped
 // softmax cross-entropy and an Adam step.
 //
 // Parameter layout (float32 master copy, n_params = 43,440):
@@ -30,7 +30,7 @@ constexpr int kNW = kIn * kHid + kHid * kHid + kHid * kOut;   // 43,008
 constexpr int kNP = kNW + kHid + kHid + kOut;                 // 43,440
 __constant__ int kHeadOff[kHeads + 1] = {0, 17, 34, 51, 71, 91, 111, 131, 151, 171};
 
-typedef short s4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef float f4 __attribute__((ext_vector_type(4)));
 
 // float -> bf16, round to nearest even (NaN stays a quiet NaN)
@@ -59,27 +59,28 @@ struct GemmArgs {
 };
 
 // C = op(A) op(B): a 256-thread workgroup owns a 64 x 64 tile of C; wave w
-// owns rows 16w..16w+15 as four 16 x 16 MFMA tiles.  Each K step of 16
-// stages the 64 x 16 slice of op(A) and the 16 x 64 slice of op(B) in LDS
-// (zero outside the matrices), then every lane feeds its fragments:
-//   A: row = lane % 16, k = 4 (lane / 16) + i
-//   B: col = lane % 16, k = 4 (lane / 16) + i
+// owns rows 16w..16w+15 as four 16 x 16 MFMA tiles.  Each K step of 32
+// stages the 64 x 32 slice of op(A) and the 32 x 64 slice of op(B) in LDS
+// (zero outside the matrices), then every lane feeds its fragments of the
+// gfx950 16 x 16 x 32 bf16 MFMA, eight k values each:
+//   A: row = lane % 16, k = 8 (lane / 16) + i
+//   B: col = lane % 16, k = 8 (lane / 16) + i
 //   D: row = 4 (lane / 16) + i, col = lane % 16
 __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
-  __shared__ uint16_t As[64][16 + 4];
-  __shared__ uint16_t Bs[16][64 + 4];
+  __shared__ uint16_t As[64][32 + 4];
+  __shared__ uint16_t Bs[32][64 + 4];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   f4 acc[4];
 #pragma unroll
   for (int j = 0; j < 4; ++j) acc[j] = f4{0.f, 0.f, 0.f, 0.f};
-  for (int k0 = 0; k0 < g.K; k0 += 16) {
-    // stage: 1024 elements of each operand, 4 per thread
+  for (int k0 = 0; k0 < g.K; k0 += 32) {
+    // stage: 2048 elements of each operand, 8 per thread
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
+    for (int r = 0; r < 8; ++r) {
       const int e = t + 256 * r;
-      {  // A slice: row e / 16, col e % 16
-        const int mm = e >> 4, kk = e & 15;
+      {  // A slice: row e / 32, col e % 32
+        const int mm = e >> 5, kk = e & 31;
         const int gm = m0 + mm, gk = k0 + kk;
         uint16_t v = 0;
         if (gm < g.M && gk < g.K)
@@ -96,16 +97,16 @@ __global__ __launch_bounds__(256) void gemm_kernel(GemmArgs g) {
       }
     }
     __syncthreads();
-    const int fr = lane & 15, fk = (lane >> 4) * 4;
-    s4 a;
+    const int fr = lane & 15, fk = (lane >> 4) * 8;
+    bf8 a;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) a[i] = (short)As[16 * w + fr][fk + i];
+    for (int i = 0; i < 8; ++i) a[i] = __builtin_bit_cast(__bf16, As[16 * w + fr][fk + i]);
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      s4 b;
+      bf8 b;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) b[i] = (short)Bs[fk + i][16 * j + fr];
-      acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[j], 0, 0, 0);
+      for (int i = 0; i < 8; ++i) b[i] = __builtin_bit_cast(__bf16, Bs[fk + i][16 * j + fr]);
+      acc[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[j], 0, 0, 0);
     }
     __syncthreads();
   }
