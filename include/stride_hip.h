@@ -113,6 +113,12 @@ int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
 int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
 /* Synchronous decode into caller-owned host buffers. */
 int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris);
+/* The decoder's own device outputs, allocated at creation (float32 n_v x 3,
+ * uint32 n_t x 3) and freed by stride_decoder_destroy: a caller that lets
+ * the decoder own its outputs, as ParaDeltaV5GpuDecoder.decode() does
+ * (utils/paradelta_v5_cuda.py:686-712), passes them to stride_decode (or
+ * stride_decode_normalized for PDLT v5). */
+int stride_decoder_outputs(stride_decoder* h, float** d_verts, uint32_t** d_tris);
 /* stride_decode for callers that cannot allocate device memory (code under
  * hipGraph capture, tight serving loops, harnesses that forbid hipMalloc).
  * No counterpart in the reference.  DPR4 streams only; the decoder's device
@@ -212,7 +218,14 @@ typedef struct stride_batch stride_batch;
 
 int stride_batch_create(const uint8_t* const* blobs, const size_t* lens, int32_t n_meshes,
                         stride_batch** out);
+/* stride_batch_create, then stride_batch_set_tune(tune) before any decode
+ * (tune == NULL: the defaults). */
+int stride_batch_create_tuned(const uint8_t* const* blobs, const size_t* lens, int32_t n_meshes,
+                              const stride_tune* tune, stride_batch** out);
 void stride_batch_destroy(stride_batch* h);
+/* The batch's own device outputs (every mesh's vertices, then triangles, in
+ * mesh order; stride_batch_offsets), as stride_decoder_outputs. */
+int stride_batch_outputs(stride_batch* h, float** d_verts, uint32_t** d_tris);
 int stride_batch_sizes(const stride_batch* h, uint64_t* n_v, uint64_t* n_t,
                        uint32_t* n_meshlets);
 /* Per mesh first vertex / first triangle in the concatenated outputs,
@@ -221,7 +234,7 @@ int stride_batch_offsets(const stride_batch* h, uint64_t* v_base, uint64_t* t_ba
 int stride_batch_bytes(const stride_batch* h, uint64_t* stream_bytes, uint64_t* side_bytes);
 int stride_batch_set_tune(stride_batch* h, const stride_tune* t);
 int stride_batch_get_tune(const stride_batch* h, stride_tune* t);
-/* One launch over all meshlets of all meshes. */
+/* One launch of each decode pass over all meshlets of all meshes. */
 int stride_batch_decode(stride_batch* h, float* d_verts, uint32_t* d_tris, void* stream);
 /* Same launch replayed from a captured hipGraph (captured on first use for
  * this output pair and tune). */

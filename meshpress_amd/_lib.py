@@ -63,13 +63,16 @@ SIGNATURES = {
     "stride_decoder_sizes": (C.c_int, [p, P(u64), P(u64), P(u32)]),
     "stride_decode": (C.c_int, [p, p, p, p]),
     "stride_decode_to_host": (C.c_int, [p, p, p]),
+    "stride_decoder_outputs": (C.c_int, [p, P(p), P(p)]),
     "stride_decode_noalloc": (C.c_int, [p, p, p, p]),
     "stride_decoder_bytes": (C.c_int, [p, P(u64), P(u64)]),
     "stride_decoder_header": (C.c_int, [p, p, p]),
     "stride_decoder_set_tune": (C.c_int, [p, P(StrideTune)]),
     "stride_decoder_get_tune": (C.c_int, [p, P(StrideTune)]),
     "stride_batch_create": (C.c_int, [p, p, i32, P(p)]),
+    "stride_batch_create_tuned": (C.c_int, [p, p, i32, P(StrideTune), P(p)]),
     "stride_batch_destroy": (None, [p]),
+    "stride_batch_outputs": (C.c_int, [p, P(p), P(p)]),
     "stride_batch_sizes": (C.c_int, [p, P(u64), P(u64), P(u32)]),
     "stride_batch_offsets": (C.c_int, [p, p, p]),
     "stride_batch_bytes": (C.c_int, [p, P(u64), P(u64)]),

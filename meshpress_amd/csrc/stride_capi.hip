@@ -920,6 +920,13 @@ int stride_decoder_create(const uint8_t* data, size_t len, stride_decoder** out)
 
 void stride_decoder_destroy(stride_decoder* h) { delete h; }
 
+int stride_decoder_outputs(stride_decoder* h, float** d_verts, uint32_t** d_tris) {
+  if (!h || !d_verts || !d_tris) return fail(STRIDE_E_ARG, "null argument");
+  *d_verts = h->v5 ? h->v5->d_v_own : h->set.d_v_own;
+  *d_tris = h->v5 ? h->v5->d_t_own : h->set.d_t_own;
+  return STRIDE_OK;
+}
+
 int stride_dprb_from_dpr4(const uint8_t* data, size_t len, uint8_t* out, size_t cap,
                           size_t* out_len) {
   if (!out_len || (!data && len)) return fail(STRIDE_E_ARG, "null argument");
@@ -1189,7 +1196,27 @@ int stride_batch_create(const uint8_t* const* blobs, const size_t* lens, int32_t
   *out = h.release();
   return STRIDE_OK;
 }
+int stride_batch_create_tuned(const uint8_t* const* blobs, const size_t* lens, int32_t n_meshes,
+                              const stride_tune* tune, stride_batch** out) {
+  if (!out) return fail(STRIDE_E_ARG, "null out");
+  *out = nullptr;
+  stride_batch* h = nullptr;
+  int rc = stride_batch_create(blobs, lens, n_meshes, &h);
+  if (rc) return rc;
+  if (tune && (rc = h->set.set_tune(tune))) {
+    delete h;
+    return rc;
+  }
+  *out = h;
+  return STRIDE_OK;
+}
 void stride_batch_destroy(stride_batch* h) { delete h; }
+int stride_batch_outputs(stride_batch* h, float** d_verts, uint32_t** d_tris) {
+  if (!h || !d_verts || !d_tris) return fail(STRIDE_E_ARG, "null argument");
+  *d_verts = h->set.d_v_own;
+  *d_tris = h->set.d_t_own;
+  return STRIDE_OK;
+}
 int stride_batch_sizes(const stride_batch* h, uint64_t* n_v, uint64_t* n_t, uint32_t* n_meshlets) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
   if (n_v) *n_v = h->set.n_v;

@@ -324,3 +324,79 @@ def test_native_bench_cli(tmp_path):
         r = subprocess.run([str(build.BENCH), str(tmp_path / "c1.dpr4"), *bad],
                            capture_output=True, text=True, timeout=120)
         assert r.returncode == 1 and "set_tune" in r.stderr, bad
+
+
+def _hip_runtime():
+    """The HIP runtime torch loaded, for hipMemcpy from raw device pointers."""
+    import ctypes as C
+    import os
+
+    import torch
+    path = os.path.join(os.path.dirname(torch.__file__), "lib", "libamdhip64.so")
+    H = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+    H.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    H.hipMemcpy.restype = C.c_int
+    return H
+
+
+def _to_host(H, ptr, shape, dtype):
+    out = np.empty(shape, dtype)
+    if out.nbytes:
+        assert H.hipMemcpy(out.ctypes.data, ptr, out.nbytes, 2) == 0   # device to host
+    return out
+
+
+def test_owned_outputs_and_tuned_batch(coracle):
+    """stride_decoder_outputs / stride_batch_outputs hand out the handles' own
+    device outputs, which stride_decode and stride_batch_decode fill;
+    stride_batch_create_tuned applies its launch shape before any decode."""
+    import ctypes as C
+
+    from meshpress_amd import _lib
+    L = _lib.lib()
+    H = _hip_runtime()
+    blobs = [np.load(p)["data"].tobytes() for p in sorted(GOLD.glob("dup_*.npz"))]
+    want = [coracle.decode(b) for b in blobs]
+    for b, (Vo, To) in zip(blobs, want):
+        buf = np.frombuffer(b, np.uint8)
+        h = C.c_void_p()
+        _lib.check(L.stride_decoder_create(buf.ctypes.data, len(buf), C.byref(h)))
+        try:
+            nv, nt, nm = C.c_uint64(), C.c_uint64(), C.c_uint32()
+            _lib.check(L.stride_decoder_sizes(h, C.byref(nv), C.byref(nt), C.byref(nm)))
+            dv, dt = C.c_void_p(), C.c_void_p()
+            _lib.check(L.stride_decoder_outputs(h, C.byref(dv), C.byref(dt)))
+            _lib.check(L.stride_decode(h, dv, dt, None))
+            V = _to_host(H, dv, (nv.value, 3), np.float32)
+            T = _to_host(H, dt, (nt.value, 3), np.uint32)
+            assert_same(V, T, Vo, To)
+        finally:
+            L.stride_decoder_destroy(h)
+    bufs = [np.frombuffer(b, np.uint8) for b in blobs]
+    ptrs = (C.c_void_p * len(bufs))(*[x.ctypes.data for x in bufs])
+    lens = (C.c_size_t * len(bufs))(*[len(x) for x in bufs])
+    tune = _lib.make_tune({**_lib.StrideTune().as_dict(), "waves": 1, "stage_words": 512},
+                          dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=8, tok_group=8))
+    bad = _lib.make_tune(tune.as_dict(), dict(group_size=99))
+    hb = C.c_void_p()
+    assert L.stride_batch_create_tuned(ptrs, lens, len(bufs), C.byref(bad), C.byref(hb)) != 0
+    assert not hb.value
+    _lib.check(L.stride_batch_create_tuned(ptrs, lens, len(bufs), C.byref(tune), C.byref(hb)))
+    try:
+        got = _lib.StrideTune()
+        _lib.check(L.stride_batch_get_tune(hb, C.byref(got)))
+        assert (got.group_size, got.lds_bytes, got.tok_chains, got.tok_group) == (16, 16384, 8, 8)
+        nv, nt, nm = C.c_uint64(), C.c_uint64(), C.c_uint32()
+        _lib.check(L.stride_batch_sizes(hb, C.byref(nv), C.byref(nt), C.byref(nm)))
+        vb = np.zeros(len(bufs) + 1, np.uint64)
+        tb = np.zeros(len(bufs) + 1, np.uint64)
+        _lib.check(L.stride_batch_offsets(hb, vb.ctypes.data, tb.ctypes.data))
+        dv, dt = C.c_void_p(), C.c_void_p()
+        _lib.check(L.stride_batch_outputs(hb, C.byref(dv), C.byref(dt)))
+        _lib.check(L.stride_batch_decode(hb, dv, dt, None))
+        V = _to_host(H, dv, (nv.value, 3), np.float32)
+        T = _to_host(H, dt, (nt.value, 3), np.uint32)
+        for i, (Vo, To) in enumerate(want):
+            assert_same(V[vb[i]:vb[i + 1]], T[tb[i]:tb[i + 1]], Vo, To)
+    finally:
+        L.stride_batch_destroy(hb)
