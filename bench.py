@@ -408,20 +408,21 @@ def learned_encode(head_prep, n_heldout):
     kw = dict(precision_error=bbox_error(12), precision_mode="bbox_frac")
     ex = STRIDEEncoder(**kw)
     le = STRIDEEncoder(selector="learned", **kw)
+    lf = STRIDEEncoder(selector="learned+fallback", **kw)
     specs = workloads.learned_specs(50, 1)[:n_heldout]
     n_cpu = host_threads()
     preps = []
     for fam, n, seed in specs:
         V, T = workloads.mesh_with_tris(fam, n, seed)
         preps.append(prepare(V, T, **kw))
-    for e in (ex, le):                                   # warm-up
+    for e in (ex, le, lf):                               # warm-up
         e.encode_prepared(preps[0])
     rows = []
     for name, prep in [("heightfield_1900_q12", head_prep)] + [(f"{f}_{n}_{sd}", p) for (f, n, sd), p
                                                                 in zip(specs, preps)]:
         r = {"mesh": name, "n_tris": int(prep["n_t"])}
         outs = []
-        for tag, e in (("exhaustive", ex), ("learned", le)):
+        for tag, e in (("exhaustive", ex), ("learned", le), ("learned_fallback", lf)):
             data = e.encode_prepared(prep)
             st = e.last_stats
             r[tag] = {"bytes": len(data), "search_s": st["search_s"],
@@ -430,8 +431,8 @@ def learned_encode(head_prep, n_heldout):
                       "pred_K": [int(x) for x in np.asarray(st["pred_K"]).reshape(-1)]}
             V, T = ParaDeltaV5GpuDecoder(data).decode_to_host()
             outs.append((V.view(np.uint32).copy(), T))
-        r["decode_equal"] = bool(np.array_equal(outs[0][0], outs[1][0])
-                                 and np.array_equal(outs[0][1], outs[1][1]))
+        r["decode_equal"] = all(bool(np.array_equal(outs[0][0], o[0]) and np.array_equal(outs[0][1], o[1]))
+                                for o in outs[1:])
         # the same exhaustive search on the host's threads (encode-side CPU
         # baseline); its choices equal the GPU search's
         cpu = cpu_exhaustive(prep, n_cpu)
@@ -448,18 +449,24 @@ def learned_encode(head_prep, n_heldout):
         log(f"learned {name}: search {ex_['search_s'] * 1e3:.1f} ms (predictor "
             f"{ex_['predictor_s'] * 1e3:.1f} + coders {ex_['coder_s'] * 1e3:.1f}) -> "
             f"{le_['search_s'] * 1e3:.1f} ms ({le_['predictor_s'] * 1e3:.1f} + "
-            f"{le_['coder_s'] * 1e3:.1f}), bytes {ex_['bytes']} -> {le_['bytes']}; "
+            f"{le_['coder_s'] * 1e3:.1f}), bytes {ex_['bytes']} -> {le_['bytes']} "
+            f"(with fallback {r['learned_fallback']['search_s'] * 1e3:.1f} ms, "
+            f"{r['learned_fallback']['bytes']}); "
             f"CPU exhaustive {r['exhaustive_cpu']['search_s'] * 1e3:.1f} ms on {n_cpu} threads "
             f"(same choice: {r['exhaustive_cpu']['same_choice']})")
 
     def agg(rs):
         be = sum(r["exhaustive"]["bytes"] for r in rs)
         bl = sum(r["learned"]["bytes"] for r in rs)
+        bf = sum(r["learned_fallback"]["bytes"] for r in rs)
         te = sum(r["exhaustive"]["search_s"] for r in rs)
         tl = sum(r["learned"]["search_s"] for r in rs)
+        tf = sum(r["learned_fallback"]["search_s"] for r in rs)
         tc = sum(r["exhaustive_cpu"]["search_s"] for r in rs)
         return {"meshes": len(rs), "exhaustive_search_s": te, "learned_search_s": tl,
                 "speedup": te / tl, "regret": (bl - be) / be,
+                "fallback_search_s": tf, "fallback_speedup": te / tf,
+                "fallback_regret": (bf - be) / be,
                 "decode_equal": all(r["decode_equal"] for r in rs),
                 "cpu_exhaustive_search_s": tc, "cpu_threads": n_cpu,
                 "speedup_vs_cpu_exhaustive": tc / tl,

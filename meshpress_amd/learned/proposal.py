@@ -48,7 +48,7 @@ def gather_streams(vals: np.ndarray, soff: np.ndarray, idx: np.ndarray):
 
 class LearnedProposal:
     def __init__(self, mlp: ProposalMLP | None = None, weights=None, top: int = 2,
-                 refine_rounds: int = 1, tau_pred: float = 0.3, tau_coder: float = 0.5):
+                 refine_rounds: int = 1, tau_pred: float = 0.2, tau_coder: float = 0.9):
         if mlp is None:
             path = Path(weights) if weights is not None else WEIGHTS
             if not path.exists():

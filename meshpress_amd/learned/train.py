@@ -127,7 +127,7 @@ def train(X, Y, Xv, Yv, *, epochs, batch, lr, seed=0, progress=True):
     return net, hist
 
 
-def evaluate_encodes(specs, net, workers, tau_pred=0.3, tau_coder=0.5):
+def evaluate_encodes(specs, net, workers, tau_pred=0.2, tau_coder=0.9):
     """Per held-out mesh: bytes, search seconds and decode equality for the
     exhaustive, learned, learned+fallback and baseline encoders."""
     import torch

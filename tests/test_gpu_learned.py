@@ -64,6 +64,8 @@ def test_learned_regret_within_one_percent(encodes):
         assert regret <= 0.01, (k, regret)
         for (d, _), (de, _) in zip(encodes[k], encodes["exhaustive"]):
             assert (len(d) - len(de)) / len(de) <= 0.01
+    # the fallback scores the streams the network is least sure of exactly
+    assert total(encodes, "learned+fallback") <= total(encodes, "learned")
 
 
 def test_learned_streams_decode_like_exhaustive(encodes):
