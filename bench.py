@@ -192,7 +192,10 @@ def single_mesh(name, V, T, bits, steps, warmup, tune=True):
         f"prep {enc_stats['prep_s']:.1f}s search+write {enc_stats['search_write_s']:.1f}s")
     dec = ParaDeltaV5GpuDecoder(data)
     sweep = None
-    if tune:
+    if isinstance(tune, dict):          # a given launch shape (profiling runs)
+        dec.set_tune(**tune)
+        log(f"{name}: tune {tune}")
+    elif tune:
         best, sweep = autotune(dec)
         log(f"{name}: tuned {best}")
     s_wall, s_gpu = time_calls(dec.decode, steps, warmup)
@@ -495,6 +498,9 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline workload only")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the decoders' default launch shape (profiling runs)")
+    ap.add_argument("--tune", default=None, metavar="JSON",
+                    help="launch shape of the headline decoder instead of autotuning it, "
+                         "e.g. '{\"tok_chains\": 8, \"tok_group\": 8}' (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (counter calibration)")
@@ -543,8 +549,9 @@ def main():
         V, T = workloads.icosphere_l8()
     else:
         sys.exit(f"unknown workload {name}")
+    head_tune = json.loads(a.tune) if a.tune else not a.no_autotune
     dec, data, head, head_prep = single_mesh(name, V, T, bits, a.steps, a.warmup,
-                                             tune=not a.no_autotune)
+                                             tune=head_tune)
     details = {}
     cpu = None
     if not a.no_cpu:
