@@ -64,7 +64,9 @@ BENCH_CANDIDATES = ([dict(mode=2, group_size=gs, lds_bytes=lds, grid=0, use_queu
                     + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=32, tok_chains=0, grid=0,
                             use_queue=0) for gs, lds in ((16, 16384), (8, 16384))]
                     # the chain token pass: chains per meshlet x meshlets per token wave,
-                    # axis groups of 16 or 21 meshlets
+                    # axis groups of 8, 16 or 21 meshlets (which pair wins depends on
+                    # the mesh: 8 x 8 with groups of 16 on the 12-bit heightfield, the
+                    # default 16 x 4 with groups of 21 on its 16-bit twin)
                     + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_chains=kc, tok_group=tg,
                             grid=0, use_queue=0)
                        for kc, tg in ((4, 16), (8, 8), (16, 4))
