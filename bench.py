@@ -322,7 +322,10 @@ def batched(n_meshes, steps, warmup, predictor, tune=True, modes=("graph", "laun
     log(f"batch: {n_meshes} meshes, {n_t:,} tris, {bd.n_meshlets:,} meshlets, "
         f"{bd.stream_bytes:,} stream bytes, encode {t_enc:.0f} s")
     sweep = None
-    if tune:
+    if isinstance(tune, dict):          # a given launch shape (profiling runs)
+        bd.set_tune(**tune)
+        log(f"batch: tune {tune}")
+    elif tune:
         best, sweep = autotune(bd, mode_kw={"mode": "graph"})
         log(f"batch: tuned {best}")
     r = {"workload": f"batch{n_meshes}_log_uniform_1k_200k_q12", "n_meshes": n_meshes,
@@ -499,7 +502,8 @@ def main():
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the decoders' default launch shape (profiling runs)")
     ap.add_argument("--tune", default=None, metavar="JSON",
-                    help="launch shape of the headline decoder instead of autotuning it, "
+                    help="launch shape of the headline (or --workload batch) decoder instead "
+                         "of autotuning it, "
                          "e.g. '{\"tok_chains\": 8, \"tok_group\": 8}' (profiling runs)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
@@ -527,7 +531,8 @@ def main():
 
     name = a.workload
     if name == "batch":
-        r = batched(a.batch, a.steps, a.warmup, a.batch_predictor, tune=not a.no_autotune,
+        r = batched(a.batch, a.steps, a.warmup, a.batch_predictor,
+                    tune=json.loads(a.tune) if a.tune else not a.no_autotune,
                     modes=tuple(a.batch_modes.split(",")), cache=a.batch_cache)
         mode = "graph" if "graph" in r else tuple(k for k in ("launch", "per_mesh") if k in r)[0]
         print(json.dumps({
