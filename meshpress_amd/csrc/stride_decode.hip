@@ -758,7 +758,7 @@ __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
   }
 }
 
-template <bool NARROW>
+template <bool NARROW, bool CHECK>
 __global__ __launch_bounds__(64) void stride_axes_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
     const MeshParams* __restrict__ meshes, const GroupEntry* __restrict__ groups,
@@ -826,11 +826,11 @@ __global__ __launch_bounds__(64) void stride_axes_kernel(
     const bool i32 = __all(NARROW && irlp32_exact(n0, n1, n2, K));
     const bool rice = __all(axis_rice(src, bb, e, P, d));
     bool ok;
-    if (i32 && rice) ok = split_axis_g<true, true>(src, bb, e, P, d, wv, codes);
-    else if (i32) ok = split_axis_g<true, false>(src, bb, e, P, d, wv, codes);
-    else if (rice) ok = split_axis_g<false, true>(src, bb, e, P, d, wv, codes);
-    else ok = split_axis_g<false, false>(src, bb, e, P, d, wv, codes);
-    if (!ok && overflow) overflow[g] = 1u;
+    if (i32 && rice) ok = split_axis_g<true, true, CHECK>(src, bb, e, P, d, wv, codes);
+    else if (i32) ok = split_axis_g<true, false, CHECK>(src, bb, e, P, d, wv, codes);
+    else if (rice) ok = split_axis_g<false, true, CHECK>(src, bb, e, P, d, wv, codes);
+    else ok = split_axis_g<false, false, CHECK>(src, bb, e, P, d, wv, codes);
+    if (CHECK && !ok) overflow[g] = 1u;
   }
   __syncthreads();
   // a group lies in one mesh; each meshlet's size and output offset come
@@ -893,12 +893,19 @@ hipError_t launch_group(const GroupLaunch& d, hipStream_t stream) {
       hipError_t e = hipGetLastError();
       if (e != hipSuccess) return e;
     }
-    if (d.narrow)
-      hipLaunchKernelGGL(stride_axes_kernel<true>, dim3(d.n_groups), dim3(64), d.lds_bytes, stream,
-                         d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
+    // range checks only where a caller collects them (the layout's validation decode)
+    if (d.narrow && d.overflow)
+      hipLaunchKernelGGL((stride_axes_kernel<true, true>), dim3(d.n_groups), dim3(64), d.lds_bytes,
+                         stream, d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
+    else if (d.narrow)
+      hipLaunchKernelGGL((stride_axes_kernel<true, false>), dim3(d.n_groups), dim3(64), d.lds_bytes,
+                         stream, d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
+    else if (d.overflow)
+      hipLaunchKernelGGL((stride_axes_kernel<false, true>), dim3(d.n_groups), dim3(64), d.lds_bytes,
+                         stream, d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
     else
-      hipLaunchKernelGGL(stride_axes_kernel<false>, dim3(d.n_groups), dim3(64), d.lds_bytes, stream,
-                         d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
+      hipLaunchKernelGGL((stride_axes_kernel<false, false>), dim3(d.n_groups), dim3(64), d.lds_bytes,
+                         stream, d.buf, d.tab, d.meshes, d.groups, d.walk, d.V, d.overflow);
     return hipGetLastError();
   }
   const uint32_t grid = d.grid ? std::min(d.grid, d.n_groups) : d.n_groups;

@@ -1044,7 +1044,10 @@ STRIDE_DEV uint32_t split_code_words(uint32_t n_local, bool narrow) {
 // leaves the range of CodeT.
 // RICE: both substreams of the lane are Rice coded (the kernels choose it
 // per wave when every lane's are), so codewords take codeword_rice.
-template <bool I32, class Src, class CodeT, bool RICE = false>
+// CHECK: track whether every code fits CodeT (good()).  The layout's
+// validation decode needs it; later decodes of the same stream produce the
+// same codes and skip the two instructions a step it costs.
+template <bool I32, class Src, class CodeT, bool RICE = false, bool CHECK = true>
 struct AxisChain {
   BitStream<Src> sd, sp;
   bool deg, peg, ok;
@@ -1088,7 +1091,7 @@ struct AxisChain {
   }
   STRIDE_DEV void first(uint32_t w0, CodeT* __restrict__ codes) {
     codes[w0 & 0xFFu] = (CodeT)anchor;
-    ok &= code_fits<CodeT>(anchor);
+    if constexpr (CHECK) ok &= code_fits<CodeT>(anchor);
   }
   STRIDE_DEV void step(uint32_t w, CodeT* __restrict__ codes) {
     const bool root = walk_root(w);
@@ -1110,7 +1113,7 @@ struct AxisChain {
       r = unzigzag32(u);
     } else {
       r = root ? codeword_s(sd, deg, dk) : codeword_s(sp, peg, pk);
-      if constexpr (I32 && sizeof(CodeT) == 2) {
+      if constexpr (CHECK && I32 && sizeof(CodeT) == 2) {
         if (r != (int64_t)(int32_t)r) bad |= 0x80000000u;   // no 16-bit code then
       }
     }
@@ -1125,30 +1128,30 @@ struct AxisChain {
       const uint32_t r32 = (uint32_t)r;
       const uint32_t base = (uint32_t)(root ? prev32 : pred);
       const uint32_t c = base + r32;
-      bad |= c | ((base ^ c) & (r32 ^ c) & 0x80000000u);
+      if constexpr (CHECK) bad |= c | ((base ^ c) & (r32 ^ c) & 0x80000000u);
       codes[w & 0xFFu] = (CodeT)c;
       prev32 = (int32_t)c;
     } else {
       const int64_t pred =
           irlp64((int64_t)ca, (int64_t)cb, (int64_t)cc, n0, n1, n2, K);
       const int64_t c = (root ? prev : pred) + r;
-      ok &= code_fits<CodeT>(c);
+      if constexpr (CHECK) ok &= code_fits<CodeT>(c);
       codes[w & 0xFFu] = (CodeT)c;
       prev = c;
     }
   }
   // every code of the chain lies in CodeT
-  STRIDE_DEV bool good() const { return ok && (bad >> 16) == 0u; }
+  STRIDE_DEV bool good() const { return !CHECK || (ok && (bad >> 16) == 0u); }
 };
 
 // The two-pass axis lane: the chain along the meshlet's walk slot in HBM
 // (read four entries ahead).  Returns false when a code leaves CodeT.
-template <bool I32, bool RICE = false, class Src, class CodeT>
+template <bool I32, bool RICE = false, bool CHECK = true, class Src, class CodeT>
 STRIDE_DEV bool split_axis_g(const Src& words, uint64_t base,
                              const MeshletEntry& e, const MeshParams& P, int d,
                              const Walk4* __restrict__ walk, CodeT* __restrict__ codes) {
   const uint32_t n_local = e.n_local;
-  AxisChain<I32, Src, CodeT, RICE> ax;
+  AxisChain<I32, Src, CodeT, RICE, CHECK> ax;
   ax.init(words, base, e, P, d);
   // the walk slot is 16-byte aligned, with readable words past its end
   // (quad (n_local >> 2) + 1 at most is read).  Entries 1..3, then whole
