@@ -912,9 +912,22 @@ STRIDE_DEV int64_t codeword_s(Stream& s, bool eg, int k) {
 // are exact whenever len <= 32, and the value is dropped otherwise.
 // egm: all ones for Exp-Golomb(k), 0 for Rice(k) (a mask, not a bool: the
 // coder choice then costs an and and a bit-field insert, no compares).
+// Leading zeros of t, 32 for t == 0.  v_ffbh_u32 returns all ones for 0, so
+// one min replaces the zero test, which the compiler otherwise emits as a
+// 64-bit compare on the window, a select and a hazard wait.
+STRIDE_DEV uint32_t clz_or_32(uint32_t t) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t q;
+  asm("v_ffbh_u32 %0, %1" : "=v"(q) : "v"(t));
+  return q < 32u ? q : 32u;
+#else
+  return t ? (uint32_t)__builtin_clz(t) : 32u;
+#endif
+}
+
 STRIDE_DEV uint32_t codeword_win(uint64_t W, uint32_t egm, int k, uint32_t& u) {
   const uint32_t T = (uint32_t)(W >> 32);
-  const uint32_t q = T ? (uint32_t)__builtin_clz(T) : 32u;
+  const uint32_t q = clz_or_32(T);
   const uint32_t nb = (q & egm) + (uint32_t)k;
   const uint32_t len = q + 1 + nb;
   const uint32_t x = T >> ((32u - len) & 31u);
@@ -931,7 +944,7 @@ STRIDE_DEV uint32_t codeword_win(uint64_t W, uint32_t egm, int k, uint32_t& u) {
 // whenever len <= 32, and the value is dropped otherwise.
 STRIDE_DEV uint32_t codeword_rice(uint64_t W, int k, uint32_t& u) {
   const uint32_t T = (uint32_t)(W >> 32);
-  const uint32_t q = T ? (uint32_t)__builtin_clz(T) : 32u;
+  const uint32_t q = clz_or_32(T);
   const uint32_t len = q + 1 + (uint32_t)k;
   const uint32_t low = (T >> ((32u - len) & 31u)) & ((1u << (k & 31)) - 1u);
   u = (q << (k & 31)) | low;
