@@ -216,22 +216,32 @@ def single_mesh(name, V, T, bits, steps, warmup, tune=True):
     return dec, data, r, prep
 
 
-# PDLT v5 launch shapes: (mode, group_size, lds_bytes, token-pass LDS words)
-PDLT_LAYOUTS = [(4, 16, 24576, 4096), (4, 8, 24576, 4096), (4, 16, 16384, 4096),
-                (4, 16, 49152, 4096), (4, 16, 24576, 8192), (4, 16, 16384, 8192),
-                (4, 8, 12288, 8192), (3, 16, 49152, 4096), (3, 16, 24576, 4096)]
+# PDLT v5 launch shapes: mode 4 (token pass, then the chain pass over
+# groups of group_size meshlets in lds_bytes) with token lanes (tok_chains 0;
+# stage_words = the token pass's LDS words, tok_group 64) or token chains
+# (about tok_chains a meshlet, tok_group meshlets a wave), or mode 3 (one
+# kernel)
+PDLT_LAYOUTS = ([dict(mode=4, group_size=gs, lds_bytes=lds, stage_words=sw, tok_chains=0,
+                      tok_group=64)
+                 for gs, lds, sw in ((16, 24576, 4096), (8, 24576, 4096), (16, 16384, 4096),
+                                     (16, 49152, 4096), (16, 24576, 8192), (16, 16384, 8192),
+                                     (8, 12288, 8192))]
+                + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_chains=kc, tok_group=tg)
+                   for kc, tg in ((16, 4), (8, 8)) for gs, lds in ((16, 24576), (8, 12288))]
+                + [dict(mode=3, group_size=16, lds_bytes=lds, stage_words=4096, tok_chains=0,
+                        tok_group=64) for lds in (49152, 24576)])
 
 
 def tune_layout(dec):
     """The fastest PDLT_LAYOUTS entry for a v5 decoder (its own chain: the
     float32 chain's smaller LDS blocks favour other group sizes)."""
     sweep = []
-    for mode, gs, lds, sw in PDLT_LAYOUTS:
-        dec.set_tune(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
+    for t in PDLT_LAYOUTS:
+        dec.set_tune(**t)
         _, g = time_calls(dec.decode_world, 10, 3)
-        sweep.append([mode, gs, lds, sw, round(g * 1e3, 4)])
-    mode, gs, lds, sw, _ = min(sweep, key=lambda x: x[-1])
-    best = dict(mode=mode, group_size=gs, lds_bytes=lds, stage_words=sw)
+        sweep.append(dict(t, ms=round(g * 1e3, 4)))
+    best = dict(min(sweep, key=lambda x: x["ms"]))
+    best.pop("ms")
     dec.set_tune(**best)
     return best, sweep
 

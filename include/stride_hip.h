@@ -181,9 +181,14 @@ int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
  *     words per meshlet {256,512,1024}, use_queue = persistent grid pulling
  *     meshlets from a device atomic counter.
  *   grid: 0 = one workgroup per unit of work, else a persistent grid.
- *   PDLT v5 streams: mode 4 (default: token pass, then chain groups of
- *   group_size meshlets in lds_bytes; stage_words = the token pass's LDS
- *   words, tok_group its meshlets per wave) or mode 3 (one kernel).
+ *   PDLT v5 streams: mode 4 (default: the token pass, then chain groups of
+ *   group_size meshlets in lds_bytes) or mode 3 (one kernel).  The token
+ *   pass runs token chains as for DPR4 (default tok_chains 16, tok_group 4)
+ *   or, with tok_chains = 0, one token lane per meshlet (stage_words = its
+ *   LDS words, tok_group <= 64 meshlets per wave).  A stream with a meshlet
+ *   too large for a chain token wave (48 KiB of LDS with its 16-bit token
+ *   records) runs the token lanes, 64 meshlets per wave, and get_tune
+ *   reports tok_chains 0.
  *   get_tune reports in `reserved` the meshlets decoded by mode 0. */
 typedef struct stride_tune {
   int32_t waves;
@@ -195,9 +200,9 @@ typedef struct stride_tune {
   int32_t lds_bytes;
   int32_t reserved;
   int32_t tok_group;   /* mode 4: meshlets per token-pass wave, 1..64 (0: the
-                          default: 4 for DPR4 chain token waves, 32 for one
-                          token lane per meshlet, 64 for PDLT v5) */
-  int32_t tok_chains;  /* DPR4 mode 4: 0 = one token lane per meshlet; 1..16 =
+                          default: 4 for chain token waves, 32 for DPR4 token
+                          lanes, 64 for PDLT v5 token lanes) */
+  int32_t tok_chains;  /* mode 4: 0 = one token lane per meshlet; 1..16 =
                           the chain token pass, each meshlet's strip tokens cut
                           into about tok_chains chains of >= 16 tokens parsed
                           by one lane each (up to 64 chains and tok_group

@@ -21,7 +21,10 @@ struct PdltSet {
   int group_size = 16;
   int lds_budget = 24576;
   int tok_lds_budget = 16384;
-  int tok_group = 64;                   // meshlets per token-pass wave
+  int tok_group = 4;                    // meshlets per token-pass wave
+  int tok_chains = 16;                  // > 0: token chains (about this many a meshlet)
+  bool chain_waves = false;             // tok_groups are chain token waves (pdlt_layout_chains)
+  uint32_t n_chains = 0;                // entries at d_chains
   int chain32 = 0;                      // 1: the encoder's float32 chain
   int two_pass = 1;                     // 1: token pass + chain pass, 0: one kernel
   // device
@@ -29,8 +32,10 @@ struct PdltSet {
   PdltEntry* d_tab = nullptr;
   PdltGroup* d_groups = nullptr;
   PdltGroup* d_tok_groups = nullptr;
+  ChainEntry* d_chains = nullptr;       // parser state at every token chain's start
   uint32_t* d_walk = nullptr;           // two-pass walks (n_v - n_b words)
   uint32_t* d_seen = nullptr;           // two-pass seen sets + walk lengths (kSeenWords per meshlet)
+  uint32_t* d_refs = nullptr;           // two-pass boundary refs (n_refs words)
   double* d_bnd = nullptr;              // boundary table, normalised f64 (n_b x 3)
   float* d_v_own = nullptr;
   uint32_t* d_t_own = nullptr;

@@ -24,6 +24,7 @@ struct PdltHost {
   uint32_t hdr_n_v = 0, hdr_n_t = 0;
   uint32_t n_b = 0, n_m = 0;
   uint64_t n_v = 0, n_t = 0;            // decoded counts
+  uint64_t n_refs = 0;                  // boundary refs over all meshlets
   size_t stream_bytes = 0;
   std::vector<uint32_t> words;          // the stream as little-endian words, zero padded
   std::vector<double> bnd;              // boundary table, normalised f64 (n_b x 3)
@@ -186,7 +187,7 @@ inline int pdlt_parse(const uint8_t* d, size_t len, PdltHost& H, std::string& er
     }
     status[m] = pdlt_scan(src, off[m], end_all, n_b, H.tab[m]);
   });
-  uint64_t iv = n_b, tv = 0;
+  uint64_t iv = n_b, tv = 0, rv = 0;
   for (uint32_t m = 0; m < n_m; ++m) {
     if (status[m]) return scan_status(status[m], m, err);
     if (m + 1 < n_m && H.tab[m].end_bit != off[m + 1]) {
@@ -195,15 +196,18 @@ inline int pdlt_parse(const uint8_t* d, size_t len, PdltHost& H, std::string& er
     }
     H.tab[m].int_base = (uint32_t)iv;
     H.tab[m].t_off = (uint32_t)tv;
+    H.tab[m].ref_base = (uint32_t)rv;
     iv += H.tab[m].n_int;
     tv += H.tab[m].n_tris;
-    if (iv > 0xFFFFFFFFull || 3 * tv > 0xFFFFFFFFull) {
+    rv += H.tab[m].n_bnd;
+    if (iv > 0xFFFFFFFFull || 3 * tv > 0xFFFFFFFFull || rv > 0xFFFFFFFFull) {
       err = "more than 2^32 output values";
       return STRIDE_E_UNSUPPORTED;
     }
   }
   H.n_v = iv;
   H.n_t = tv;
+  H.n_refs = rv;
   PdltParams& P = H.params;
   for (int i = 0; i < 3; ++i) P.w[i] = (double)H.lin3[i];
   P.delta = 2.0 * (double)H.pce;
@@ -288,6 +292,79 @@ inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, int gmax,
     m = j;
   }
   if (max_lds > 65536) { err = "a meshlet needs more than 64 KiB of LDS"; return STRIDE_E_UNSUPPORTED; }
+  lds = max_lds;
+  return STRIDE_OK;
+}
+
+// Chain token waves of the two-pass decode (pdlt_chain_tokens_kernel):
+// consecutive meshlets, up to gmax meshlets and 64 token chains (about k0 a
+// meshlet, pdlt_chain_tokens) a wave, whose LDS fits kPdltChainLds bytes:
+// the staged bits from the first meshlet header to the end of the last
+// token section (+ 3 words: the refs' read-ahead window), the refs (n_bnd
+// words a meshlet), the 16-bit token records (each meshlet's padded to an
+// even count), 16 bytes of final FIFO per chain and 256 first-sighting
+// words.  Sets each entry's tok_ref_off, each group's refs base (blk_base),
+// records base (seen_base) and chain range [pad[0], pad[1]), and writes the
+// parser state at every cut to `chains` (stream = the stream's words).
+// STRIDE_E_UNSUPPORTED when a meshlet alone does not fit (its tokens then
+// need the token lanes).
+constexpr uint32_t kPdltChainLds = 49152;
+inline int pdlt_layout_chains(std::vector<PdltEntry>& tab, const std::vector<uint32_t>& stream,
+                              int k0, int gmax, std::vector<PdltGroup>& groups,
+                              std::vector<ChainEntry>& chains, uint32_t& lds, std::string& err) {
+  if (k0 < 1 || k0 > kMaxChains) { err = "tok_chains must be in [0, 16]"; return STRIDE_E_ARG; }
+  if (gmax < 1 || gmax > 64) { err = "tok_group must be in [1, 64]"; return STRIDE_E_ARG; }
+  groups.clear();
+  const uint32_t n_m = (uint32_t)tab.size();
+  std::vector<uint32_t> cbase(n_m, 0);
+  auto need = [](uint64_t w, uint64_t refs, uint64_t recw) {
+    return 4 * (w + refs + recw) + 16 * 64 + 4 * 256;
+  };
+  uint32_t max_lds = 0, nc = 0, m = 0;
+  while (m < n_m) {
+    PdltGroup g{};
+    g.m0 = m;
+    g.w_lo = tab[m].start_bit >> 5;
+    uint32_t refs = 0, recw = 0, words = 0, cn = 0, j = m;
+    while (j < n_m && j - m < (uint32_t)gmax) {
+      const PdltEntry& e = tab[j];
+      const uint32_t nt = pdlt_tokens_of(e);
+      const uint32_t cnt = chain_count(nt, pdlt_chain_tokens(e, (uint32_t)k0));
+      const uint32_t w = std::max(words, ((pdlt_tokens_end(e) + 31) >> 5) + 3 - g.w_lo);
+      const uint32_t rw = ((nt + 1) & ~1u) / 2;
+      if (need(w, refs + e.n_bnd, recw + rw) > kPdltChainLds || cn + cnt > 64) {
+        if (j == m) {
+          err = "meshlet " + std::to_string(j) + " too large for the chain token pass";
+          return STRIDE_E_UNSUPPORTED;
+        }
+        break;
+      }
+      tab[j].tok_ref_off = refs;
+      cbase[j] = nc + cn;
+      refs += e.n_bnd;
+      recw += rw;
+      words = w;
+      cn += cnt;
+      ++j;
+    }
+    g.m1 = j;
+    g.n_words = words;
+    g.blk_base = words;
+    g.seen_base = words + refs;
+    g.pad[0] = nc;
+    g.pad[1] = nc + cn;
+    nc += cn;
+    max_lds = std::max(max_lds, (uint32_t)need(words, refs, recw));
+    groups.push_back(g);
+    m = j;
+  }
+  chains.assign(nc, ChainEntry{0u, 0u});
+  const GlobalWords src{stream.data(), (uint32_t)stream.size()};
+  pdlt_detail::par_for(n_m, [&](int64_t i) {
+    const PdltEntry& e = tab[i];
+    chain_starts_at(src, e.strips_bit, e.strips_bit, pdlt_tokens_of(e), e.n_bnd + e.n_int,
+                    pdlt_chain_tokens(e, (uint32_t)k0), chains.data() + cbase[i]);
+  });
   lds = max_lds;
   return STRIDE_OK;
 }

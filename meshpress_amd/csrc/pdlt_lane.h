@@ -50,7 +50,8 @@ struct PdltEntry {
   uint32_t t_off;        // first output triangle
   uint32_t blk_off;      // LDS block, 32-bit words from the group's block base
   uint32_t tok_ref_off;  // two-pass layout: refs in the token group's LDS, words
-  uint32_t pad[2];
+  uint32_t ref_base;     // two-pass layout: first word of its refs in the refs buffer
+  uint32_t pad;
 };
 static_assert(sizeof(PdltEntry) == 64, "PdltEntry is one 64-byte line");
 
@@ -68,12 +69,36 @@ static_assert(sizeof(PdltGroup) == 32, "PdltGroup is 32 bytes");
 // Two-pass layout: the walk of meshlet m (n_int words and one spare word,
 // which the token lane's unconditional entry stores may reach) goes to the
 // walk buffer at pdlt_walk_off; its seen set (8 words) and walk length
-// (1 word) to the seen buffer at kSeenWords * m.
+// (1 word) to the seen buffer at kSeenWords * m; its boundary refs (n_bnd
+// words) to the refs buffer at ref_base (the meshlets' refs back to back).
 constexpr uint32_t kSeenWords = 9;
 
 // Walk buffer words: n_int + n_meshlets.
 STRIDE_DEV uint64_t pdlt_walk_off(uint32_t int_base, uint32_t m, uint32_t n_boundary) {
   return (uint64_t)(int_base - n_boundary) + m;
+}
+
+// Strip tokens of a meshlet (a strip of length L has max(L, 1) triangles
+// and two more tokens).
+STRIDE_DEV uint32_t pdlt_tokens_of(const PdltEntry& e) {
+  return e.n_tris + 2 * (e.n_strips & kStripMask);
+}
+
+// Chain token pass: meshlets of up to kPdltChainMaxTokens tokens are cut
+// into about k0 chains (chain_tokens), so a cut's strip token index (9 bits)
+// and strip token count (10 bits) fit a ChainEntry; longer ones stay one
+// chain.  Returns the tokens per chain.
+constexpr uint32_t kPdltChainMaxTokens = 511;
+STRIDE_DEV uint32_t pdlt_chain_tokens(const PdltEntry& e, uint32_t k0) {
+  const uint32_t n = pdlt_tokens_of(e);
+  if (n > kPdltChainMaxTokens) return n;
+  return chain_tokens(n, e.n_bnd + e.n_int, e.n_strips, k0);
+}
+
+// One past the last strip-token bit of a meshlet: its residual section, or
+// its end when it has no interior vertex.
+STRIDE_DEV uint32_t pdlt_tokens_end(const PdltEntry& e) {
+  return e.n_int ? e.axis_bit[0] : e.end_bit;
 }
 
 // Decode constants, widened to float64 as the reference decoder does.

@@ -6,11 +6,13 @@
 //
 //   stride_bench FILE [--iters N] [--warmup W] [--mode M] [--group G]
 //                [--lds BYTES] [--grid R] [--queue 0|1] [--stage WORDS] [--tok N]
-//                [--chain 64|32]
+//                [--chains K] [--chain 64|32]
 //
-// --tok sets the meshlets per token-pass wave of mode 4.  PDLT v5 uses
-// --mode (3 one kernel, 4 token pass + chain pass), --group, --lds, --stage
-// (the token pass's LDS words), --tok and --chain (float64 or float32 chain).
+// --tok sets the meshlets per token-pass wave of mode 4, --chains the token
+// chains a meshlet is cut into (0: one token lane per meshlet).  PDLT v5
+// uses --mode (3 one kernel, 4 token pass + chain pass), --group, --lds,
+// --stage (the token lanes' LDS words), --tok, --chains and --chain
+// (float64 or float32 chain).
 #include <hip/hip_runtime.h>
 
 #include <cstdio>

@@ -1159,11 +1159,19 @@ int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t) {
     if (t->mode != 0 && t->mode != 3 && t->mode != 4)
       return fail(STRIDE_E_ARG, "PDLT v5 modes: 3 (one kernel) or 4 (token pass + chain pass)");
     const int tok = t->stage_words ? 4 * t->stage_words : h->v5->tok_lds_budget;
-    const int tg = t->tok_group ? t->tok_group : 64;
+    const int kc = t->tok_chains < 0 ? h->v5->tok_chains : t->tok_chains;
+    if (kc < 0 || kc > kMaxChains) return fail(STRIDE_E_ARG, "tok_chains must be in [0, 16]");
+    const int tg = t->tok_group ? t->tok_group : (kc > 0 ? 4 : 64);
     if (tg < 1 || tg > 64) return fail(STRIDE_E_ARG, "tok_group must be in [1, 64]");
+    const int prev_tg = h->v5->tok_group, prev_kc = h->v5->tok_chains;
     h->v5->tok_group = tg;
+    h->v5->tok_chains = kc;
     const int rc = h->v5->set_layout(t->group_size, t->lds_bytes, tok);
-    if (rc) return rc;
+    if (rc) {
+      h->v5->tok_group = prev_tg;
+      h->v5->tok_chains = prev_kc;
+      return rc;
+    }
     if (t->mode) h->v5->two_pass = t->mode == 4;
     return STRIDE_OK;
   }
@@ -1179,6 +1187,7 @@ int stride_decoder_get_tune(const stride_decoder* h, stride_tune* t) {
     t->lds_bytes = h->v5->lds_budget;
     t->stage_words = h->v5->tok_lds_budget / 4;
     t->tok_group = h->v5->tok_group;
+    t->tok_chains = h->v5->chain_waves ? h->v5->tok_chains : 0;
     return STRIDE_OK;
   }
   h->set.get_tune(t);

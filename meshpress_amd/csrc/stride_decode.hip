@@ -18,6 +18,7 @@
 
 #include "stride_bits.h"
 #include "stride_lane.h"
+#include "stride_wave.h"
 #include "stride_decode.h"
 #include "stride_device.h"
 
@@ -53,11 +54,6 @@ struct WaveLds {
   int32_t anchor[4];
 };
 
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
 
 // Phase A (lane 0): strips + FIFO -> local triangles and the walk.
 template <class Src, int SW>
@@ -355,17 +351,6 @@ __global__ __launch_bounds__(64 * WAVES) void stride_decode_kernel(
 // Group decode: one wave per group of consecutive meshlets, one lane per
 // meshlet; the per-lane decoder is decode_lane (stride_lane.h).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ uint32_t wave_excl_scan(uint32_t x, uint32_t& total) {
-  const int lane = threadIdx.x & 63;
-  uint32_t s = x;
-#pragma unroll
-  for (int o = 1; o < 64; o <<= 1) {
-    uint32_t t = __shfl_up(s, o, 64);
-    if (lane >= o) s += t;
-  }
-  total = __shfl(s, 63, 64);
-  return s - x;
-}
 
 __global__ __launch_bounds__(64) void stride_decode_group_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -567,40 +552,6 @@ __global__ void stride_chain_scan_kernel(const uint8_t* __restrict__ buf,
   chain_starts(src, e.start_bit, e, k0, out + cb);
 }
 
-__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
-  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
-                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
-}
-
-// Cross-lane moves by DPP (GFX9 data-parallel primitives: no LDS traffic).
-// update_dpp(old, src, ctrl, row_mask, bank_mask, bound_ctrl) returns src
-// from the lane that ctrl names, or `old` where there is none.
-//   0x111..0x118 row_shr:1..8, 0x142 row_bcast:15, 0x143 row_bcast:31,
-//   0x138 wave_shr:1 (lane i takes lane i - 1, lane 0 takes `old`).
-__device__ __forceinline__ int wave_shr1(int x, int lane0) {
-  return __builtin_amdgcn_update_dpp(lane0, x, 0x138, 0xF, 0xF, false);
-}
-// Inclusive max over lanes 0..lane of values >= 0 (0 = none): lanes with no
-// source read 0 (bound_ctrl), so each step is one max with a DPP operand.
-__device__ __forceinline__ uint32_t wave_incl_max(uint32_t x) {
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, true));
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, true));
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, true));
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, true));
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));
-  x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));
-  return x;
-}
-
-struct RecSink {
-  uint16_t* __restrict__ r;
-  __device__ __forceinline__ void operator()(uint32_t t, uint32_t rec) const { r[t] = (uint16_t)rec; }
-};
-
-template <bool kFind>
-__device__ __forceinline__ void chain_run(ChainLane<LdsCursor>& cl, LdsCursor& cur, RecSink& sink) {
-  while (!cl.done()) cl.template step<kFind>(cur, sink);
-}
 
 __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
     const uint8_t* __restrict__ buf, const MeshletEntry* __restrict__ tab,
@@ -667,29 +618,7 @@ __global__ __launch_bounds__(64) void stride_chain_tokens_kernel(
   // phase 2 is one wave: its LDS hand-offs need ordering, not a workgroup
   // barrier (which would also wait for the global stores in flight)
   wave_sync();
-  // FIFO placeholders, four meshlets at a time (16 lanes each, one per
-  // rank): the true FIFO at the start of chain c + 1 replaces chain c's
-  // final FIFO; its placeholders stand for entries of the true FIFO at the
-  // start of chain c (chain 0's final FIFO holds local ids only)
-  for (uint32_t j0 = 0; j0 < nmesh; j0 += 4) {
-    const uint32_t jm = j0 + ((uint32_t)lane >> 4);
-    const uint32_t js = jm < nmesh ? jm : j0;
-    const uint32_t mnl = __shfl(e.n_local, (int)js, 64);
-    const uint32_t mcb = __shfl(cfirst, (int)js, 64);
-    const uint32_t mmc = jm < nmesh ? __shfl(nchain, (int)js, 64) : 0u;
-    uint32_t cmax = 0;
-    for (uint32_t q = j0; q < j0 + 4 && q < nmesh; ++q)
-      cmax = max(cmax, (uint32_t)__builtin_amdgcn_readlane((int)nchain, (int)q));
-    const uint32_t rk = (uint32_t)lane & 15u;
-    for (uint32_t c = 1; c + 1 < cmax; ++c) {
-      if (c + 1 < mmc) {
-        const uint32_t x = fb[16 * (mcb + c) + rk];
-        fb[16 * (mcb + c) + rk] =
-            x >= mnl ? fb[16 * (mcb + c - 1) + ((x - mnl) & 15u)] : (uint8_t)x;
-      }
-      wave_sync();
-    }
-  }
+  resolve_chain_fifos(fb, nmesh, e.n_local, cfirst, nchain);
   for (uint32_t j = 0; j < nmesh; ++j) {
     const uint32_t nl = (uint32_t)__builtin_amdgcn_readlane((int)e.n_local, (int)j);
     const uint32_t nt = (uint32_t)__builtin_amdgcn_readlane((int)ntok, (int)j);

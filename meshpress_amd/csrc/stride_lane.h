@@ -540,31 +540,32 @@ STRIDE_DEV uint32_t chain_end(const ChainEntry& c, uint32_t n_tok, uint32_t lm) 
   return t0 + lm < n_tok ? t0 + lm : n_tok;
 }
 
-// The parser state at every chain cut of meshlet e (its header at bit
-// `start` of src), written to out[0 .. chain_count).  The stream was
-// validated by the index pre-pass.
+// The parser state at every cut of a meshlet's n_tok strip tokens, cut
+// every lm tokens (the first at bit tok_bit of src; ChainEntry.pos counts
+// from bit `base`), written to out[0 .. chain_count).  A strip of length 0
+// has three root tokens, as one of length 1 (PDLT v5 allows it; DPR4's
+// pre-pass rejects it).  The stream was validated by the pre-pass.
 template <class Src>
-STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry& e,
-                             uint32_t k0, ChainEntry* __restrict__ out) {
-  const uint32_t n_tok = meshlet_tokens(e);
-  const uint32_t lm = chain_tokens(n_tok, e.n_local, e.n_strips, k0);
+STRIDE_DEV void chain_starts_at(const Src& src, uint32_t base, uint32_t tok_bit, uint32_t n_tok,
+                                uint32_t n_local, uint32_t lm, ChainEntry* __restrict__ out) {
   BitReader<Src> r;
-  r.init(src, start + 48);
+  r.init(src, tok_bit);
   int ib = 1;
-  while ((1u << ib) < e.n_local + 1) ++ib;
+  while ((1u << ib) < n_local + 1) ++ib;
   Fifo fifo;
   fifo.clear();
   uint32_t j = 0, lp2 = 0, c = 0, next = 0;
   for (uint32_t t = 0; t < n_tok; ++t) {
     if (t == next) {
       ChainEntry ce;
-      ce.pos = (r.pos - start) | (t << 16);
+      ce.pos = (r.pos - base) | (t << 16);
       ce.state = j | (lp2 << 9) | (fifo.n << 19) | (c << 24);
       next = chain_end(ce, n_tok, lm);
       out[c++] = ce;
     }
     if (j == lp2) {
-      lp2 = r.bits(16) + 2u;
+      const uint32_t L = r.bits(16);
+      lp2 = (L ? L : 1u) + 2u;
       j = 0;
     }
     if (j >= 3) r.bits(1);
@@ -575,13 +576,22 @@ STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry&
     }
     ++j;
   }
-  if (n_tok == 0) {   // not produced by a valid stream; one empty chain
-    out[0] = ChainEntry{0u, 0u};
-  }
+  if (n_tok == 0) out[0] = ChainEntry{0u, 0u};   // one empty chain
+}
+
+// The chain cuts of DPR4 meshlet e (its header at bit `start` of src; the
+// tokens follow the 48-bit header).
+template <class Src>
+STRIDE_DEV void chain_starts(const Src& src, uint32_t start, const MeshletEntry& e,
+                             uint32_t k0, ChainEntry* __restrict__ out) {
+  const uint32_t n_tok = meshlet_tokens(e);
+  chain_starts_at(src, start, start + 48, n_tok, e.n_local,
+                  chain_tokens(n_tok, e.n_local, e.n_strips, k0), out);
 }
 
 // Phase 1: the lane of one chain, over a Cursor (LdsCursor in the kernel).
-template <class Cursor>
+// kZeroLen: strips of length 0 occur (PDLT v5) and take three tokens.
+template <class Cursor, bool kZeroLen = false>
 struct ChainLane {
   uint32_t t, t1;       // next token, one past the chain's last
   uint32_t j, lp2;      // token index in the strip, tokens in the strip (length + 2)
@@ -625,7 +635,8 @@ struct ChainLane {
   STRIDE_DEV void step(Cursor& cur, Sink& sink) {
     uint32_t W = Cursor::window(Wn);
     const bool start = j == lp2;
-    lp2 = start ? (W >> 16) + 2u : lp2;
+    const uint32_t L = W >> 16;
+    lp2 = start ? (kZeroLen && L == 0 ? 1u : L) + 2u : lp2;
     W = start ? (W << 16) : W;
     j = start ? 0u : j;
     const bool root = j < 3;
@@ -1024,6 +1035,30 @@ __device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst,
     for (uint32_t u = 0; u < kStageDepth; ++u) {
       const uint32_t i = i0 + u * nthr + tid;
       if (i < n) dst[i] = __builtin_bswap32(t[u]);
+    }
+  }
+}
+#endif
+
+#if defined(__HIPCC__)
+// dst[i] = src[i] for i = first, first + step, ... < n, kStageDepth loads in
+// flight before the stores (as stage_words; a load-then-store loop waits a
+// memory round trip per element).  Slots past n re-read element i0, in
+// bounds, and store nothing.
+__device__ __forceinline__ void copy_strided(uint32_t* __restrict__ dst,
+                                             const uint32_t* __restrict__ src, uint32_t n,
+                                             uint32_t first, uint32_t step) {
+  for (uint32_t i0 = first; i0 < n; i0 += kStageDepth * step) {
+    uint32_t t[kStageDepth];
+#pragma unroll
+    for (uint32_t u = 0; u < kStageDepth; ++u) {
+      const uint32_t i = i0 + u * step;
+      t[u] = src[i < n ? i : i0];
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < kStageDepth; ++u) {
+      const uint32_t i = i0 + u * step;
+      if (i < n) dst[i] = t[u];
     }
   }
 }

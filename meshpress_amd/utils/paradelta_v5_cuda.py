@@ -116,7 +116,8 @@ class ParaDeltaV5GpuDecoder:
     def set_tune(self, **kw) -> None:
         """Launch shape: mode (4 = two passes, 2 = split groups, 1 =
         lane-per-meshlet groups, 0 = wave per meshlet; 3 or 4 for PDLT v5),
-        group_size, lds_bytes, tok_group, waves, stage_words, grid, use_queue."""
+        group_size, lds_bytes, tok_group, tok_chains, waves, stage_words, grid,
+        use_queue."""
         t = _lib.make_tune(self.get_tune(), kw)
         _lib.check(self._L.stride_decoder_set_tune(self._h, C.byref(t)))
 

@@ -14,13 +14,20 @@ pytestmark = [pytest.mark.gpu,
               pytest.mark.skipif(not gpu_available(), reason="needs a ROCm GPU")]
 
 GOLD = sorted((Path(__file__).resolve().parent / "golden").glob("pdlt_*.npz"))
-# mode 4: token pass + chain pass (default); mode 3: one kernel.  stage_words
-# is the token pass's LDS budget in words.
-LAYOUTS = [dict(mode=4, group_size=16, lds_bytes=32768, stage_words=8192),
-           dict(mode=3, group_size=16, lds_bytes=32768),
-           dict(mode=4, group_size=1, lds_bytes=4096, stage_words=1024),
-           dict(mode=3, group_size=5, lds_bytes=16384),
-           dict(mode=4, group_size=16, lds_bytes=65536, stage_words=16384)]
+# mode 4: token pass + chain pass (default); mode 3: one kernel.  Token lanes
+# (tok_chains 0): stage_words is the token pass's LDS budget in words.
+LANES = dict(tok_chains=0, tok_group=64)
+LAYOUTS = [dict(mode=4, group_size=16, lds_bytes=32768, stage_words=8192, **LANES),
+           dict(mode=3, group_size=16, lds_bytes=32768, **LANES),
+           dict(mode=4, group_size=1, lds_bytes=4096, stage_words=1024, **LANES),
+           dict(mode=3, group_size=5, lds_bytes=16384, **LANES),
+           dict(mode=4, group_size=16, lds_bytes=65536, stage_words=16384, **LANES)]
+# mode 4 with token chains: about tok_chains chains a meshlet, tok_group
+# meshlets (and up to 64 chains) a token wave
+CHAIN_LAYOUTS = [dict(mode=4, group_size=16, lds_bytes=24576, tok_chains=16, tok_group=4),
+                 dict(mode=4, group_size=8, lds_bytes=16384, tok_chains=4, tok_group=16),
+                 dict(mode=4, group_size=16, lds_bytes=24576, tok_chains=1, tok_group=64),
+                 dict(mode=4, group_size=5, lds_bytes=16384, tok_chains=7, tok_group=3)]
 
 
 def decoder(data, chain="f64", **tune):
@@ -43,12 +50,17 @@ def test_golden_streams_all_layouts(path):
     data = np.load(path)["data"].tobytes()
     print(f"pdlt golden {path.stem}", flush=True)
     Vo, To = PO.decode_v5(data)
-    for tune in LAYOUTS:
+    dec = decoder(data)
+    assert (dec.get_tune()["tok_chains"], dec.get_tune()["tok_group"]) == (16, 4)   # the default
+    assert_same(*dec.decode_to_host(), Vo, To)
+    for tune in LAYOUTS + CHAIN_LAYOUTS:
         dec = decoder(data, **tune)
         assert dec.version == 5
+        assert dec.get_tune()["tok_chains"] == tune["tok_chains"]
         assert_same(*dec.decode_to_host(), Vo, To)
     V32, _ = PO.decode_v5(data, chain="f32")
     assert_same(*decoder(data, chain="f32").decode_to_host(), V32, To)
+    assert_same(*decoder(data, chain="f32", **CHAIN_LAYOUTS[0]).decode_to_host(), V32, To)
 
 
 def test_normalised_decode_matches_reference_semantics():
@@ -73,8 +85,45 @@ def test_fresh_streams():
     for i, (V, T, kw) in enumerate(cases):
         print(f"pdlt fresh case {i} n_t={len(T)}", flush=True)
         data = encode_v5(V, T, **kw)
-        assert_same(*decoder(data).decode_to_host(), *PO.decode_v5(data))
+        ref = PO.decode_v5(data)
+        assert_same(*decoder(data).decode_to_host(), *ref)
         assert_same(*decoder(data, chain="f32").decode_to_host(), *PO.decode_v5(data, chain="f32"))
+        for tune in (dict(mode=4, **LANES), CHAIN_LAYOUTS[1], CHAIN_LAYOUTS[3]):
+            assert_same(*decoder(data, **tune).decode_to_host(), *ref)
+
+
+def test_zero_length_strips_on_every_token_pass():
+    """Strip length 0 (one triangle, as length 1) through the token lanes,
+    the token chains and the single kernel."""
+    from meshpress_amd import workloads as w
+    from meshpress_amd.encoder.paradelta_v5 import encode_v5
+    from tests.test_pdlt_host import strip_lengths
+    V, T = w.displaced_torus(40, 16, 1.0, 0.3, 0.05, 4, 5)
+    data = bytearray(encode_v5(V, T, max_verts=64, max_tris=96))
+    ref = PO.decode_v5(bytes(data))
+    for at, L in strip_lengths(bytes(data)):
+        if L == 1:
+            data[(at + 15) >> 3] &= ~(1 << (7 - ((at + 15) & 7))) & 0xFF
+    data = bytes(data)
+    assert [L for _, L in strip_lengths(data)].count(0) > 0
+    for tune in (dict(mode=3), dict(mode=4, **LANES),
+                 dict(mode=4, tok_chains=2, tok_group=8), dict(mode=4, tok_chains=16, tok_group=4)):
+        assert_same(*decoder(data, **tune).decode_to_host(), *ref)
+
+
+def test_chain_tune_round_trip_and_errors():
+    data = np.load(GOLD[-1])["data"].tobytes()
+    dec = decoder(data, **CHAIN_LAYOUTS[0])
+    t = dec.get_tune()
+    assert (t["tok_chains"], t["tok_group"]) == (16, 4)
+    with pytest.raises(ValueError, match="tok_chains"):
+        dec.set_tune(tok_chains=17)
+    assert dec.get_tune() == t            # a refused tune leaves the layout as it was
+    dec.set_tune(tok_chains=-1, group_size=8)   # -1 keeps the chains
+    assert (dec.get_tune()["tok_chains"], dec.get_tune()["group_size"]) == (16, 8)
+    dec.set_tune(tok_chains=0, tok_group=0)
+    assert (dec.get_tune()["tok_chains"], dec.get_tune()["tok_group"]) == (0, 64)
+    assert_same(*dec.decode_to_host(), *PO.decode_v5(data))
 
 
 def test_repeated_decode_is_stable():
