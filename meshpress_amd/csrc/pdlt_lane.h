@@ -362,20 +362,20 @@ STRIDE_DEV void pdlt_tokens(const uint32_t* __restrict__ st, int64_t rel, const 
 // the walk.  rc = the axis-d reconstruction array of the meshlet, whose
 // boundary entries are filled.
 //
-// P.chain32 selects the arithmetic: 0 = float64, as decode_paradelta_v5
-// (encoder/paradelta_v5.py:422-446); 1 = the float32 chain the encoder
+// T selects the arithmetic (the decoder runs T = float exactly when
+// P.chain32 = 1): double = float64, as decode_paradelta_v5
+// (encoder/paradelta_v5.py:422-446); float = the float32 chain the encoder
 // computed its codes against (encoder/paradelta_v5_nb.py:44-59,113-176:
 // float32 boundary positions, float32 centroid, float32 LIN3 terms summed
 // left to right, code * float32 delta), so every position is the encoder's
-// own reconstruction; T is float for it (the values are float32 numbers), or
-// double.
+// own reconstruction.
 template <class T>
 STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const PdltEntry& e, int d,
                           const PdltParams& P, T* __restrict__ rc,
                           const uint32_t* __restrict__ walk, uint32_t n_emit) {
   if (n_emit == 0) return;   // no residual section is read (n_int may be 0)
   const uint32_t n_bnd = e.n_bnd;
-  const bool c32 = P.chain32 != 0;
+  constexpr bool c32 = sizeof(T) == 4;
   double fb;
   if (c32) {
     float s = 0.f;
@@ -413,41 +413,45 @@ STRIDE_DEV void pdlt_axis(const uint32_t* __restrict__ st, int64_t rel, const Pd
   BitStream<LdsStaged> bs;
   bs.init(LdsStaged{st, 0}, p);
   const bool eg = tag != 1;
+  // A step reads its three reconstructions first, then the next walk entry
+  // (walk[n_emit] lies in the block: the seen set follows the walk), then
+  // decodes its code, so the reads' latency passes under the decode; a
+  // strip root's entry names v three times, whose reads are not used (a
+  // select, not a branch around them).  The window refills without a
+  // branch.
+  uint32_t ent = walk[0];
   for (uint32_t i = 0; i < n_emit; ++i) {
+    const uint32_t v = ent & 0xFFu, a = (ent >> 8) & 0xFFu;
+    const uint32_t b = (ent >> 16) & 0xFFu, c = ent >> 24;
+    const T xa = rc[a], xb = rc[b], xc = rc[c];
+    ent = walk[i + 1];
     int32_t code;
     if (tag == 0) {
       code = (int32_t)((int64_t)mn + (int64_t)(bw ? (uint32_t)(bs.win >> (64 - bw)) : 0u));
-      bs.drop(bw);
+      bs.drop_nb(bw);
     } else {
       uint32_t u;
       const uint32_t len = codeword_win(bs.win, eg ? ~0u : 0u, (int)k, u);
       if (len) {
-        bs.drop(len);
+        bs.drop_nb(len);
         code = (int32_t)((u >> 1) ^ (0u - (u & 1u)));
       } else {
         code = (int32_t)codeword_s(bs, eg, (int)k);
       }
     }
-    const uint32_t ent = walk[i];
-    const uint32_t v = ent & 0xFFu, a = (ent >> 8) & 0xFFu;
-    const uint32_t b = (ent >> 16) & 0xFFu, c = ent >> 24;
-    if (c32) {
-      float pred = (float)fb;
-      if (a != v) {
-        const float t0 = w0f * (float)rc[a];
-        const float t1 = w1f * (float)rc[b];
-        const float t2 = w2f * (float)rc[c];
-        pred = (t0 + t1) + t2;
-      }
+    if constexpr (c32) {
+      const float t0 = w0f * (float)xa;
+      const float t1 = w1f * (float)xb;
+      const float t2 = w2f * (float)xc;
+      const float lin = (t0 + t1) + t2;
+      const float pred = a != v ? lin : (float)fb;
       rc[v] = (T)(pred + (float)code * deltaf);
     } else {
-      double pred = fb;
-      if (a != v) {
-        const double t0 = w0 * (double)rc[a];
-        const double t1 = w1 * (double)rc[b];
-        const double t2 = w2 * (double)rc[c];
-        pred = (t0 + t1) + t2;
-      }
+      const double t0 = w0 * (double)xa;
+      const double t1 = w1 * (double)xb;
+      const double t2 = w2 * (double)xc;
+      const double lin = (t0 + t1) + t2;
+      const double pred = a != v ? lin : fb;
       rc[v] = (T)(pred + (double)code * delta);
     }
   }

@@ -849,6 +849,22 @@ struct BitStream {
     valid -= n;
     fill();
   }
+  // fill() and drop() without a branch: the word after the read-ahead one
+  // is read on every call (cheap from LDS; it lies no further than the next
+  // fill() would read) and taken in only when the window has room
+  STRIDE_DEV void fill_nb() {
+    const bool f = valid <= 32;
+    const uint32_t nx = src(next);
+    win |= f ? (uint64_t)ahead << (32 - valid) : 0ull;
+    valid += f ? 32u : 0u;
+    ahead = f ? nx : ahead;
+    next += f ? 1u : 0u;
+  }
+  STRIDE_DEV void drop_nb(uint32_t n) {
+    win = n < 64 ? (win << n) : 0ull;
+    valid -= n;
+    fill_nb();
+  }
   STRIDE_DEV uint32_t bits(uint32_t n) {   // n <= 32
     const uint32_t v = n ? (uint32_t)(win >> (64 - n)) : 0u;
     drop(n);
