@@ -132,17 +132,21 @@ __device__ __forceinline__ void pdlt_group_tail(const uint32_t* __restrict__ lds
   if (active && role > 0)
     pdlt_axis(lds_c, rel, e, role - 1, P, recon + (uint32_t)(role - 1) * nl, walk, *n_emit);
   __syncthreads();
-  // interior vertices, one meshlet at a time, 64 coordinates per step
+  // interior vertices, one meshlet at a time, 64 coordinates per step; the
+  // meshlet's fields come from its slot lane (lane j), not a table read
   for (uint32_t j = 0; j < G.m1 - G.m0; ++j) {
-    const PdltEntry ej = tab[G.m0 + j];
-    const uint32_t nlj = ej.n_bnd + ej.n_int;
-    const uint32_t* bj = lds + G.blk_base + ej.blk_off;
+    const uint32_t nbj = (uint32_t)__builtin_amdgcn_readlane((int)e.n_bnd, (int)j);
+    const uint32_t nij = (uint32_t)__builtin_amdgcn_readlane((int)e.n_int, (int)j);
+    const uint32_t boj = (uint32_t)__builtin_amdgcn_readlane((int)e.blk_off, (int)j);
+    const uint32_t ibj = (uint32_t)__builtin_amdgcn_readlane((int)e.int_base, (int)j);
+    const uint32_t nlj = nbj + nij;
+    const uint32_t* bj = lds + G.blk_base + boj;
     const T* rj = reinterpret_cast<const T*>(bj);
-    const uint32_t* sj = bj + RW * nlj + ej.n_bnd + ej.n_int;
-    float* vo = V + 3ull * ej.int_base;
-    const uint32_t n3 = 3 * ej.n_int;
+    const uint32_t* sj = bj + RW * nlj + nbj + nij;
+    float* vo = V + 3ull * ibj;
+    const uint32_t n3 = 3 * nij;
     for (uint32_t t = lane; t < n3; t += 64) {
-      const uint32_t i = t / 3, d = t - 3 * i, v = ej.n_bnd + i;
+      const uint32_t i = t / 3, d = t - 3 * i, v = nbj + i;
       const bool reached = ((sj[v >> 5] >> (v & 31)) & 1u) != 0;
       vo[t] = pdlt_out(reached ? (double)rj[d * nlj + v] : 0.0, P, (int)d);
     }
@@ -201,6 +205,7 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
   uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
   const int lane = threadIdx.x;
   const PdltGroup G = groups[blockIdx.x];
+  const uint32_t rb = tab[G.m0].ref_base;   // read early: the refs go out last
   stage_words(lds, words, G.w_lo, G.n_words, (uint32_t)lane, 64);
   __syncthreads();
   const uint32_t m = G.m0 + (uint32_t)lane;
@@ -218,7 +223,7 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
     so[8] = n_emit;
   }
   __syncthreads();
-  pdlt_refs_out(lds, G, tab[G.m0].ref_base, refs_g, lane);
+  pdlt_refs_out(lds, G, rb, refs_g, lane);
 }
 
 // Two-pass decode, pass 1 with token chains: a wave takes consecutive
