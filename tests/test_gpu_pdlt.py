@@ -111,6 +111,21 @@ def test_zero_length_strips_on_every_token_pass():
         assert_same(*decoder(data, **tune).decode_to_host(), *ref)
 
 
+def test_long_strip_meshlets():
+    """A one-meshlet stream of 8,000 triangles in one strip decodes through
+    the token chains as one chain; one of 20,000 needs more LDS for its
+    token records than a chain wave has, and runs the token lanes."""
+    from tests.test_pdlt_host import long_strip_stream
+    for n, chains in ((8000, 16), (20000, 0)):
+        print(f"pdlt long strip n_tris={n}", flush=True)
+        data = long_strip_stream(n)
+        ref = PO.decode_v5(data)
+        dec = decoder(data)
+        assert dec.get_tune()["tok_chains"] == chains
+        assert_same(*dec.decode_to_host(), *ref)
+        assert_same(*decoder(data, mode=3).decode_to_host(), *ref)
+
+
 def test_chain_tune_round_trip_and_errors():
     data = np.load(GOLD[-1])["data"].tobytes()
     dec = decoder(data, **CHAIN_LAYOUTS[0])

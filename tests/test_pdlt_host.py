@@ -121,6 +121,47 @@ def strip_lengths(data):
     return out
 
 
+def long_strip_stream(n_tris):
+    """The one-triangle golden stream with its strip stretched to n_tris
+    triangles: the three root tokens, then n_tris - 1 body tokens (edge bit
+    0, FIFO hit on the oldest entry), the residual section kept (no token
+    names a new vertex).  Layout as encoder/paradelta_v5.py:144-283 writes it:
+    the meshlet header at the first offset, n_tris in its third field, the
+    strip length after the header, 3-bit root tokens for 3 locals."""
+    data = np.load(ROOT / "tests" / "golden" / "pdlt_one_tri.npz")["data"].tobytes()
+    h = PO.read_header(data)
+    assert (h["n_meshlets"], h["n_boundary"], h["n_t"]) == (1, 0, 1)
+    bits = "".join(f"{b:08b}" for b in data)
+    m0 = h["meshlets_bit"]
+    assert bits[m0:m0 + 80] == f"{0:016b}{3:016b}{1:016b}{1:016b}{1:016b}"
+    head = bits[:m0]
+    head = head[:57 * 8] + f"{n_tris:032b}" + head[61 * 8:]        # header n_t
+    body = (f"{0:016b}{3:016b}{n_tris:016b}{1:016b}{n_tris:016b}" + bits[m0 + 80:m0 + 89]
+            + "0" * 7 * (n_tris - 1) + bits[m0 + 89:])
+    out = head + body
+    out += "0" * (-len(out) % 8)
+    return bytes(int(out[i:i + 8], 2) for i in range(0, len(out), 8))
+
+
+def test_long_strip_meshlets(pdlt_check, tmp_path):
+    """One meshlet of 8,000 and one of 20,000 triangles in a single strip:
+    the first runs the chain token pass as one chain (more than 511
+    tokens), the second needs more LDS for its token records than a chain
+    wave has (its layout is refused; the decoder then runs token lanes)."""
+    for n in (8000, 20000):
+        data = long_strip_stream(n)
+        Vo, To = PO.decode_v5(data)
+        assert To.shape == (n, 3)
+        for two_pass in (0, 1):
+            rc, V, T = run(pdlt_check, data, tmp_path, two_pass=two_pass)
+            assert rc == 0 and same(V, T, Vo, To), (n, two_pass)
+        rc, V, T = run(pdlt_check, data, tmp_path, two_pass=2)
+        if n == 8000:
+            assert rc == 0 and same(V, T, Vo, To)
+        else:
+            assert rc == 2 and "too large for the chain token pass" in V
+
+
 def test_zero_length_strips_decode_as_length_one(pdlt_check, tmp_path):
     """A strip length of 0 stands for one triangle (its three roots), as
     length 1 does (decode_paradelta_v5 runs L - 1 body steps): rewriting
