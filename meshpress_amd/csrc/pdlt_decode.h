@@ -33,6 +33,8 @@ struct PdltSet {
   PdltGroup* d_groups = nullptr;
   PdltGroup* d_tok_groups = nullptr;
   ChainEntry* d_chains = nullptr;       // parser state at every token chain's start
+  RefChunk* d_rchunks = nullptr;        // boundary-ref chunks of the chain token pass
+  uint32_t n_rchunks = 0;
   uint32_t* d_walk = nullptr;           // two-pass walks (n_v - n_b words)
   uint32_t* d_seen = nullptr;           // two-pass seen sets + walk lengths (kSeenWords per meshlet)
   uint32_t* d_refs = nullptr;           // two-pass boundary refs (n_refs words)

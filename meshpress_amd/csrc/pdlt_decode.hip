@@ -228,8 +228,8 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
 
 // Two-pass decode, pass 1 with token chains: a wave takes consecutive
 // meshlets (up to 64 chains), whose bits from the first header to the end of
-// the last token section are staged in LDS.  Meshlet lanes decode the
-// boundary refs into LDS; phase 1: one lane per chain writes a 16-bit
+// the last token section are staged in LDS.  Chunk lanes decode the
+// boundary refs into LDS, 16 refs a lane (RefChunk); phase 1: one lane per chain writes a 16-bit
 // record per token (ChainLane); phase 2: the whole wave takes the meshlets
 // one after another, resolves the FIFO placeholders, finds first sightings
 // (an LDS atomic min of generation << 16 | token per local id) and forms
@@ -242,9 +242,10 @@ __global__ __launch_bounds__(64) void pdlt_tokens_kernel(
 // first sightings (256 words).
 __global__ __launch_bounds__(64) void pdlt_chain_tokens_kernel(
     const uint32_t* __restrict__ words, const PdltEntry* __restrict__ tab,
-    const PdltGroup* __restrict__ groups, const ChainEntry* __restrict__ chains, uint32_t k0,
-    uint32_t n_b, uint32_t* __restrict__ walk_g, uint32_t* __restrict__ seen_g,
-    uint32_t* __restrict__ refs_g, uint32_t* __restrict__ T) {
+    const PdltGroup* __restrict__ groups, const ChainEntry* __restrict__ chains,
+    const RefChunk* __restrict__ rchunks, uint32_t k0, uint32_t n_b,
+    uint32_t* __restrict__ walk_g, uint32_t* __restrict__ seen_g, uint32_t* __restrict__ refs_g,
+    uint32_t* __restrict__ T) {
   extern __shared__ uint64_t lds64[];
   uint32_t* lds = reinterpret_cast<uint32_t*>(lds64);
   const int lane = threadIdx.x;
@@ -255,6 +256,14 @@ __global__ __launch_bounds__(64) void pdlt_chain_tokens_kernel(
   if ((uint32_t)lane < nmesh) e = tab[G.m0 + (uint32_t)lane];
   ChainEntry ce{};
   if ((uint32_t)lane < nch) ce = chains[c0 + (uint32_t)lane];
+  // the wave's boundary-ref chunks: from its first meshlet's first chunk to
+  // its last meshlet's last
+  const uint32_t r0 = (uint32_t)__builtin_amdgcn_readlane((int)e.chunk_base, 0);
+  const uint32_t nrc =
+      (uint32_t)__builtin_amdgcn_readlane((int)(e.chunk_base + pdlt_ref_chunks(e.n_bnd)),
+                                          (int)(nmesh - 1)) - r0;
+  RefChunk rk{};
+  if ((uint32_t)lane < nrc) rk = rchunks[r0 + (uint32_t)lane];
   stage_words(lds, words, G.w_lo, G.n_words, (uint32_t)lane, 64);
   const uint32_t n_local = e.n_bnd + e.n_int;
   const uint32_t ntok = (uint32_t)lane < nmesh ? pdlt_tokens_of(e) : 0u;
@@ -278,9 +287,17 @@ __global__ __launch_bounds__(64) void pdlt_chain_tokens_kernel(
   const uint32_t c_sf = __shfl(e.n_strips, (int)mi, 64);
   const uint32_t c_ntok = __shfl(ntok, (int)mi, 64);
   const uint32_t c_lm = __shfl(lm, (int)mi, 64);
+  // the chunk lane's meshlet, found as the chain lane's
+  const bool chunk_lane = (uint32_t)lane < nrc;
+  const uint32_t rci = rk.pos >> 16;
+  const uint64_t rstarts = __ballot(chunk_lane && rci == 0u);
+  const uint32_t ri = chunk_lane ? lanes_below(rstarts) - (rci != 0u ? 1u : 0u) : 0u;
+  const uint32_t r_start = __shfl(e.start_bit, (int)ri, 64);
+  const uint32_t r_nb = __shfl(e.n_bnd, (int)ri, 64);
+  const uint32_t r_off = __shfl(e.tok_ref_off, (int)ri, 64);
   const int64_t rel = -32 * (int64_t)G.w_lo;
   __syncthreads();
-  if ((uint32_t)lane < nmesh) pdlt_refs(lds, rel, e, refs + e.tok_ref_off);
+  if (chunk_lane) pdlt_refs_chunk(lds, rel, r_start, r_nb, rk, refs + r_off);
   if (chain_lane) {
     LdsCursor cur{lds, (uint32_t)((int64_t)c_bit + rel) + (ce.pos & 0xFFFFu)};
     ChainLane<LdsCursor, true> cl;
@@ -442,6 +459,7 @@ PdltSet::~PdltSet() {
   (void)hipFree(d_walk);
   (void)hipFree(d_seen);
   (void)hipFree(d_chains);
+  (void)hipFree(d_rchunks);
   (void)hipFree(d_refs);
 }
 
@@ -449,7 +467,8 @@ size_t PdltSet::side_bytes() const {
   return H.tab.size() * sizeof(PdltEntry) + groups.size() * sizeof(PdltGroup) +
          H.bnd.size() * sizeof(double) +
          (two_pass ? tok_groups.size() * sizeof(PdltGroup) : 0) +
-         (two_pass && chain_waves ? n_chains * sizeof(ChainEntry) : 0);
+         (two_pass && chain_waves ? n_chains * sizeof(ChainEntry) + n_rchunks * sizeof(RefChunk)
+                                  : 0);
 }
 
 int PdltSet::build(const uint8_t* d, size_t len) {
@@ -482,9 +501,10 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   // token chains where every meshlet fits a chain token wave, else the
   // token lanes
   std::vector<ChainEntry> ch;
+  std::vector<RefChunk> rch;
   bool cw = false;
   if (!rc && tok_chains > 0) {
-    rc = pdlt_layout_chains(tab, H.words, tok_chains, tok_group, tg, ch, tlds, err);
+    rc = pdlt_layout_chains(tab, H.words, tok_chains, tok_group, tg, ch, rch, tlds, err);
     cw = rc == STRIDE_OK;
     if (rc == STRIDE_E_UNSUPPORTED) rc = STRIDE_OK;
   }
@@ -498,6 +518,7 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   PdltGroup* ng = nullptr;
   PdltGroup* nt = nullptr;
   ChainEntry* nc = nullptr;
+  RefChunk* nr = nullptr;
   hipError_t e = hipSuccess;
   if (!gr.empty()) {
     e = hipMalloc(&ng, gr.size() * sizeof(PdltGroup));
@@ -514,12 +535,18 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
     if (e == hipSuccess)
       e = hipMemcpy(nc, ch.data(), ch.size() * sizeof(ChainEntry), hipMemcpyHostToDevice);
   }
+  if (e == hipSuccess && !rch.empty()) {
+    e = hipMalloc(&nr, rch.size() * sizeof(RefChunk));
+    if (e == hipSuccess)
+      e = hipMemcpy(nr, rch.data(), rch.size() * sizeof(RefChunk), hipMemcpyHostToDevice);
+  }
   if (e == hipSuccess && !tab.empty())
     e = hipMemcpy(d_tab, tab.data(), tab.size() * sizeof(PdltEntry), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     (void)hipFree(ng);
     (void)hipFree(nt);
     (void)hipFree(nc);
+    (void)hipFree(nr);
     return set_error(STRIDE_E_HIP, std::string("PDLT launch layout: ") + hipGetErrorString(e));
   }
   H.tab.swap(tab);
@@ -533,11 +560,14 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   (void)hipFree(d_groups);
   (void)hipFree(d_tok_groups);
   (void)hipFree(d_chains);
+  (void)hipFree(d_rchunks);
   d_groups = ng;
   d_tok_groups = nt;
   d_chains = nc;
+  d_rchunks = nr;
   chain_waves = cw;
   n_chains = (uint32_t)ch.size();
+  n_rchunks = (uint32_t)rch.size();
   ready = true;
   return STRIDE_OK;
 }
@@ -565,7 +595,7 @@ int PdltSet::decode(float* V, uint32_t* T, int world, hipStream_t s) {
   if (two_pass) {
     if (chain_waves)
       hipLaunchKernelGGL(pdlt_chain_tokens_kernel, dim3((uint32_t)tok_groups.size()), dim3(64),
-                         tok_lds_bytes, s, d_words, d_tab, d_tok_groups, d_chains,
+                         tok_lds_bytes, s, d_words, d_tab, d_tok_groups, d_chains, d_rchunks,
                          (uint32_t)tok_chains, H.n_b, d_walk, d_seen, d_refs, T);
     else
       hipLaunchKernelGGL(pdlt_tokens_kernel, dim3((uint32_t)tok_groups.size()), dim3(64),

@@ -303,15 +303,17 @@ inline int pdlt_layout_tokens(std::vector<PdltEntry>& tab, int budget, int gmax,
 // token section (+ 3 words: the refs' read-ahead window), the refs (n_bnd
 // words a meshlet), the 16-bit token records (each meshlet's padded to an
 // even count), 16 bytes of final FIFO per chain and 256 first-sighting
-// words.  Sets each entry's tok_ref_off, each group's refs base (blk_base),
-// records base (seen_base) and chain range [pad[0], pad[1]), and writes the
-// parser state at every cut to `chains` (stream = the stream's words).
-// STRIDE_E_UNSUPPORTED when a meshlet alone does not fit (its tokens then
-// need the token lanes).
+// words; at most 64 boundary-ref chunks a wave.  Sets each entry's
+// tok_ref_off and chunk_base, each group's refs base (blk_base), records
+// base (seen_base) and chain range [pad[0], pad[1]), and writes the parser
+// state at every cut to `chains` and the ref chunks to `rchunks` (stream =
+// the stream's words).  STRIDE_E_UNSUPPORTED when a meshlet alone does not
+// fit (its tokens then need the token lanes).
 constexpr uint32_t kPdltChainLds = 49152;
 inline int pdlt_layout_chains(std::vector<PdltEntry>& tab, const std::vector<uint32_t>& stream,
                               int k0, int gmax, std::vector<PdltGroup>& groups,
-                              std::vector<ChainEntry>& chains, uint32_t& lds, std::string& err) {
+                              std::vector<ChainEntry>& chains, std::vector<RefChunk>& rchunks,
+                              uint32_t& lds, std::string& err) {
   if (k0 < 1 || k0 > kMaxChains) { err = "tok_chains must be in [0, 16]"; return STRIDE_E_ARG; }
   if (gmax < 1 || gmax > 64) { err = "tok_group must be in [1, 64]"; return STRIDE_E_ARG; }
   groups.clear();
@@ -320,31 +322,35 @@ inline int pdlt_layout_chains(std::vector<PdltEntry>& tab, const std::vector<uin
   auto need = [](uint64_t w, uint64_t refs, uint64_t recw) {
     return 4 * (w + refs + recw) + 16 * 64 + 4 * 256;
   };
-  uint32_t max_lds = 0, nc = 0, m = 0;
+  uint32_t max_lds = 0, nc = 0, nr = 0, m = 0;
   while (m < n_m) {
     PdltGroup g{};
     g.m0 = m;
     g.w_lo = tab[m].start_bit >> 5;
-    uint32_t refs = 0, recw = 0, words = 0, cn = 0, j = m;
+    uint32_t refs = 0, recw = 0, words = 0, cn = 0, rn = 0, j = m;
     while (j < n_m && j - m < (uint32_t)gmax) {
       const PdltEntry& e = tab[j];
       const uint32_t nt = pdlt_tokens_of(e);
       const uint32_t cnt = chain_count(nt, pdlt_chain_tokens(e, (uint32_t)k0));
+      const uint32_t rcn = pdlt_ref_chunks(e.n_bnd);
       const uint32_t w = std::max(words, ((pdlt_tokens_end(e) + 31) >> 5) + 3 - g.w_lo);
       const uint32_t rw = ((nt + 1) & ~1u) / 2;
-      if (need(w, refs + e.n_bnd, recw + rw) > kPdltChainLds || cn + cnt > 64) {
-        if (j == m) {
-          err = "meshlet " + std::to_string(j) + " too large for the chain token pass";
-          return STRIDE_E_UNSUPPORTED;
-        }
-        break;
+      // a chunk's code offset from the header must fit 16 bits
+      if (e.strips_bit - e.start_bit > 0xFFFFu ||
+          (j == m && need(w, refs + e.n_bnd, recw + rw) > kPdltChainLds)) {
+        err = "meshlet " + std::to_string(j) + " too large for the chain token pass";
+        return STRIDE_E_UNSUPPORTED;
       }
+      if (need(w, refs + e.n_bnd, recw + rw) > kPdltChainLds || cn + cnt > 64 || rn + rcn > 64)
+        break;
       tab[j].tok_ref_off = refs;
+      tab[j].chunk_base = nr + rn;
       cbase[j] = nc + cn;
       refs += e.n_bnd;
       recw += rw;
       words = w;
       cn += cnt;
+      rn += rcn;
       ++j;
     }
     g.m1 = j;
@@ -354,16 +360,34 @@ inline int pdlt_layout_chains(std::vector<PdltEntry>& tab, const std::vector<uin
     g.pad[0] = nc;
     g.pad[1] = nc + cn;
     nc += cn;
+    nr += rn;
     max_lds = std::max(max_lds, (uint32_t)need(words, refs, recw));
     groups.push_back(g);
     m = j;
   }
   chains.assign(nc, ChainEntry{0u, 0u});
+  rchunks.assign(nr, RefChunk{0u, 0u});
   const GlobalWords src{stream.data(), (uint32_t)stream.size()};
   pdlt_detail::par_for(n_m, [&](int64_t i) {
     const PdltEntry& e = tab[i];
     chain_starts_at(src, e.strips_bit, e.strips_bit, pdlt_tokens_of(e), e.n_bnd + e.n_int,
                     pdlt_chain_tokens(e, (uint32_t)k0), chains.data() + cbase[i]);
+    // ref chunks: the code offset and the ref before every kRefChunk-th ref
+    RefChunk* rc = rchunks.data() + e.chunk_base;
+    rc[0] = RefChunk{64u, 0u};
+    if (e.n_bnd > 1) {
+      BitReader<GlobalWords> r;
+      r.init(src, e.start_bit + 64);
+      uint32_t prev = r.bits(32);
+      const uint32_t k = r.bits(8);
+      for (uint32_t q = 1; q < e.n_bnd; ++q) {
+        if (q % kRefChunk == 0)
+          rc[q / kRefChunk] = RefChunk{(r.pos - e.start_bit) | ((q / kRefChunk) << 16), prev};
+        uint32_t z = 0;
+        while (r.bits(1) == 0) ++z;
+        prev = prev + ((z << k) | r.bits((int)k)) + 1u;
+      }
+    }
   });
   lds = max_lds;
   return STRIDE_OK;

@@ -51,7 +51,7 @@ struct PdltEntry {
   uint32_t blk_off;      // LDS block, 32-bit words from the group's block base
   uint32_t tok_ref_off;  // two-pass layout: refs in the token group's LDS, words
   uint32_t ref_base;     // two-pass layout: first word of its refs in the refs buffer
-  uint32_t pad;
+  uint32_t chunk_base;   // chain token pass: its first boundary-ref chunk
 };
 static_assert(sizeof(PdltEntry) == 64, "PdltEntry is one 64-byte line");
 
@@ -93,6 +93,21 @@ STRIDE_DEV uint32_t pdlt_chain_tokens(const PdltEntry& e, uint32_t k0) {
   const uint32_t n = pdlt_tokens_of(e);
   if (n > kPdltChainMaxTokens) return n;
   return chain_tokens(n, e.n_bnd + e.n_int, e.n_strips, k0);
+}
+
+// Boundary-ref chunks of the chain token pass: a meshlet's refs are cut
+// every kRefChunk refs so that several lanes decode them at once (every
+// meshlet has at least one chunk).  pos: bit offset of the chunk's first
+// Rice code from the meshlet header (chunk 0: 64, the first ref's u32), in
+// the low 16 bits | chunk index << 16; prev: the ref before the chunk's
+// first.
+constexpr uint32_t kRefChunk = 16;
+struct RefChunk {
+  uint32_t pos;
+  uint32_t prev;
+};
+STRIDE_DEV uint32_t pdlt_ref_chunks(uint32_t n_bnd) {
+  return n_bnd > kRefChunk ? (n_bnd + kRefChunk - 1) / kRefChunk : 1u;
 }
 
 // One past the last strip-token bit of a meshlet: its residual section, or
@@ -315,6 +330,32 @@ STRIDE_DEV void pdlt_refs(const uint32_t* __restrict__ st, int64_t rel, const Pd
       prev = prev + rice_raw(bs, k) + 1u;
       refs[i] = prev;
     }
+  }
+}
+
+// The refs of one chunk of a meshlet (header at stream bit `start`, n_bnd
+// refs) into refs[] (the meshlet's whole ref array), from the staged stream
+// as pdlt_refs.
+STRIDE_DEV void pdlt_refs_chunk(const uint32_t* __restrict__ st, int64_t rel, uint32_t start,
+                                uint32_t n_bnd, const RefChunk& ch,
+                                uint32_t* __restrict__ refs) {
+  if (n_bnd == 0) return;
+  const uint32_t c = ch.pos >> 16;
+  const uint32_t end = (c + 1) * kRefChunk < n_bnd ? (c + 1) * kRefChunk : n_bnd;
+  const uint32_t h = (uint32_t)((int64_t)start + rel);
+  const uint32_t k = n_bnd > 1 ? win32(st, h + 96) >> 24 : 0u;
+  BitStream<LdsStaged> bs;
+  bs.init(LdsStaged{st, 0}, h + (ch.pos & 0xFFFFu));
+  uint32_t prev = ch.prev, i = c * kRefChunk;
+  if (c == 0) {
+    prev = bs.bits(32);
+    refs[0] = prev;
+    bs.drop(n_bnd > 1 ? 8u : 0u);
+    i = 1;
+  }
+  for (; i < end; ++i) {
+    prev = prev + rice_raw(bs, k) + 1u;
+    refs[i] = prev;
   }
 }
 

@@ -75,8 +75,9 @@ int main(int argc, char** argv) {
   if (tp == 2) {
     std::vector<PdltGroup> tg;
     std::vector<ChainEntry> ch;
+    std::vector<RefChunk> rch;
     uint32_t tlds = 0;
-    rc = pdlt_layout_chains(H.tab, H.words, k0, 64, tg, ch, tlds, err);
+    rc = pdlt_layout_chains(H.tab, H.words, k0, 64, tg, ch, rch, tlds, err);
     if (rc) {
       std::fprintf(stderr, "chain layout (%d): %s\n", rc, err.c_str());
       return 2;
@@ -93,7 +94,12 @@ int main(int argc, char** argv) {
         const uint32_t nb = e.n_bnd, nl = e.n_bnd + e.n_int, nt = pdlt_tokens_of(e);
         const uint32_t lm = pdlt_chain_tokens(e, (uint32_t)k0), cnt = chain_count(nt, lm);
         uint32_t* refs = lds + G.blk_base + e.tok_ref_off;
-        pdlt_refs(lds, rel, e, refs);
+        // the refs chunk by chunk, as the chunk lanes decode them
+        for (uint32_t q = 0; q < pdlt_ref_chunks(nb); ++q) {
+          const RefChunk rk = rch.at(e.chunk_base + q);
+          if ((rk.pos >> 16) != q) return 5;
+          pdlt_refs_chunk(lds, rel, e.start_bit, nb, rk, refs);
+        }
         // phase 1: every chain on its own
         std::vector<uint16_t> rec(nt);
         std::vector<uint8_t> fin(16 * cnt);
