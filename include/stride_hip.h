@@ -188,7 +188,7 @@ int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
  *   LDS words, tok_group <= 64 meshlets per wave).  A stream with a meshlet
  *   too large for a chain token wave (48 KiB of LDS with its 16-bit token
  *   records) runs the token lanes, 64 meshlets per wave, and get_tune
- *   reports tok_chains 0.
+ *   reports tok_chains 0 and tok_group 64.
  *   get_tune reports in `reserved` the meshlets decoded by mode 0. */
 typedef struct stride_tune {
   int32_t waves;

@@ -1186,7 +1186,10 @@ int stride_decoder_get_tune(const stride_decoder* h, stride_tune* t) {
     t->group_size = h->v5->group_size;
     t->lds_bytes = h->v5->lds_budget;
     t->stage_words = h->v5->tok_lds_budget / 4;
-    t->tok_group = h->v5->tok_group;
+    // token lanes in place of chains (a meshlet too large for a chain wave)
+    // run 64 meshlets a wave
+    const bool fell_back = h->v5->tok_chains > 0 && !h->v5->chain_waves;
+    t->tok_group = fell_back ? 64 : h->v5->tok_group;
     t->tok_chains = h->v5->chain_waves ? h->v5->tok_chains : 0;
     return STRIDE_OK;
   }

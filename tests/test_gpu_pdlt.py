@@ -121,7 +121,7 @@ def test_long_strip_meshlets():
         data = long_strip_stream(n)
         ref = PO.decode_v5(data)
         dec = decoder(data)
-        assert dec.get_tune()["tok_chains"] == chains
+        assert (dec.get_tune()["tok_chains"], dec.get_tune()["tok_group"]) == (chains, 4 if chains else 64)
         assert_same(*dec.decode_to_host(), *ref)
         assert_same(*decoder(data, mode=3).decode_to_host(), *ref)
 
