@@ -91,6 +91,45 @@ class IrlpCost:
         return d_out[:S].cpu().numpy()
 
 
+class IrlpCostAxes:
+    """Device-resident samples of all three axes, uploaded once (columns
+    (4, 3, n): a, b, c, v by axis, and int64 meshlet offsets); candidate
+    costs are launched per axis without a synchronisation and read back
+    together (collect), one synchronisation per batch of launches."""
+
+    def __init__(self, cols, off):
+        torch = _torch()
+        cols = np.ascontiguousarray(cols, dtype=np.int32)
+        self.n = int(cols.shape[2])
+        self.n_m = len(off) - 1
+        self.cols = _dev(cols if self.n else np.zeros((4, 3, 1), np.int32), torch.int32)
+        self.off = _dev(off, torch.int64)
+        self.evals = 0
+
+    def launch(self, d: int, cands: np.ndarray, K: int):
+        """Device int64 costs of the candidates (S, 3) on axis d."""
+        torch = _torch()
+        cands = np.ascontiguousarray(np.asarray(cands, dtype=np.int32).reshape(-1, 3))
+        S = len(cands)
+        d_c = _dev(cands, torch.int32)
+        d_out = torch.empty(max(S, 1), dtype=torch.int64, device="cuda")
+        row = 4 * max(self.n, 1)                 # bytes of one (column, axis) row
+        col = lambda j: self.cols.data_ptr() + row * (3 * j + d)   # noqa: E731
+        _lib.check(_lib.lib().stride_irlp_cost(
+            col(0), col(1), col(2), col(3), _lib.ptr(self.off), self.n_m, _lib.ptr(d_c), S, int(K),
+            _lib.ptr(d_out), _lib.stream_ptr()))
+        self.evals += S
+        return d_out[:S]
+
+    @staticmethod
+    def collect(outs):
+        """Host arrays of launch() results, in one transfer."""
+        torch = _torch()
+        sizes = [len(o) for o in outs]
+        flat = torch.cat(outs).cpu().numpy() if sum(sizes) else np.zeros(0, np.int64)
+        return np.split(flat, np.cumsum(sizes)[:-1])
+
+
 def irls_l1(X, y, n_iter=IRLS_ITERS, eps=IRLS_EPS):
     w, *_ = np.linalg.lstsq(X, y, rcond=None)
     for _ in range(n_iter):

@@ -152,8 +152,9 @@ class ProposalMLP:
                                            _lib.stream_ptr()))
         return pp, cls, cp
 
-    def infer(self, F: np.ndarray):
-        """Host features (n, 32) -> host (pred_prob, coder class, coder prob)."""
+    def infer(self, F: np.ndarray, want=("pred", "cls", "cp")):
+        """Host features (n, 32) -> host (pred_prob, coder class, coder prob);
+        outputs not named in `want` are returned as None (not copied back)."""
         n = len(F)
         if n == 0:
             return (np.zeros((0, 3 * PRED_CLASSES), np.float32), np.zeros((0, 6), np.uint8),
@@ -161,7 +162,8 @@ class ProposalMLP:
         x = self.features_to_device(F)
         _, _, z = self.forward(x)
         pp, cls, cp = self.heads(z)
-        return pp.cpu().numpy(), cls.cpu().numpy(), cp.cpu().numpy()
+        return tuple(t.cpu().numpy() if w in want else None
+                     for t, w in ((pp, "pred"), (cls, "cls"), (cp, "cp")))
 
     # ------------------------------------------------------------ I/O
     def params_host(self) -> np.ndarray:

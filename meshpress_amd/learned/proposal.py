@@ -69,33 +69,52 @@ class LearnedProposal:
         t0 = time.perf_counter()
         sel = search.subsample_sel(nat.n_meshlets)
         cols, off = nat.para_columns(sel)
-        pp, _, _ = self.mlp.infer(nat.features(0, sel))
+        pp, _, _ = self.mlp.infer(nat.features(0, sel), want=("pred",))
         w = np.diff(off).astype(np.float64)
         pred_n = search.CANON_N.copy()
         pred_K = search.CANON_K.copy()
         evals, conf, fb = 0, [], []
         full = None
         if cols.shape[2] and w.sum() > 0:
+            # one upload of the three axes' samples; each stage's launches
+            # (the candidates of every axis, then each refinement round) are
+            # read back together
+            cost = search.IrlpCostAxes(cols, off)
+            S, order, cand = [], [], []
             for d in range(3):
-                S = w @ pp[:, PRED_CLASSES * d:PRED_CLASSES * (d + 1)].astype(np.float64)
-                order = np.argsort(-S, kind="stable")
-                cand = sorted({0, *order[:self.top].tolist()})
-                cost = search.IrlpCost(cols[0, d], cols[1, d], cols[2, d], cols[3, d], off)
-                cc = cost(VARIANTS[cand], VARIANT_K)
-                i = int(np.argmin(cc))
-                best_n, best_c, best_K = VARIANTS[cand[i]].astype(np.int64), int(cc[i]), VARIANT_K
-                canonical = cand[i] == 0
-                for _ in range(self.refine_rounds):
-                    mv = np.repeat(best_n[None], len(MOVES), axis=0)
+                S.append(w @ pp[:, PRED_CLASSES * d:PRED_CLASSES * (d + 1)].astype(np.float64))
+                order.append(np.argsort(-S[d], kind="stable"))
+                cand.append(sorted({0, *order[d][:self.top].tolist()}))
+            ccs = cost.collect([cost.launch(d, VARIANTS[cand[d]], VARIANT_K) for d in range(3)])
+            best_n, best_c, canonical = [], [], []
+            for d in range(3):
+                i = int(np.argmin(ccs[d]))
+                best_n.append(VARIANTS[cand[d][i]].astype(np.int64))
+                best_c.append(int(ccs[d][i]))
+                canonical.append(cand[d][i] == 0)
+            best_K = VARIANT_K
+            active = [True, True, True]
+            for _ in range(self.refine_rounds):
+                axes = [d for d in range(3) if active[d]]
+                if not axes:
+                    break
+                mvs = []
+                for d in axes:
+                    mv = np.repeat(best_n[d][None], len(MOVES), axis=0)
                     for j, (k, sg) in enumerate(MOVES):
                         mv[j, k] += sg
-                    mc = cost(mv, VARIANT_K)
+                    mvs.append(mv)
+                mcs = cost.collect([cost.launch(d, mv, VARIANT_K) for d, mv in zip(axes, mvs)])
+                for d, mv, mc in zip(axes, mvs, mcs):
                     j = int(np.argmin(mc))
-                    if int(mc[j]) >= best_c:
-                        break
-                    best_n, best_c, canonical = mv[j].copy(), int(mc[j]), False
-                evals += cost.evals
-                share = float(S[order[0]] / S.sum()) if S.sum() > 0 else 1.0
+                    if int(mc[j]) >= best_c[d]:
+                        active[d] = False
+                        continue
+                    best_n[d], best_c[d], canonical[d] = mv[j].copy(), int(mc[j]), False
+            evals += cost.evals
+            for d in range(3):
+                bn, bK, canon = best_n[d], best_K, canonical[d]
+                share = float(S[d][order[d][0]] / S[d].sum()) if S[d].sum() > 0 else 1.0
                 conf.append(share)
                 if fallback and share < self.tau_pred:
                     # the exhaustive per-axis search; both results are then
@@ -105,16 +124,16 @@ class LearnedProposal:
                     n_e, K_e, c_e, ev = search.fit_axis_cols((cols, off), full, d)
                     fcost = search.IrlpCost(full[0][0, d], full[0][1, d], full[0][2, d],
                                             full[0][3, d], full[1])
-                    c_l = int(fcost(best_n[None], int(best_K))[0])
+                    c_l = int(fcost(bn[None], int(bK))[0])
                     evals += ev + fcost.evals
                     fb.append(d)
                     if c_e <= c_l:
-                        best_n, best_K = n_e, K_e
-                        canonical = K_e == 0 and np.array_equal(n_e, search.CANON_N[d])
-                if canonical:
+                        bn, bK = n_e, K_e
+                        canon = K_e == 0 and np.array_equal(n_e, search.CANON_N[d])
+                if canon:
                     pred_n[d], pred_K[d] = search.CANON_N[d], 0
                 else:
-                    pred_n[d], pred_K[d] = best_n, best_K
+                    pred_n[d], pred_K[d] = bn, bK
         if stats is not None:
             stats.update(irlp_evals=evals, pred_confidence=conf, pred_fallback_axes=fb,
                          proposal_predictor_s=time.perf_counter() - t0)
@@ -124,7 +143,7 @@ class LearnedProposal:
     def select_codes(self, nat, soff, stats=None, fallback=False):
         """(tag, k) per residual stream of the last residuals() call."""
         t0 = time.perf_counter()
-        _, cls, cp = self.mlp.infer(nat.features(1))
+        _, cls, cp = self.mlp.infer(nat.features(1), want=("cls", "cp") if fallback else ("cls",))
         soff = np.asarray(soff, np.int64)
         lens = np.diff(soff)
         tagk = class_tagk(cls.reshape(-1))
