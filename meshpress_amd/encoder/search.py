@@ -93,9 +93,9 @@ class IrlpCost:
 
 class IrlpCostAxes:
     """Device-resident samples of all three axes, uploaded once (columns
-    (4, 3, n): a, b, c, v by axis, and int64 meshlet offsets); candidate
-    costs are launched per axis without a synchronisation and read back
-    together (collect), one synchronisation per batch of launches."""
+    (4, 3, n): a, b, c, v by axis, and int64 meshlet offsets).  run() takes
+    the candidates of several axes, uploads them in one copy, launches one
+    cost evaluation per axis and reads every cost back in one transfer."""
 
     def __init__(self, cols, off):
         torch = _torch()
@@ -106,28 +106,27 @@ class IrlpCostAxes:
         self.off = _dev(off, torch.int64)
         self.evals = 0
 
-    def launch(self, d: int, cands: np.ndarray, K: int):
-        """Device int64 costs of the candidates (S, 3) on axis d."""
+    def run(self, items, K: int):
+        """[(axis, candidates (S, 3))] -> [int64 costs (S,)], in order."""
         torch = _torch()
-        cands = np.ascontiguousarray(np.asarray(cands, dtype=np.int32).reshape(-1, 3))
-        S = len(cands)
-        d_c = _dev(cands, torch.int32)
-        d_out = torch.empty(max(S, 1), dtype=torch.int64, device="cuda")
+        cs = [np.asarray(c, dtype=np.int32).reshape(-1, 3) for _, c in items]
+        total = sum(len(c) for c in cs)
+        if total == 0:
+            return [np.zeros(0, np.int64) for _ in cs]
+        d_c = _dev(np.concatenate(cs), torch.int32)
+        d_out = torch.empty(total, dtype=torch.int64, device="cuda")
         row = 4 * max(self.n, 1)                 # bytes of one (column, axis) row
-        col = lambda j: self.cols.data_ptr() + row * (3 * j + d)   # noqa: E731
-        _lib.check(_lib.lib().stride_irlp_cost(
-            col(0), col(1), col(2), col(3), _lib.ptr(self.off), self.n_m, _lib.ptr(d_c), S, int(K),
-            _lib.ptr(d_out), _lib.stream_ptr()))
-        self.evals += S
-        return d_out[:S]
-
-    @staticmethod
-    def collect(outs):
-        """Host arrays of launch() results, in one transfer."""
-        torch = _torch()
-        sizes = [len(o) for o in outs]
-        flat = torch.cat(outs).cpu().numpy() if sum(sizes) else np.zeros(0, np.int64)
-        return np.split(flat, np.cumsum(sizes)[:-1])
+        base = self.cols.data_ptr()
+        o = 0
+        for (d, _), c in zip(items, cs):
+            col = [base + row * (3 * j + d) for j in range(4)]
+            _lib.check(_lib.lib().stride_irlp_cost(
+                col[0], col[1], col[2], col[3], _lib.ptr(self.off), self.n_m,
+                d_c.data_ptr() + 12 * o, len(c), int(K), d_out.data_ptr() + 8 * o,
+                _lib.stream_ptr()))
+            o += len(c)
+        self.evals += total
+        return np.split(d_out.cpu().numpy(), np.cumsum([len(c) for c in cs])[:-1])
 
 
 def irls_l1(X, y, n_iter=IRLS_ITERS, eps=IRLS_EPS):

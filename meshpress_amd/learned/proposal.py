@@ -76,16 +76,16 @@ class LearnedProposal:
         evals, conf, fb = 0, [], []
         full = None
         if cols.shape[2] and w.sum() > 0:
-            # one upload of the three axes' samples; each stage's launches
-            # (the candidates of every axis, then each refinement round) are
-            # read back together
+            # one upload of the three axes' samples; each stage (the
+            # candidates of every axis, then each refinement round) is one
+            # candidate upload, one launch per axis and one read-back
             cost = search.IrlpCostAxes(cols, off)
             S, order, cand = [], [], []
             for d in range(3):
                 S.append(w @ pp[:, PRED_CLASSES * d:PRED_CLASSES * (d + 1)].astype(np.float64))
                 order.append(np.argsort(-S[d], kind="stable"))
                 cand.append(sorted({0, *order[d][:self.top].tolist()}))
-            ccs = cost.collect([cost.launch(d, VARIANTS[cand[d]], VARIANT_K) for d in range(3)])
+            ccs = cost.run([(d, VARIANTS[cand[d]]) for d in range(3)], VARIANT_K)
             best_n, best_c, canonical = [], [], []
             for d in range(3):
                 i = int(np.argmin(ccs[d]))
@@ -104,7 +104,7 @@ class LearnedProposal:
                     for j, (k, sg) in enumerate(MOVES):
                         mv[j, k] += sg
                     mvs.append(mv)
-                mcs = cost.collect([cost.launch(d, mv, VARIANT_K) for d, mv in zip(axes, mvs)])
+                mcs = cost.run(list(zip(axes, mvs)), VARIANT_K)
                 for d, mv, mc in zip(axes, mvs, mcs):
                     j = int(np.argmin(mc))
                     if int(mc[j]) >= best_c[d]:
