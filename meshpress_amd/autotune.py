@@ -64,13 +64,20 @@ BENCH_CANDIDATES = ([dict(mode=2, group_size=gs, lds_bytes=lds, grid=0, use_queu
                     + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_group=32, tok_chains=0, grid=0,
                             use_queue=0) for gs, lds in ((16, 16384), (8, 16384))]
                     # the chain token pass: chains per meshlet x meshlets per token wave,
-                    # axis groups of 8, 16 or 21 meshlets (which pair wins depends on
+                    # axis groups of 6 to 21 meshlets (which pair wins depends on
                     # the mesh: 8 x 8 with groups of 16 on the 12-bit heightfield, the
-                    # default 16 x 4 with groups of 21 on its 16-bit twin)
+                    # default 16 x 4 with groups of 21 on its 16-bit twin, 16 x 2
+                    # with groups of 6 on the 1.3M-triangle icosphere)
                     + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_chains=kc, tok_group=tg,
                             grid=0, use_queue=0)
                        for kc, tg in ((4, 16), (8, 8), (16, 4))
                        for gs, lds in ((8, 24576), (16, 16384), (21, 24576))]
+                    # meshes of a few thousand meshlets (fewer waves than SIMDs):
+                    # half-filled token waves and axis groups of 6 keep more
+                    # waves in flight
+                    + [dict(mode=4, group_size=gs, lds_bytes=16384, tok_chains=16, tok_group=tg,
+                            grid=0, use_queue=0)
+                       for tg, gs in ((2, 6), (2, 8), (4, 6))]
                     + [dict(mode=1, group_size=gs, lds_bytes=lds, grid=0, use_queue=0)
                        for gs in (32, 64) for lds in (16384, 40960)])
 
