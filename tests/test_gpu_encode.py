@@ -44,6 +44,31 @@ def test_irlp_cost_golden():
     assert np.array_equal(got, want)
 
 
+def test_irlp_cost_axes_equals_per_axis_costs():
+    """The three-axis evaluator (one sample upload, one candidate upload and
+    one read-back per batch) against the oracle's costs of every axis,
+    including an empty candidate list."""
+    from meshpress_amd.encoder import search
+    from tests.oracle_c import load
+    z = np.load(GOLD / "coder_choice.npz")
+    rng = np.random.default_rng(3)
+    n = len(z["irlp_a"])
+    # three axes' samples: the golden axis and two permutations of it
+    perms = [np.arange(n), rng.permutation(n), rng.permutation(n)]
+    cols = np.stack([np.stack([z[f"irlp_{c}"].astype(np.int32)[p] for p in perms])
+                     for c in "abcv"])
+    off = z["irlp_off"].astype(np.int64)
+    cost = search.IrlpCostAxes(cols, off)
+    items = [(0, z["irlp_cands"][:3]), (2, z["irlp_cands"][2:6]), (1, np.zeros((0, 3), np.int32)),
+             (1, z["irlp_cands"][1:2] + 1)]
+    got = cost.run(items, 5)
+    for (d, cands), g in zip(items, got):
+        want = load().irlp_cost(cols[0, d], cols[1, d], cols[2, d], cols[3, d], off,
+                                np.asarray(cands).reshape(-1, 3), 5)
+        assert np.array_equal(g, want), d
+    assert cost.evals == 8
+
+
 @pytest.mark.parametrize("name", ["uv_12x24", "torus_30x10", "hf_24", "ico_2", "one_tri"])
 def test_fit_predictor_golden(name):
     from meshpress_amd.encoder import search
