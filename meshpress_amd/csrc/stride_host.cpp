@@ -21,16 +21,20 @@ namespace stride {
 // Small helpers
 // ---------------------------------------------------------------------------
 
+// Runs fn(i, thread) for i in [0, n) on up to n_threads threads, one per
+// 128 items at most: a thread's start and join cost tens of microseconds,
+// more than a small call's whole work.
 template <class F>
 static void parallel_for(int64_t n, int n_threads, F&& fn) {
-  if (n_threads <= 1 || n < 64) {
+  const int nt = (int)std::min<int64_t>(n_threads, n / 128);
+  if (nt <= 1) {
     for (int64_t i = 0; i < n; ++i) fn(i, 0);
     return;
   }
   std::atomic<int64_t> next{0};
   std::vector<std::thread> th;
   const int64_t chunk = 16;
-  for (int t = 0; t < n_threads; ++t) {
+  for (int t = 0; t < nt; ++t) {
     th.emplace_back([&, t]() {
       for (;;) {
         int64_t s = next.fetch_add(chunk);
