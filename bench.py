@@ -228,7 +228,8 @@ PDLT_LAYOUTS = ([dict(mode=4, group_size=gs, lds_bytes=lds, stage_words=sw, tok_
                                      (8, 12288, 8192))]
                 + [dict(mode=4, group_size=gs, lds_bytes=lds, tok_chains=kc, tok_group=tg)
                    for kc, tg in ((16, 4), (8, 8))
-                   for gs, lds in ((16, 24576), (16, 16384), (8, 12288), (8, 16384))]
+                   for gs, lds in ((16, 20480), (12, 20480), (16, 24576), (16, 16384), (8, 12288),
+                                   (8, 16384))]
                 + [dict(mode=3, group_size=16, lds_bytes=lds, stage_words=4096, tok_chains=0,
                         tok_group=64) for lds in (49152, 24576)])
 

@@ -182,7 +182,8 @@ int stride_decoder_header(const stride_decoder* h, float* f13, int32_t* i16);
  *     meshlets from a device atomic counter.
  *   grid: 0 = one workgroup per unit of work, else a persistent grid.
  *   PDLT v5 streams: mode 4 (default: the token pass, then chain groups of
- *   group_size meshlets in lds_bytes) or mode 3 (one kernel).  The token
+ *   group_size meshlets in lds_bytes, default 16 and 20480) or mode 3 (one
+ *   kernel).  The token
  *   pass runs token chains as for DPR4 (default tok_chains 16, tok_group 4)
  *   or, with tok_chains = 0, one token lane per meshlet (stage_words = its
  *   LDS words, tok_group <= 64 meshlets per wave).  A stream with a meshlet

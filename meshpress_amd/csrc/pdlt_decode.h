@@ -19,7 +19,7 @@ struct PdltSet {
   uint32_t lds_bytes = 0;               // dynamic LDS of the group launch
   uint32_t tok_lds_bytes = 0;           // dynamic LDS of the token launch
   int group_size = 16;
-  int lds_budget = 24576;
+  int lds_budget = 20480;
   int tok_lds_budget = 16384;
   int tok_group = 4;                    // meshlets per token-pass wave
   int tok_chains = 16;                  // > 0: token chains (about this many a meshlet)
