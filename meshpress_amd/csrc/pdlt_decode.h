@@ -27,6 +27,7 @@ struct PdltSet {
   uint32_t n_chains = 0;                // entries at d_chains
   int chain32 = 0;                      // 1: the encoder's float32 chain
   int two_pass = 1;                     // 1: token pass + chain pass, 0: one kernel
+  int layout_two_pass = -1;             // two_pass when the layout was made (its blocks differ)
   // device
   uint32_t* d_words = nullptr;
   PdltEntry* d_tab = nullptr;

@@ -87,8 +87,9 @@ __device__ __forceinline__ void pdlt_gather_group(uint32_t* lds, const PdltGroup
 // T: the reconstruction type, double (float64 chain) or float (float32 chain,
 // half the LDS per vertex); a block holds three T arrays of n_local, then
 // refs, walk and seen set.  kRefsG: the boundary gather reads the refs from
-// HBM (pdlt_gather_group; the blocks' refs are then not used), else the
-// four lanes of a slot gather from its block's refs.
+// HBM (pdlt_gather_group) and the blocks' refs slot, if pdlt_chain_block_refs
+// keeps one, is not used; else the four lanes of a slot gather from its
+// block's refs.
 template <class T, bool kRefsG = false>
 __device__ __forceinline__ void pdlt_group_tail(const uint32_t* __restrict__ lds_c, uint32_t* lds,
                                                 const PdltGroup& G, const PdltEntry& e, bool active,
@@ -102,7 +103,8 @@ __device__ __forceinline__ void pdlt_group_tail(const uint32_t* __restrict__ lds
   uint32_t* blk = lds + G.blk_base + e.blk_off;
   T* recon = reinterpret_cast<T*>(blk);
   const uint32_t* refs = blk + RW * nl;
-  const uint32_t* walk = refs + e.n_bnd;
+  constexpr bool kSlot = !kRefsG || pdlt_chain_block_refs(sizeof(T) == 4);
+  const uint32_t* walk = refs + (kSlot ? e.n_bnd : 0u);
   const uint32_t* n_emit = walk + e.n_int + 8;
   if constexpr (kRefsG) pdlt_gather_group<T>(lds, G, e, lane, refs_g, bnd, P);
   // boundary reconstructions: the four lanes of a slot gather its refs
@@ -142,7 +144,7 @@ __device__ __forceinline__ void pdlt_group_tail(const uint32_t* __restrict__ lds
     const uint32_t nlj = nbj + nij;
     const uint32_t* bj = lds + G.blk_base + boj;
     const T* rj = reinterpret_cast<const T*>(bj);
-    const uint32_t* sj = bj + RW * nlj + nbj + nij;
+    const uint32_t* sj = bj + RW * nlj + (kSlot ? nbj : 0u) + nij;
     float* vo = V + 3ull * ibj;
     const uint32_t n3 = 3 * nij;
     for (uint32_t t = lane; t < n3; t += 64) {
@@ -425,9 +427,11 @@ __global__ __launch_bounds__(64) void pdlt_chain_kernel(
   PdltEntry e = {};
   if (active) e = tab[m];
   const uint32_t nl = e.n_bnd + e.n_int;
+  // the block: reconstructions, the refs slot where pdlt_chain_block_refs
+  // keeps one (the refs come from HBM), walk, seen set
   uint32_t* blk = lds + G.blk_base + e.blk_off;
-  uint32_t* refs = blk + (3 * sizeof(R) / 4) * nl;
-  uint32_t* walk = refs + e.n_bnd;
+  uint32_t* walk = blk + (3 * sizeof(R) / 4) * nl +
+                   (pdlt_chain_block_refs(sizeof(R) == 4) ? e.n_bnd : 0u);
   uint32_t* seen = walk + e.n_int;
   const int64_t rel = -32 * (int64_t)G.w_lo;
   if (active) {
@@ -497,7 +501,10 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   uint32_t lds = 0, tlds = 0;
   std::vector<PdltEntry> tab = H.tab;
   // the float32 chain keeps three float32 per vertex, the float64 chain three float64
-  int rc = pdlt_layout(tab, gs, budget, gr, lds, err, chain32 ? 3u : 6u);
+  // (the chain pass of the two-pass decode reads the refs from HBM; its
+  // blocks keep the refs slot where pdlt_chain_block_refs says so)
+  int rc = pdlt_layout(tab, gs, budget, gr, lds, err, chain32 ? 3u : 6u,
+                       two_pass == 0 || pdlt_chain_block_refs(chain32 != 0));
   // token chains where every meshlet fits a chain token wave, else the
   // token lanes
   std::vector<ChainEntry> ch;
@@ -568,6 +575,7 @@ int PdltSet::set_layout(int gs, int budget, int tok_budget) {
   chain_waves = cw;
   n_chains = (uint32_t)ch.size();
   n_rchunks = (uint32_t)rch.size();
+  layout_two_pass = two_pass;
   ready = true;
   return STRIDE_OK;
 }
@@ -582,6 +590,8 @@ int PdltSet::decode(float* V, uint32_t* T, int world, hipStream_t s) {
   if ((!V && H.n_v) || (!T && H.n_t)) return set_error(STRIDE_E_ARG, "null output pointer");
   if (!ready)
     return set_error(STRIDE_E_HIP, "the decoder has no launch layout (the last set_tune failed)");
+  if (layout_two_pass != two_pass)
+    return set_error(STRIDE_E_ARG, "the launch layout was made for the other decode mode");
   PdltParams p = H.params;
   p.world = world ? 1 : 0;
   p.chain32 = chain32;

@@ -219,9 +219,11 @@ inline int pdlt_parse(const uint8_t* d, size_t len, PdltHost& H, std::string& er
 
 // Greedy wave groups: up to gs consecutive meshlets whose staged words and
 // LDS blocks fit `budget` bytes (a meshlet alone always forms a group).
-// Sets each entry's blk_off; lds = the largest group's LDS bytes.
+// Sets each entry's blk_off; lds = the largest group's LDS bytes.  refs:
+// the blocks hold the meshlets' boundary refs (the one-kernel decode); the
+// chain pass of the two-pass decode reads them from HBM instead.
 inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vector<PdltGroup>& groups,
-                       uint32_t& lds, std::string& err, uint32_t rw = 6) {
+                       uint32_t& lds, std::string& err, uint32_t rw = 6, bool refs = true) {
   if (gs < 1 || gs > 16) { err = "group_size must be in [1, 16]"; return STRIDE_E_ARG; }
   if (budget < 4096 || budget > 65536) { err = "lds_bytes must be in [4096, 65536]"; return STRIDE_E_ARG; }
   groups.clear();
@@ -237,7 +239,7 @@ inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vec
       // the words holding [start, end) plus the axis lanes' read-ahead (the
       // stream window reads up to 3 words past its position), even count
       const uint32_t w = ((((tab[j].end_bit - 1) >> 5) + 6 - g.w_lo) + 1) & ~1u;
-      const uint32_t b = pdlt_block_words(tab[j].n_bnd, tab[j].n_int, rw);
+      const uint32_t b = pdlt_block_words(tab[j].n_bnd, tab[j].n_int, rw, refs);
       if (j > m && 4ull * (w + blk + b) > (uint64_t)budget) break;
       tab[j].blk_off = blk;
       blk += b;

@@ -140,15 +140,22 @@ enum : uint32_t {
 };
 
 // LDS block of one meshlet, in 32-bit words: reconstruction (3 axes x
-// n_local float64, axis major), refs (n_bnd), walk (n_int), the seen set
-// (8 words) and the walk length (1 word); even, so blocks stay 8-byte
-// aligned.
+// n_local float64, axis major), refs (n_bnd; only where the block's own
+// lanes decode them, refs = true), walk (n_int), the seen set (8 words) and
+// the walk length (1 word); even, so blocks stay 8-byte aligned.
 // rw = words of reconstruction per local: 6 (three float64, the float64
 // chain) or 3 (three float32, the float32 chain)
-STRIDE_DEV uint32_t pdlt_block_words(uint32_t n_bnd, uint32_t n_int, uint32_t rw = 6) {
+STRIDE_DEV uint32_t pdlt_block_words(uint32_t n_bnd, uint32_t n_int, uint32_t rw = 6,
+                                     bool refs = true) {
   const uint32_t n_local = n_bnd + n_int;
-  return (rw * n_local + n_bnd + n_int + 9 + 1) & ~1u;
+  return (rw * n_local + (refs ? n_bnd : 0u) + n_int + 9 + 1) & ~1u;
 }
+// Whether the chain pass's blocks keep the refs slot (the chain pass reads
+// the refs from HBM and never uses it).  The float32 chain drops it: 0.220 ->
+// 0.209 ms on the 1900 x 1900 heightfield.  The float64 chain keeps it: without
+// it every LDS budget measured about 2% slower (0.2525 -> 0.258 ms in groups of
+// 16 in 20 KB), and group-size caps did not recover that.
+constexpr bool pdlt_chain_block_refs(bool chain32) { return !chain32; }
 
 STRIDE_DEV uint32_t pdlt_idx_bits(uint32_t n_local) {
   uint32_t b = 1;

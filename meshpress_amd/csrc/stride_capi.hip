@@ -1163,16 +1163,20 @@ int stride_decoder_set_tune(stride_decoder* h, const stride_tune* t) {
     if (kc < 0 || kc > kMaxChains) return fail(STRIDE_E_ARG, "tok_chains must be in [0, 16]");
     const int tg = t->tok_group ? t->tok_group : (kc > 0 ? 4 : 64);
     if (tg < 1 || tg > 64) return fail(STRIDE_E_ARG, "tok_group must be in [1, 64]");
-    const int prev_tg = h->v5->tok_group, prev_kc = h->v5->tok_chains;
+    // the mode decides the chain groups' blocks (the one-kernel decode keeps
+    // the refs in LDS), so it is set before the layout
+    const int prev_tg = h->v5->tok_group, prev_kc = h->v5->tok_chains,
+              prev_tp = h->v5->two_pass;
     h->v5->tok_group = tg;
     h->v5->tok_chains = kc;
+    if (t->mode) h->v5->two_pass = t->mode == 4;
     const int rc = h->v5->set_layout(t->group_size, t->lds_bytes, tok);
     if (rc) {
       h->v5->tok_group = prev_tg;
       h->v5->tok_chains = prev_kc;
+      h->v5->two_pass = prev_tp;
       return rc;
     }
-    if (t->mode) h->v5->two_pass = t->mode == 4;
     return STRIDE_OK;
   }
   return h->set.set_tune(t);

@@ -53,7 +53,9 @@ int main(int argc, char** argv) {
   }
   std::vector<PdltGroup> groups;
   uint32_t lds_bytes = 0;
-  rc = pdlt_layout(H.tab, gs, budget, groups, lds_bytes, err, chain32 ? 3u : 6u);
+  // the chain pass's blocks hold a refs slot only where pdlt_chain_block_refs keeps one
+  const bool slot = !two_pass || pdlt_chain_block_refs(chain32);
+  rc = pdlt_layout(H.tab, gs, budget, groups, lds_bytes, err, chain32 ? 3u : 6u, slot);
   if (rc) {
     std::fprintf(stderr, "layout (%d): %s\n", rc, err.c_str());
     return 2;
@@ -195,7 +197,9 @@ int main(int argc, char** argv) {
     }
   }
   // the kernels' block: three arrays of Tt per local (double for the float64
-  // chain, float for the float32 chain), then refs, walk, seen set
+  // chain, float for the float32 chain), then refs (the one-kernel decode;
+  // the chain pass reads them from the refs buffer and may keep the slot),
+  // walk, seen set
   auto run_groups = [&](auto tag) {
   using Tt = decltype(tag);
   constexpr uint32_t RW = 3 * sizeof(Tt) / 4;
@@ -211,7 +215,7 @@ int main(int argc, char** argv) {
       uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
       uint32_t* refs = b + RW * nl;
-      uint32_t* walk = refs + e.n_bnd;
+      uint32_t* walk = refs + (slot ? e.n_bnd : 0u);
       uint32_t* seen = walk + e.n_int;
       if (!two_pass) {
         pdlt_tokens(lds, rel, e, refs, walk, BitSeen{seen}, seen + 8, T.data() + 3ull * e.t_off);
@@ -219,8 +223,6 @@ int main(int argc, char** argv) {
       }
       for (uint32_t i = 0; i < e.n_int; ++i) walk[i] = walk_g.at(pdlt_walk_off(e.int_base, m, H.n_b) + i);
       for (uint32_t i = 0; i < kSeenWords; ++i) seen[i] = seen_g.at((size_t)kSeenWords * m + i);
-      // the chain pass gathers from the token pass's refs
-      for (uint32_t i = 0; i < e.n_bnd; ++i) refs[i] = refs_g.at((size_t)e.ref_base + i);
     }
     // phase B: boundary reconstructions
     for (uint32_t m = G.m0; m < G.m1; ++m) {
@@ -228,7 +230,8 @@ int main(int argc, char** argv) {
       uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
       Tt* recon = reinterpret_cast<Tt*>(b);
-      const uint32_t* refs = b + RW * nl;
+      // the chain pass gathers from the token pass's refs
+      const uint32_t* refs = two_pass ? refs_g.data() + e.ref_base : b + RW * nl;
       for (uint32_t i = 0; i < e.n_bnd; ++i)
         for (int d = 0; d < 3; ++d)
           recon[d * nl + i] = (Tt)pdlt_bnd(H.bnd.at(3ull * refs[i] + d), P);
@@ -239,7 +242,7 @@ int main(int argc, char** argv) {
       uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
       Tt* recon = reinterpret_cast<Tt*>(b);
-      const uint32_t* walk = b + RW * nl + e.n_bnd;
+      const uint32_t* walk = b + RW * nl + (slot ? e.n_bnd : 0u);
       const uint32_t n_emit = walk[e.n_int + 8];
       for (int d = 0; d < 3; ++d) pdlt_axis(lds, rel, e, d, P, recon + d * nl, walk, n_emit);
     }
@@ -249,7 +252,7 @@ int main(int argc, char** argv) {
       const uint32_t* b = blk(e);
       const uint32_t nl = e.n_bnd + e.n_int;
       const Tt* rj = reinterpret_cast<const Tt*>(b);
-      const uint32_t* sj = b + RW * nl + e.n_bnd + e.n_int;
+      const uint32_t* sj = b + RW * nl + (slot ? e.n_bnd : 0u) + e.n_int;
       for (uint32_t t = 0; t < 3 * e.n_int; ++t) {
         const uint32_t i = t / 3, d = t - 3 * i, v = e.n_bnd + i;
         const bool reached = ((sj[v >> 5] >> (v & 31)) & 1u) != 0;

@@ -64,9 +64,12 @@ def test_lane_logic_matches_oracle_on_goldens(pdlt_check, tmp_path, path):
     Vn, _ = PO.decode_v5(data, world=False)
     assert rc == 0 and same(V, T, Vn, To)
     for world in (True, False):
-        rc, V, T = run(pdlt_check, data, tmp_path, 7, 20480, world=world, chain32=True)
         V32, _ = PO.decode_v5(data, world=world, chain="f32")
-        assert rc == 0 and same(V, T, V32, To)
+        # the float32 chain pass's blocks hold no refs slot (two-pass)
+        for two_pass in (0, 1, 2):
+            rc, V, T = run(pdlt_check, data, tmp_path, 7, 20480, world=world, chain32=True,
+                           two_pass=two_pass)
+            assert rc == 0 and same(V, T, V32, To), (world, two_pass)
 
 
 def test_lane_logic_matches_oracle_on_fresh_streams(pdlt_check, tmp_path):
@@ -82,9 +85,11 @@ def test_lane_logic_matches_oracle_on_fresh_streams(pdlt_check, tmp_path):
         rc, Vl, Tl = run(pdlt_check, data, tmp_path)
         assert rc == 0
         assert same(Vl, Tl, *PO.decode_v5(data))
-        rc, Vl, Tl = run(pdlt_check, data, tmp_path, chain32=True)
-        assert rc == 0
-        assert same(Vl, Tl, *PO.decode_v5(data, chain="f32"))
+        V32 = PO.decode_v5(data, chain="f32")
+        for two_pass in (0, 2):
+            rc, Vl, Tl = run(pdlt_check, data, tmp_path, chain32=True, two_pass=two_pass)
+            assert rc == 0
+            assert same(Vl, Tl, *V32), two_pass
         rc, Vl, Tl = run(pdlt_check, data, tmp_path, 7, 12288, two_pass=1)
         assert rc == 0
         assert same(Vl, Tl, *PO.decode_v5(data))
