@@ -154,7 +154,8 @@ STRIDE_DEV uint32_t pdlt_block_words(uint32_t n_bnd, uint32_t n_int, uint32_t rw
 // the refs from HBM and never uses it).  The float32 chain drops it: 0.220 ->
 // 0.209 ms on the 1900 x 1900 heightfield.  The float64 chain keeps it: without
 // it every LDS budget measured about 2% slower (0.2525 -> 0.258 ms in groups of
-// 16 in 20 KB), and group-size caps did not recover that.
+// 16 in 20 KB), and so did the same words at the block's end (the same group
+// packing) or a slot twice as long: the walk's position matters there.
 constexpr bool pdlt_chain_block_refs(bool chain32) { return !chain32; }
 
 STRIDE_DEV uint32_t pdlt_idx_bits(uint32_t n_local) {
