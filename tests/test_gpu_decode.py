@@ -164,6 +164,46 @@ def test_decoder_errors():
         ParaDeltaV5GpuDecoder(data, meshlets_per_block=0)
 
 
+def test_decoders_created_and_used_from_many_threads():
+    """Creating decoders is thread safe (a handle is used by one thread at a
+    time): eight threads create and decode their own decoders at once, DPR4
+    and PDLT v5 streams alike, and every output keeps the golden digests."""
+    import threading
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    gold = []
+    for p in stream_fixtures():
+        with np.load(p) as z:
+            gold.append((p.stem, z["data"].tobytes(), z["V_sha"].tobytes().hex(),
+                         z["T_sha"].tobytes().hex()))
+    assert any(g[0].startswith("pdlt_") for g in gold) and len(gold) >= 4
+    ParaDeltaV5GpuDecoder(gold[0][1]).decode_to_host()   # library and device ready
+    n_threads, reps = 8, 2
+    start = threading.Barrier(n_threads)
+    got, errors = [], []
+
+    def work(t):
+        try:
+            start.wait()
+            for _ in range(reps):
+                for i in range(len(gold)):
+                    k = (i + t) % len(gold)
+                    V, T = ParaDeltaV5GpuDecoder(gold[k][1]).decode_to_host()
+                    got.append((k, sha(V), sha(T.astype(np.int64))))
+        except Exception as e:   # reported after the join
+            errors.append((t, repr(e)))
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(n_threads)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=120)
+    assert not any(th.is_alive() for th in threads)
+    assert not errors, errors[:3]
+    assert len(got) == n_threads * reps * len(gold)
+    for k, vs, ts in got:
+        assert (vs, ts) == (gold[k][2], gold[k][3]), gold[k][0]
+
+
 def test_corrupted_streams_agree_with_oracle(coracle):
     """Bit flips in the body: when the oracle rejects a stream the GPU
     decoder raises at construction; when both accept, outputs match."""

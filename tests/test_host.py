@@ -178,6 +178,38 @@ def test_decoder_rejects_bad_headers_before_device_use():
         _lib.check(rc)
 
 
+def test_error_messages_are_per_thread():
+    """The last error is per thread: eight threads reject their own bad
+    headers at once (ctypes releases the GIL around each call) and each
+    reads back its own message."""
+    import ctypes as C
+    import threading
+    from meshpress_amd import _lib
+    L = _lib.lib()
+    n_threads, reps = 8, 300
+    start = threading.Barrier(n_threads)
+    wrong = []
+
+    def work(t):
+        magic = 0x10203040 + 0x01010101 * t
+        arr = np.frombuffer(magic.to_bytes(4, "big") + b"\x03" + b"\x00" * 95, np.uint8)
+        want = "bad magic 0x%08X" % magic
+        h = C.c_void_p()
+        start.wait()
+        for _ in range(reps):
+            rc = L.stride_decoder_create(_lib.ptr(arr), len(arr), C.byref(h))
+            msg = _lib.last_error()
+            if rc != _lib.E_MAGIC or msg != want:
+                wrong.append((t, rc, msg))
+
+    threads = [threading.Thread(target=work, args=(t,)) for t in range(n_threads)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    assert not wrong, wrong[:5]
+
+
 def test_encoder_argument_errors():
     from meshpress_amd.encoder import STRIDEEncoder
     from meshpress_amd.encoder.prep import prepare
