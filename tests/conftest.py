@@ -52,7 +52,22 @@ def small_meshes():
     # two disjoint triangles (two strips, shared nothing)
     out.append(("two_tris", np.array([[0.0, 0, 0], [1, 0, 0], [0, 1, 0], [3, 3, 3], [4, 3, 3],
                                       [3, 4, 3.5]]), np.array([[0, 1, 2], [3, 4, 5]])))
+    # a two-row ribbon whose meshlets reach the maximum of 256 locals
+    # (local id 255, 9-bit raw ids)
+    out.append(("ribbon_2x1000",) + ribbon(1000))
     return out
+
+
+def ribbon(n, seed=0):
+    """A strip of 2 x n vertices (2 (n - 1) triangles) with a little z noise:
+    every vertex lies on the mesh border, so each meshlet of max_verts=256 and
+    max_tris=256 holds 256 vertices and 254 triangles."""
+    rng = np.random.default_rng(seed)
+    x, y = np.meshgrid(np.linspace(0.0, 10.0, n), np.array([0.0, 0.1]))
+    V = np.stack([x.ravel(), y.ravel(), 0.05 * rng.standard_normal(x.size)], 1)
+    c = np.arange(n - 1)
+    T = np.concatenate([np.stack([c, c + 1, c + n], 1), np.stack([c + 1, c + n + 1, c + n], 1)])
+    return V, T.astype(np.int64)
 
 
 def encode_with_oracle(V, T, *, pred="canonical", precision_error=0.0005,
