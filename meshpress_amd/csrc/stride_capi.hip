@@ -756,13 +756,16 @@ struct DecodeSet {
     return STRIDE_OK;
   }
 
+  // an output may be null only when it has no elements (an empty stream)
+  bool outputs_ok(const float* V, const uint32_t* T) const { return (V || !n_v) && (T || !n_t); }
+
   int decode(float* V, uint32_t* T, hipStream_t s) {
-    if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
+    if (!outputs_ok(V, T)) return fail(STRIDE_E_ARG, "null output pointer");
     return enqueue(V, T, s, 0, (int)mp.size());
   }
 
   int decode_per_mesh(float* V, uint32_t* T, hipStream_t s) {
-    if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
+    if (!outputs_ok(V, T)) return fail(STRIDE_E_ARG, "null output pointer");
     for (int i = 0; i < (int)mp.size(); ++i) {
       int rc = enqueue(V, T, s, i, i + 1);
       if (rc) return rc;
@@ -771,7 +774,7 @@ struct DecodeSet {
   }
 
   int decode_graph(float* V, uint32_t* T, hipStream_t s) {
-    if (!V || !T) return fail(STRIDE_E_ARG, "null output pointer");
+    if (!outputs_ok(V, T)) return fail(STRIDE_E_ARG, "null output pointer");
     auto key = std::make_tuple((void*)V, (void*)T, cfg.mode, cfg.waves, cfg.stage_words,
                                cfg.grid, cfg.use_queue, cfg.group_size,
                                (cfg.lds_bytes * 128 + cfg.tok_group) * 32 + cfg.tok_chains);
@@ -1024,7 +1027,7 @@ int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* str
 int stride_decode_noalloc(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream) {
   if (!h) return fail(STRIDE_E_ARG, "null handle");
   if (h->v5) return fail(STRIDE_E_ARG, "stride_decode_noalloc decodes DPR4 streams only");
-  if (!d_verts || !d_tris) return fail(STRIDE_E_ARG, "null output pointer");
+  if (!h->set.outputs_ok(d_verts, d_tris)) return fail(STRIDE_E_ARG, "null output pointer");
   DecodeSet& S = h->set;
   int dev = -1;
   HIP_TRY(hipGetDevice(&dev));

@@ -260,6 +260,44 @@ def test_batch_decode_matches_single(coracle):
                 assert_same(Vi, Ti, Vo, To)
 
 
+def empty_stream():
+    """A DPR4 v3 stream of no meshlets: the small hand stream's 86-byte
+    header with n_v_dup, n_t and n_meshlets set to 0, and nothing after it."""
+    from tests.test_oracle import hand_stream_small
+    h = bytearray(hand_stream_small()[0][:86])
+    h[74:86] = bytes(12)
+    return bytes(h)
+
+
+def test_empty_streams(coracle):
+    """A stream of no meshlets decodes to empty outputs, alone and inside a
+    batch (first, between and last), in every batch launch form."""
+    import torch
+    from meshpress_amd.batch import BatchDecoder
+    from meshpress_amd.utils.paradelta_v5_cuda import ParaDeltaV5GpuDecoder
+    from tests.test_oracle import hand_stream_fifo, hand_stream_small
+    empty = empty_stream()
+    Vo, To = coracle.decode(empty)
+    assert Vo.shape == (0, 3) and To.shape == (0, 3)
+    dec = ParaDeltaV5GpuDecoder(empty)
+    assert (dec.n_meshlets, dec.n_v_total, dec.n_t_total) == (0, 0, 0)
+    for _ in range(2):
+        assert_same(*dec.decode_to_host(), Vo, To)
+    blobs = [empty, hand_stream_small()[0], empty, hand_stream_fifo()[0], empty]
+    want = [coracle.decode(b) for b in blobs]
+    bd = BatchDecoder(blobs)
+    for tune in (dict(), dict(mode=4, group_size=16, lds_bytes=16384, tok_chains=0),
+                 dict(mode=0, waves=4, stage_words=512, grid=128, use_queue=1)):
+        if tune:
+            bd.set_tune(**tune)
+        for mode in ("launch", "per_mesh", "graph", "graph"):
+            v, t = bd.decode(mode)
+            torch.cuda.synchronize()
+            vh, th = v.cpu().numpy(), t.cpu().numpy().view(np.uint32)
+            for (Vi, Ti), (Vw, Tw) in zip(bd.split(vh, th), want):
+                assert_same(Vi, Ti, Vw, Tw)
+
+
 @pytest.mark.parametrize("bits", [12, 16])
 def test_icosphere_l8_full_size(coracle, bits):
     """The 1,310,720-triangle icosphere at full size: the GPU decode equals
