@@ -109,7 +109,9 @@ int stride_decoder_create_sblb(const uint8_t* blob, size_t len, stride_decoder**
 int stride_decoder_sizes(const stride_decoder* h, uint64_t* n_v, uint64_t* n_t,
                          uint32_t* n_meshlets);
 /* Decoded world-space positions (float32 xyz, n_v x 3) and uint32 indices
- * (n_t x 3) into caller-owned device buffers; asynchronous on `stream`. */
+ * (n_t x 3) into caller-owned device buffers; asynchronous on `stream`.  An
+ * output may be NULL only when it has no elements (a stream of no meshlets);
+ * otherwise STRIDE_E_ARG. */
 int stride_decode(stride_decoder* h, float* d_verts, uint32_t* d_tris, void* stream);
 /* Synchronous decode into caller-owned host buffers. */
 int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris);
