@@ -242,6 +242,18 @@ def test_hand_streams(oracle, coracle, make):
     assert np.array_equal(q[1], np.float32([2, 3, 4]))
 
 
+def test_empty_stream_decodes_empty(oracle, coracle):
+    """A header with n_meshlets = 0 decodes to empty (0, 3) outputs in both
+    restatements, as in the GPU decoder.  The reference's decode_dup raises
+    ValueError there instead (np.concatenate of no arrays,
+    paradelta_v5_dup.py:608), a by-product of its stacking, not a check."""
+    h = bytearray(hand_stream_small()[0][:86])
+    h[74:86] = bytes(12)
+    for dec in (oracle.decode_dup, coracle.decode):
+        V, T = dec(bytes(h))
+        assert V.shape == (0, 3) and T.shape == (0, 3) and V.dtype == np.float32
+
+
 def test_decode_errors(oracle):
     data, _, _ = hand_stream_small()
     with pytest.raises(ValueError, match="bad magic"):
