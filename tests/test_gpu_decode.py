@@ -59,11 +59,21 @@ SHAPES = [dict(), dict(mode=2, group_size=16, lds_bytes=24576), dict(mode=4, tok
 SHAPE_IDS = ["default", "split", "chains2", "chains16", "token_lanes"]
 
 
-@pytest.mark.parametrize("shape", SHAPES, ids=SHAPE_IDS)
-@pytest.mark.parametrize("path", stream_fixtures(), ids=lambda p: p.stem)
+def golden_cases():
+    """(path, shape) pairs: DPR4 streams in every launch shape, PDLT v5
+    streams in the default one (their launch shapes are covered by
+    tests/test_gpu_pdlt.py)."""
+    out = []
+    for p in stream_fixtures():
+        for shape, sid in zip(SHAPES, SHAPE_IDS):
+            if shape and p.stem.startswith("pdlt_"):
+                continue
+            out.append(pytest.param(p, shape, id=f"{p.stem}-{sid}"))
+    return out
+
+
+@pytest.mark.parametrize("path,shape", golden_cases())
 def test_golden_streams(path, shape):
-    if shape and path.stem.startswith("pdlt_"):
-        pytest.skip("PDLT v5 streams have their own launch shapes (tests/test_gpu_pdlt.py)")
     z = np.load(path)
     V, T = gpu_decode(z["data"].tobytes(), **shape)
     assert sha(V) == z["V_sha"].tobytes().hex()
