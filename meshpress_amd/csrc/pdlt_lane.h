@@ -140,9 +140,10 @@ enum : uint32_t {
 };
 
 // LDS block of one meshlet, in 32-bit words: reconstruction (3 axes x
-// n_local float64, axis major), refs (n_bnd; only where the block's own
-// lanes decode them, refs = true), walk (n_int), the seen set (8 words) and
-// the walk length (1 word); even, so blocks stay 8-byte aligned.
+// n_local float64, axis major), refs (n_bnd words when refs = true: the
+// one-kernel decode's lanes decode them there, the float64 chain pass keeps
+// the slot unused, pdlt_chain_block_refs), walk (n_int), the seen set (8
+// words) and the walk length (1 word); even, so blocks stay 8-byte aligned.
 // rw = words of reconstruction per local: 6 (three float64, the float64
 // chain) or 3 (three float32, the float32 chain)
 STRIDE_DEV uint32_t pdlt_block_words(uint32_t n_bnd, uint32_t n_int, uint32_t rw = 6,
