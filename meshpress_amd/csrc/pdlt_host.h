@@ -220,8 +220,10 @@ inline int pdlt_parse(const uint8_t* d, size_t len, PdltHost& H, std::string& er
 // Greedy wave groups: up to gs consecutive meshlets whose staged words and
 // LDS blocks fit `budget` bytes (a meshlet alone always forms a group).
 // Sets each entry's blk_off; lds = the largest group's LDS bytes.  refs:
-// the blocks hold the meshlets' boundary refs (the one-kernel decode); the
-// chain pass of the two-pass decode reads them from HBM instead.
+// the blocks hold a slot of n_bnd words for the boundary refs (the
+// one-kernel decode decodes them there; the chain pass of the two-pass
+// decode reads them from HBM and keeps the slot only where
+// pdlt_chain_block_refs says so).
 inline int pdlt_layout(std::vector<PdltEntry>& tab, int gs, int budget, std::vector<PdltGroup>& groups,
                        uint32_t& lds, std::string& err, uint32_t rw = 6, bool refs = true) {
   if (gs < 1 || gs > 16) { err = "group_size must be in [1, 16]"; return STRIDE_E_ARG; }
