@@ -1101,8 +1101,8 @@ int stride_decode_to_host(stride_decoder* h, float* verts, uint32_t* tris) {
   DecodeSet& s = h->set;
   int rc = s.decode(s.d_v_own, s.d_t_own, nullptr);
   if (rc) return rc;
-  HIP_TRY(hipMemcpy(verts, s.d_v_own, 12 * s.n_v, hipMemcpyDeviceToHost));
-  HIP_TRY(hipMemcpy(tris, s.d_t_own, 12 * s.n_t, hipMemcpyDeviceToHost));
+  if (s.n_v) HIP_TRY(hipMemcpy(verts, s.d_v_own, 12 * s.n_v, hipMemcpyDeviceToHost));
+  if (s.n_t) HIP_TRY(hipMemcpy(tris, s.d_t_own, 12 * s.n_t, hipMemcpyDeviceToHost));
   return STRIDE_OK;
 }
 
