@@ -308,6 +308,40 @@ def test_empty_streams(coracle):
                 assert_same(Vi, Ti, Vw, Tw)
 
 
+def test_c_abi_decode_to_host():
+    """stride_decode_to_host, the C ABI's synchronous decode into host
+    buffers: a DPR4 and a PDLT v5 golden stream keep their digests, and a
+    stream of no meshlets takes null outputs."""
+    import ctypes as C
+    from meshpress_amd import _lib
+    L = _lib.lib()
+
+    def decode(data, outputs):
+        arr = np.frombuffer(data, np.uint8)
+        h = C.c_void_p()
+        _lib.check(L.stride_decoder_create(_lib.ptr(arr), len(arr), C.byref(h)))
+        try:
+            nv, nt, nm = C.c_uint64(), C.c_uint64(), C.c_uint32()
+            _lib.check(L.stride_decoder_sizes(h, C.byref(nv), C.byref(nt), C.byref(nm)))
+            V = np.empty((nv.value, 3), np.float32)
+            T = np.empty((nt.value, 3), np.uint32)
+            if outputs:
+                _lib.check(L.stride_decode_to_host(h, _lib.ptr(V), _lib.ptr(T)))
+            else:
+                _lib.check(L.stride_decode_to_host(h, None, None))
+            return V, T
+        finally:
+            L.stride_decoder_destroy(h)
+
+    for name in ("dup_hf_24", "pdlt_hf_24"):
+        z = np.load(GOLD / f"{name}.npz")
+        V, T = decode(z["data"].tobytes(), True)
+        assert sha(V) == z["V_sha"].tobytes().hex(), name
+        assert sha(T.astype(np.int64)) == z["T_sha"].tobytes().hex(), name
+    V, T = decode(empty_stream(), False)
+    assert V.shape == (0, 3) and T.shape == (0, 3)
+
+
 @pytest.mark.parametrize("bits", [12, 16])
 def test_icosphere_l8_full_size(coracle, bits):
     """The 1,310,720-triangle icosphere at full size: the GPU decode equals
